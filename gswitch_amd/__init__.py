@@ -1,0 +1,313 @@
+"""Python binding of the MI355X traversal engine (ctypes over the C-ABI).
+
+The engine itself is C++/HIP (gswitch_amd/csrc, C-ABI in
+include/gswitch_amd.h); this module only loads libgswitch_amd.so and turns
+numpy arrays into pointers. There is no fallback: if the library is missing
+the import fails.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgswitch_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "gswitch_amd.h")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        "libgswitch_amd.so is not built; run `python -m gswitch_amd.build` "
+        "(or __graft_entry__.build())")
+
+_lib = C.CDLL(LIB_PATH)
+
+OK = 0
+STATUS = {0: "OK", 1: "INVALID", 2: "HIP", 3: "ALLOC", 4: "OVERFLOW",
+          5: "UNSUPPORTED", 6: "INTERNAL"}
+LB_NAMES = ["TM", "WM", "CM", "STRICT", "TWOD", "ELB", "TWC"]
+
+
+class GswError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (STATUS.get(code, code), msg))
+        self.code = code
+
+
+class RunOpts(C.Structure):
+    _fields_ = [("configs", C.c_char_p), ("max_levels", C.c_int32),
+                ("verbose", C.c_int32), ("twc_small", C.c_int32),
+                ("twc_large", C.c_int32), ("profile", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("device_ms", C.c_double), ("levels", C.c_int32),
+                ("n_records", C.c_int32), ("reached", C.c_int64)]
+
+
+class LevelRecord(C.Structure):
+    _fields_ = [("level", C.c_int32), ("dir", C.c_int8), ("fmt", C.c_int8),
+                ("lb", C.c_int8), ("fused", C.c_int8), ("sorted", C.c_int8),
+                ("pad", C.c_int8 * 3), ("frontier", C.c_int64),
+                ("active", C.c_int64), ("inactive", C.c_int64),
+                ("edges", C.c_int64), ("verts", C.c_int64), ("ms", C.c_float),
+                ("pad2", C.c_float)]
+
+
+_P = C.c_void_p
+_SIGS = {
+    "gsw_version": (C.c_char_p, []),
+    "gsw_last_error": (C.c_char_p, []),
+    "gsw_device_count": (C.c_int32, []),
+    "gsw_run_opts_init": (None, [C.POINTER(RunOpts)]),
+    "gsw_host_graph_from_edges": (C.c_int32, [C.c_int64, C.c_int64, _P, _P, _P,
+                                              C.c_int32, C.POINTER(_P)]),
+    "gsw_gen_rmat": (C.c_int32, [C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                 C.c_double, C.c_uint64, C.c_int32, C.POINTER(_P)]),
+    "gsw_gen_grid2d": (C.c_int32, [C.c_int64, C.c_int64, C.c_double, C.c_double,
+                                   C.c_uint64, C.c_int32, C.POINTER(_P)]),
+    "gsw_gen_er": (C.c_int32, [C.c_int64, C.c_double, C.c_uint64, C.c_int32,
+                               C.POINTER(_P)]),
+    "gsw_gen_ba": (C.c_int32, [C.c_int64, C.c_int32, C.c_uint64, C.c_int32,
+                               C.POINTER(_P)]),
+    "gsw_host_graph_info": (C.c_int32, [_P, C.POINTER(C.c_int64),
+                                        C.POINTER(C.c_int64),
+                                        C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_int32)]),
+    "gsw_host_graph_export": (C.c_int32, [_P, _P, _P, _P]),
+    "gsw_host_graph_destroy": (None, [_P]),
+    "gsw_degree_stats": (C.c_int32, [_P, C.c_int64, _P]),
+    "gsw_graph_create_csr": (C.c_int32, [_P, _P, _P, C.c_int64, C.c_int64,
+                                         C.c_int32, C.POINTER(_P)]),
+    "gsw_graph_from_host": (C.c_int32, [_P, C.POINTER(_P)]),
+    "gsw_graph_destroy": (C.c_int32, [_P]),
+    "gsw_bfs": (C.c_int32, [_P, C.c_int32, C.POINTER(RunOpts), _P,
+                            C.POINTER(RunStats)]),
+    "gsw_sssp": (C.c_int32, [_P, C.c_int32, C.POINTER(RunOpts), _P,
+                             C.POINTER(RunStats)]),
+    "gsw_cc": (C.c_int32, [_P, C.POINTER(RunOpts), _P, C.POINTER(RunStats)]),
+    "gsw_pagerank": (C.c_int32, [_P, C.c_float, C.c_int32, C.c_float,
+                                 C.POINTER(RunOpts), _P, C.POINTER(RunStats)]),
+    "gsw_last_records": (C.c_int32, [_P, _P, C.c_int32, C.POINTER(C.c_int32)]),
+    "gsw_tree_eval": (C.c_int32, [C.c_char_p, _P, C.c_int32,
+                                  C.POINTER(C.c_int32)]),
+    "gsw_scan_exclusive_i64": (C.c_int32, [_P, C.c_int64, _P]),
+    "gsw_wave_scan64": (C.c_int32, [_P, _P]),
+}
+for _name, (_res, _args) in _SIGS.items():
+    _fn = getattr(_lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def _check(rc):
+    if rc != OK:
+        raise GswError(rc, _lib.gsw_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+def version():
+    return _lib.gsw_version().decode()
+
+
+def device_count():
+    return _lib.gsw_device_count()
+
+
+# --------------------------------------------------------------- host graphs
+class HostGraph:
+    """Host CSR built by the library (generators, edge-list ingest)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        n, m = C.c_int64(), C.c_int64()
+        d, w = C.c_int32(), C.c_int32()
+        _check(_lib.gsw_host_graph_info(self._h, C.byref(n), C.byref(m),
+                                        C.byref(d), C.byref(w)))
+        self.n, self.m = n.value, m.value
+        self.directed, self.weighted = bool(d.value), bool(w.value)
+        self.row_ptr = np.empty(self.n + 1, np.int64)
+        self.col = np.empty(max(self.m, 1), np.int32)[: self.m]
+        self.w = np.empty(max(self.m, 1), np.uint32)[: self.m] if self.weighted else None
+        _check(_lib.gsw_host_graph_export(self._h, _ptr(self.row_ptr),
+                                          _ptr(self.col), _ptr(self.w)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.gsw_host_graph_destroy(self._h)
+            self._h = None
+
+    @property
+    def degrees(self):
+        return np.diff(self.row_ptr)
+
+    def max_degree_vertex(self):
+        """Highest-degree vertex, ties broken by the smallest id."""
+        return int(np.argmax(self.degrees))
+
+
+def _gen(fn, *args):
+    h = _P()
+    _check(fn(*args, C.byref(h)))
+    return HostGraph(h)
+
+
+def rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, weighted=False):
+    return _gen(_lib.gsw_gen_rmat, scale, edge_factor, a, b, c, seed, int(weighted))
+
+
+def grid2d(rows, cols, p_delete=0.0, long_range_frac=0.0, seed=3, weighted=False):
+    return _gen(_lib.gsw_gen_grid2d, rows, cols, p_delete, long_range_frac, seed,
+                int(weighted))
+
+
+def erdos_renyi(n, mean_degree, seed=5, weighted=False):
+    return _gen(_lib.gsw_gen_er, n, mean_degree, seed, int(weighted))
+
+
+def barabasi_albert(n, m, seed=6, weighted=False):
+    return _gen(_lib.gsw_gen_ba, n, m, seed, int(weighted))
+
+
+def from_edges(n, src, dst, w=None, directed=False):
+    src = np.ascontiguousarray(src, np.int32)
+    dst = np.ascontiguousarray(dst, np.int32)
+    wv = None if w is None else np.ascontiguousarray(w, np.uint32)
+    h = _P()
+    _check(_lib.gsw_host_graph_from_edges(n, len(src), _ptr(src), _ptr(dst),
+                                          _ptr(wv), int(directed), C.byref(h)))
+    return HostGraph(h)
+
+
+def degree_stats(row_ptr):
+    row_ptr = np.ascontiguousarray(row_ptr, np.int64)
+    out = np.zeros(6)
+    _check(_lib.gsw_degree_stats(_ptr(row_ptr), len(row_ptr) - 1, _ptr(out)))
+    return dict(zip(["avg", "std", "max", "range", "GI", "Her"], out))
+
+
+def tree_eval(name, fv):
+    fv = np.ascontiguousarray(fv, np.float64)
+    cls = C.c_int32()
+    _check(_lib.gsw_tree_eval(name.encode(), _ptr(fv), len(fv), C.byref(cls)))
+    return cls.value
+
+
+# --------------------------------------------------------------- device graphs
+def run_opts(configs=None, max_levels=0, verbose=False, twc_small=0,
+             twc_large=0, profile=True):
+    o = RunOpts()
+    _lib.gsw_run_opts_init(C.byref(o))
+    o.configs = configs.encode() if configs else None
+    if max_levels:
+        o.max_levels = max_levels
+    o.verbose = int(verbose)
+    if twc_small:
+        o.twc_small = twc_small
+    if twc_large:
+        o.twc_large = twc_large
+    o.profile = int(profile)
+    return o
+
+
+class Graph:
+    """Device-resident graph; runs reuse its frontier and vertex arrays."""
+
+    def __init__(self, host=None, row_ptr=None, col=None, w=None):
+        h = _P()
+        if host is not None:
+            _check(_lib.gsw_graph_from_host(host._h, C.byref(h)))
+            self.n, self.m = host.n, host.m
+        else:
+            row_ptr = np.ascontiguousarray(row_ptr, np.int64)
+            col = np.ascontiguousarray(col, np.int32)
+            wv = None if w is None else np.ascontiguousarray(w, np.uint32)
+            self.n, self.m = len(row_ptr) - 1, len(col)
+            _check(_lib.gsw_graph_create_csr(_ptr(row_ptr), _ptr(col), _ptr(wv),
+                                             self.n, self.m, 0, C.byref(h)))
+        self._h = h
+        self.last_stats = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _check(_lib.gsw_graph_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _opts(self, configs, kw):
+        return run_opts(configs=configs, **kw)
+
+    def bfs(self, root, configs=None, out=True, **kw):
+        o = self._opts(configs, kw)
+        st = RunStats()
+        lv = np.empty(self.n, np.int32) if out else None
+        _check(_lib.gsw_bfs(self._h, int(root), C.byref(o), _ptr(lv), C.byref(st)))
+        self.last_stats = st
+        return lv
+
+    def sssp(self, root, configs=None, out=True, **kw):
+        o = self._opts(configs, kw)
+        st = RunStats()
+        d = np.empty(self.n, np.uint32) if out else None
+        _check(_lib.gsw_sssp(self._h, int(root), C.byref(o), _ptr(d), C.byref(st)))
+        self.last_stats = st
+        return d
+
+    def cc(self, configs=None, out=True, **kw):
+        o = self._opts(configs, kw)
+        st = RunStats()
+        lab = np.empty(self.n, np.int32) if out else None
+        _check(_lib.gsw_cc(self._h, C.byref(o), _ptr(lab), C.byref(st)))
+        self.last_stats = st
+        return lab
+
+    def pagerank(self, damping=0.85, iters=50, eps=0.0, configs=None, out=True, **kw):
+        o = self._opts(configs, kw)
+        st = RunStats()
+        pr = np.empty(self.n, np.float32) if out else None
+        _check(_lib.gsw_pagerank(self._h, damping, iters, eps, C.byref(o), _ptr(pr),
+                                 C.byref(st)))
+        self.last_stats = st
+        return pr
+
+    def records(self):
+        cap = 1 << 16
+        buf = (LevelRecord * cap)()
+        n = C.c_int32()
+        _check(_lib.gsw_last_records(self._h, buf, cap, C.byref(n)))
+        out = []
+        for r in buf[: n.value]:
+            out.append(dict(level=r.level, dir="Pull" if r.dir else "Push",
+                            fmt="Bitmap" if r.fmt else "Queue",
+                            lb=LB_NAMES[r.lb], fused=bool(r.fused),
+                            sorted=bool(r.sorted), frontier=r.frontier,
+                            active=r.active, inactive=r.inactive, edges=r.edges,
+                            verts=r.verts, ms=r.ms))
+        return out
+
+
+def scan_exclusive_i64(x):
+    x = np.ascontiguousarray(x, np.int64)
+    out = np.empty(len(x) + 1, np.int64)
+    _check(_lib.gsw_scan_exclusive_i64(_ptr(x), len(x), _ptr(out)))
+    return out
+
+
+def wave_scan64(x):
+    x = np.ascontiguousarray(x, np.int32)
+    assert len(x) == 64
+    out = np.empty(64, np.int32)
+    _check(_lib.gsw_wave_scan64(_ptr(x), _ptr(out)))
+    return out

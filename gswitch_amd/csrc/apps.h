@@ -1,0 +1,164 @@
+// Shipped functors and level-loop drivers behind the C-ABI.
+//
+// Each functor restates the reference application's operator and each
+// driver its run_* loop:
+//   BFS   application/bfs.cu:16-36, 78-102   (levels, -1 unreached)
+//   SSSP  application/sssp.cu:17-53, 106-126 (uint32 distances; the
+//         reference's int / INT_MAX become uint32 / UINT32_MAX)
+//   CC    application/cc.cu:15-32, 82-108    (edge-centric hooking + pointer
+//         jumping; label = smallest vertex id of the component)
+//   PR    application/pagerank.cu:18-42, 81-103 (un-normalised form; the
+//         accumulator is FP64 and the rank stored per out-degree is FP32;
+//         eps = 0 with an iteration cap gives a fixed iteration count)
+#pragma once
+#include "gswitch/gswitch.h"
+
+namespace gsw_apps {
+
+using GU = device_graph_t<CSR, Empty>;
+using GW = device_graph_t<CSR, uint32_t>;
+using GC = device_graph_t<COO, Empty>;
+
+struct BFS : Functor<VC, int, Empty, Empty> {
+  __device__ Status filter(int v, GU g) {
+    const int lvl = *wa_of(v);
+    if (lvl == g.get_level()) return Active;
+    if (lvl < 0) return Inactive;
+    return Fixed;
+  }
+  __device__ int emit(int v, Empty *e, GU g) { return g.get_level(); }
+  __device__ bool cond(int v, int msg, GU g) { return *wa_of(v) == -1; }
+  __device__ bool comp(int *v, int msg, GU g) {
+    *v = msg;
+    return true;
+  }
+  __device__ bool compAtomic(int *v, int msg, GU g) {
+    *v = msg;
+    return true;
+  }
+};
+
+struct SSSP : Functor<VC, uint32_t, uint32_t, uint32_t> {
+  __device__ Status filter(int v, GW g) {
+    uint32_t *nd = wa_of(v);
+    uint32_t *od = ra_of(v);
+    if (*nd == *od) return Inactive;
+    if (data.window.in((double)*nd)) {
+      *od = *nd;
+      return Active;
+    }
+    return Inactive;
+  }
+  __device__ uint32_t emit(int v, uint32_t *w, GW g) { return *wa_of(v) + *w; }
+  __device__ bool comp(uint32_t *d, uint32_t msg, GW g) {
+    if (*d > msg) {
+      *d = msg;
+      return data.window.in((double)msg);
+    }
+    return false;
+  }
+  __device__ bool compAtomic(uint32_t *d, uint32_t msg, GW g) {
+    const uint32_t old = atomicMin(d, msg);
+    return old > msg && data.window.in((double)msg);
+  }
+  __device__ bool cond(int v, uint32_t msg, GW g) { return *wa_of(v) > msg; }
+  __device__ bool exit(int v, GW g) { return *wa_of(v) != *ra_of(v); }
+};
+
+struct CC : Functor<EC, int, Empty, Empty> {
+  __device__ Status filter(int v, int u, Empty *e) {
+    return (*wa_of(v) == *wa_of(u)) ? Fixed : Active;
+  }
+  __device__ void update(int v, int u, Empty *e) {
+    int *rv = wa_of(*wa_of(v));
+    int *ru = wa_of(*wa_of(u));
+    const int a = *rv, b = *ru;
+    if (a < b)
+      atomicMin(ru, a);
+    else if (a > b)
+      atomicMin(rv, b);
+  }
+};
+
+struct PR : Functor<VC, double, float, Empty> {
+  float damp;
+  float eps;
+  __device__ Status filter(int v, GU g) {
+    double *cache = wa_of(v);
+    float *pr = ra_of(v);
+    const int od = g.get_out_degree(v);
+    double np = *cache * (double)damp + (1.0 - (double)damp);
+    if (od) np /= od;
+    const float err = (float)((np - (double)*pr) / (double)*pr);
+    *cache = 0;
+    *pr = (float)np;
+    if (err > -eps && err < eps) return Inactive;
+    return Active;
+  }
+  __device__ double emit(int v, Empty *e, GU g) { return (double)*ra_of(v); }
+  __device__ bool comp(double *v, double msg, GU g) {
+    *v += msg;
+    return true;
+  }
+  __device__ bool compAtomic(double *v, double msg, GU g) {
+    atomicAdd(v, msg);
+    return true;
+  }
+};
+
+// ---- per-run engine state ------------------------------------------------
+struct engine_t {
+  inspector_t inspector;
+  selector_t selector;
+  executor_t executor;
+  feature_t fets;
+  config_t conf;
+  stat_t stats;
+  engine_t() { executor.bind(inspector); }
+  void release() {
+    stats.release();
+    executor.release();
+  }
+};
+
+struct run_opts_t {
+  ins_t force;
+  int max_levels = 1 << 20;
+  bool verbose = false;
+  bool sorted = false;
+  int twc_small = 32, twc_large = 2048;
+};
+
+inline void apply_opts(config_t &conf, const run_opts_t &o) {
+  conf.force = o.force;
+  conf.conf_sorted = o.sorted;
+  conf.twc_small = o.twc_small;
+  conf.twc_large = o.twc_large;
+}
+
+// Generic vertex-centric level loop (application/bfs.cu:87-98 shape).
+// `stop_after_inspect(level)` lets PageRank end on a fold pass.
+template <typename G, typename F, typename Stop>
+int run_vc(engine_t &en, active_set_t &as, G g, F &f, const run_opts_t &o,
+           Stop stop_after_inspect) {
+  int level = 0;
+  for (level = 0; level < o.max_levels; ++level) {
+    en.inspector.inspect(as, g, f, en.stats, en.fets, en.conf);
+    if (as.finish(g, f, en.conf)) break;
+    if (stop_after_inspect(level)) break;
+    en.selector.select(en.stats, en.fets, en.conf);
+    en.executor.filter(as, g, f, en.stats, en.fets, en.conf);
+    g.update_level();
+    en.executor.expand(as, g, f, en.stats, en.fets, en.conf);
+    if (o.verbose) {
+      const auto &r = en.executor.log.back();
+      fprintf(stderr, "level %d frontier %lld ", r.level, r.frontier);
+      en.conf.show_hints(std::cerr);
+      fprintf(stderr, "\n");
+    }
+    if (as.finish(g, f, en.conf)) break;
+  }
+  return level;
+}
+
+}  // namespace gsw_apps

@@ -1,0 +1,747 @@
+// C-ABI of the traversal engine (declarations: include/gswitch_amd.h).
+#include "../../include/gswitch_amd.h"
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "apps.h"
+#include "host/host_graph.h"
+
+using namespace gsw_apps;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+gsw_status fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+template <typename Fn> gsw_status guarded(Fn fn) {
+  try {
+    g_last_error.clear();
+    fn();
+    return GSW_OK;
+  } catch (const gsw::error &e) {
+    return fail(e.code, e.what());
+  } catch (const std::invalid_argument &e) {
+    return fail(GSW_E_INVALID, e.what());
+  } catch (const std::bad_alloc &) {
+    return fail(GSW_E_ALLOC, "host allocation failed");
+  } catch (const std::exception &e) {
+    return fail(GSW_E_INTERNAL, e.what());
+  }
+}
+
+void require_gpu() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    throw gsw::error(GSW_E_HIP, "no GPU available");
+}
+
+// ---- app-specific init / output kernels ----------------------------------
+__global__ void k_init_bfs(int *lvl, int64_t n, int root) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    lvl[v] = (v == root) ? 0 : -1;
+}
+__global__ void k_init_sssp(uint32_t *wa, uint32_t *ra, int64_t n, int root) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t x = (v == root) ? 0u : UINT32_MAX;
+    wa[v] = x;
+    ra[v] = x;
+  }
+}
+__global__ void k_init_iota(int *lab, int64_t n) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    lab[v] = (int)v;
+}
+__global__ void k_init_pr(double *wa, float *ra, const int64_t *row_ptr,
+                          int64_t n, float damp) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t od = row_ptr[v + 1] - row_ptr[v];
+    wa[v] = 0.0;
+    ra[v] = od ? (1.0f - damp) / (float)od : (1.0f - damp);
+  }
+}
+__global__ void k_pr_output(const float *ra, const int64_t *row_ptr,
+                            int64_t n, float *out) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t od = row_ptr[v + 1] - row_ptr[v];
+    out[v] = od ? (float)((double)ra[v] * (double)od) : ra[v];
+  }
+}
+
+}  // namespace
+
+struct gsw_host_graph {
+  gsw_host::HostGraph g;
+};
+
+// Per-application persistent state: frontier + functor arrays, reused by
+// every run on the same graph so that runs allocate nothing.
+template <typename F> struct app_ctx_t {
+  active_set_t as;
+  F f{};
+  engine_t en;
+  bool arch_done = false;
+  void release() {
+    as.release();
+    f.data.release();
+    en.release();
+  }
+};
+
+struct gsw_graph {
+  gsw_host::HostGraph hg;
+  graph_attr_t attr;
+  double mean_weight = 0;
+  GU dg;
+  GW dgw;
+  bool has_w = false;
+  GC coo;
+  bool has_coo = false;
+  std::unique_ptr<app_ctx_t<BFS>> bfs;
+  std::unique_ptr<app_ctx_t<SSSP>> sssp;
+  std::unique_ptr<app_ctx_t<CC>> cc;
+  std::unique_ptr<app_ctx_t<PR>> pr;
+  float *d_out_f = nullptr;
+  int32_t *d_flag = nullptr;
+  int32_t *h_flag = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<gsw_level_record> records;
+
+  void release() {
+    if (bfs) bfs->release();
+    if (sssp) sssp->release();
+    if (cc) cc->release();
+    if (pr) pr->release();
+    if (has_w) hipFree(dgw.dg_edgedata);
+    if (has_coo) coo.release();
+    dg.release();
+    if (d_out_f) hipFree(d_out_f);
+    if (d_flag) hipFree(d_flag);
+    if (h_flag) hipHostFree(h_flag);
+    if (ev0) hipEventDestroy(ev0);
+    if (ev1) hipEventDestroy(ev1);
+  }
+};
+
+namespace {
+
+void upload_graph(gsw_graph *G) {
+  G->attr.fill(G->hg.row_ptr.data(), G->hg.n);
+  G->dg.build(G->hg);
+  if (G->hg.weighted) {
+    double s = 0;
+    for (uint32_t x : G->hg.w) s += x;
+    G->mean_weight = G->hg.w.empty() ? 0 : s / (double)G->hg.w.size();
+    G->dgw.nvertexs = G->dg.nvertexs;
+    G->dgw.nedges = G->dg.nedges;
+    G->dgw.directed = G->dg.directed;
+    G->dgw.dg_row_ptr = G->dg.dg_row_ptr;
+    G->dgw.dg_adj_list = G->dg.dg_adj_list;
+    G->dgw.dgr_row_ptr = G->dg.dgr_row_ptr;
+    G->dgw.dgr_adj_list = G->dg.dgr_adj_list;
+    GSW_REQUIRE(!G->hg.directed, GSW_E_UNSUPPORTED,
+                "weighted directed graphs are not supported");
+    H_ERR(hipMalloc((void **)&G->dgw.dg_edgedata,
+                    sizeof(uint32_t) * (G->hg.m + 1)));
+    if (G->hg.m)
+      H_ERR(hipMemcpy(G->dgw.dg_edgedata, G->hg.w.data(),
+                      sizeof(uint32_t) * G->hg.m, hipMemcpyHostToDevice));
+    G->dgw.dgr_edgedata = G->dgw.dg_edgedata;
+    G->has_w = true;
+  }
+  H_ERR(hipMalloc((void **)&G->d_out_f, sizeof(float) * (G->hg.n + 1)));
+  H_ERR(hipMalloc((void **)&G->d_flag, sizeof(int32_t) * 4));
+  H_ERR(hipHostMalloc((void **)&G->h_flag, sizeof(int32_t) * 4));
+  H_ERR(hipEventCreate(&G->ev0));
+  H_ERR(hipEventCreate(&G->ev1));
+}
+
+run_opts_t to_opts(const gsw_run_opts *o) {
+  run_opts_t r;
+  if (!o) return r;
+  if (o->configs) r.force = parse_configs(o->configs);
+  if (o->max_levels > 0) r.max_levels = o->max_levels;
+  r.verbose = o->verbose != 0;
+  if (o->twc_small > 0) r.twc_small = o->twc_small;
+  if (o->twc_large > 0) r.twc_large = o->twc_large;
+  return r;
+}
+
+template <typename F> void prepare_engine(app_ctx_t<F> &c, const run_opts_t &o,
+                                          bool profile) {
+  c.en.conf = config_t{};
+  const feature_t keep = c.en.fets;
+  c.en.fets = feature_t{};
+  c.en.fets.cap = keep.cap;
+  c.en.fets.register_lim = keep.register_lim;
+  c.en.fets.thread_lim = keep.thread_lim;
+  c.en.fets.sm_num = keep.sm_num;
+  c.en.stats.reset();
+  c.en.stats.build();
+  c.en.inspector = inspector_t{};
+  c.en.executor.clear_log();
+  c.en.executor.profile = profile;
+  apply_opts(c.en.conf, o);
+}
+
+template <typename F> void arch_once(app_ctx_t<F> &c) {
+  if (!c.arch_done) {
+    c.en.fets.architecture_features();
+    c.arch_done = true;
+  }
+}
+
+void collect_records(gsw_graph *G, executor_t &ex, active_set_t &as) {
+  ex.finalize_log(as);
+  G->records.clear();
+  for (const auto &r : ex.log) {
+    gsw_level_record o{};
+    o.level = r.level;
+    o.dir = (int8_t)r.dir;
+    o.fmt = (int8_t)r.fmt;
+    o.lb = (int8_t)r.lb;
+    o.fused = (int8_t)r.fused;
+    o.sorted = (int8_t)r.sorted;
+    o.frontier = r.frontier;
+    o.active = r.active;
+    o.inactive = r.inactive;
+    o.edges = r.edges;
+    o.verts = r.verts;
+    o.ms = r.ms;
+    G->records.push_back(o);
+  }
+}
+
+template <typename Gv> struct view_t {
+  const gsw_host::HostGraph *hg;
+  graph_attr_t attr;
+  double mean_weight;
+};
+
+double elapsed(gsw_graph *G) {
+  float ms = 0;
+  H_ERR(hipEventSynchronize(G->ev1));
+  H_ERR(hipEventElapsedTime(&ms, G->ev0, G->ev1));
+  return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gsw_version(void) { return "gswitch_amd 0.1 (gfx950)"; }
+
+const char *gsw_last_error(void) { return g_last_error.c_str(); }
+
+int32_t gsw_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+void gsw_run_opts_init(gsw_run_opts *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->max_levels = 1 << 20;
+  o->twc_small = 32;
+  o->twc_large = 2048;
+  o->profile = 1;
+}
+
+// ---- host ingest -----------------------------------------------------------
+gsw_status gsw_host_graph_from_edges(int64_t n, int64_t ne, const int32_t *src,
+                                     const int32_t *dst, const uint32_t *w,
+                                     int32_t directed, gsw_host_graph **out) {
+  return guarded([&] {
+    GSW_REQUIRE(out && (ne == 0 || (src && dst)) && n >= 0 && ne >= 0,
+                GSW_E_INVALID, "invalid arguments");
+    auto *h = new gsw_host_graph();
+    try {
+      gsw_host::build_csr(n, ne, src, dst, w, directed != 0, 0, 64, h->g);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+static gsw_status finish_gen(int64_t n, std::vector<int32_t> &s,
+                             std::vector<int32_t> &d, uint64_t seed,
+                             int32_t weighted, gsw_host_graph **out) {
+  auto *h = new gsw_host_graph();
+  try {
+    const uint64_t wkey =
+        weighted ? gsw_host::stream_key(seed, gsw_host::STREAM_WEIGHT) : 0;
+    gsw_host::build_csr(n, (int64_t)s.size(), s.data(), d.data(), nullptr,
+                        false, wkey, 64, h->g);
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  return GSW_OK;
+}
+
+gsw_status gsw_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b,
+                        double c, uint64_t seed, int32_t weighted,
+                        gsw_host_graph **out) {
+  return guarded([&] {
+    GSW_REQUIRE(out && scale >= 1 && scale <= 30 && edge_factor >= 1,
+                GSW_E_INVALID, "invalid R-MAT parameters");
+    GSW_REQUIRE(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0, GSW_E_INVALID,
+                "invalid R-MAT probabilities");
+    std::vector<int32_t> s, d;
+    gsw_host::gen_rmat(scale, edge_factor, a, b, c, seed, s, d);
+    finish_gen((int64_t)1 << scale, s, d, seed, weighted, out);
+  });
+}
+
+gsw_status gsw_gen_grid2d(int64_t rows, int64_t cols, double p_delete,
+                          double long_range_frac, uint64_t seed,
+                          int32_t weighted, gsw_host_graph **out) {
+  return guarded([&] {
+    GSW_REQUIRE(out && rows >= 1 && cols >= 1 && rows * cols <= INT32_MAX,
+                GSW_E_INVALID, "invalid grid size");
+    std::vector<int32_t> s, d;
+    gsw_host::gen_grid2d(rows, cols, p_delete, long_range_frac, seed, s, d);
+    finish_gen(rows * cols, s, d, seed, weighted, out);
+  });
+}
+
+gsw_status gsw_gen_er(int64_t n, double mean_degree, uint64_t seed,
+                      int32_t weighted, gsw_host_graph **out) {
+  return guarded([&] {
+    GSW_REQUIRE(out && n >= 1 && n <= INT32_MAX && mean_degree >= 0,
+                GSW_E_INVALID, "invalid ER parameters");
+    std::vector<int32_t> s, d;
+    gsw_host::gen_er(n, mean_degree, seed, s, d);
+    finish_gen(n, s, d, seed, weighted, out);
+  });
+}
+
+gsw_status gsw_gen_ba(int64_t n, int32_t m, uint64_t seed, int32_t weighted,
+                      gsw_host_graph **out) {
+  return guarded([&] {
+    GSW_REQUIRE(out && n >= 1 && n <= INT32_MAX && m >= 1, GSW_E_INVALID,
+                "invalid BA parameters");
+    std::vector<int32_t> s, d;
+    gsw_host::gen_ba(n, m, seed, s, d);
+    finish_gen(n, s, d, seed, weighted, out);
+  });
+}
+
+gsw_status gsw_host_graph_info(const gsw_host_graph *g, int64_t *n, int64_t *m,
+                               int32_t *directed, int32_t *weighted) {
+  return guarded([&] {
+    GSW_REQUIRE(g, GSW_E_INVALID, "null graph");
+    if (n) *n = g->g.n;
+    if (m) *m = g->g.m;
+    if (directed) *directed = g->g.directed;
+    if (weighted) *weighted = g->g.weighted;
+  });
+}
+
+gsw_status gsw_host_graph_export(const gsw_host_graph *g, int64_t *row_ptr,
+                                 int32_t *col, uint32_t *w) {
+  return guarded([&] {
+    GSW_REQUIRE(g && row_ptr && (col || g->g.m == 0), GSW_E_INVALID,
+                "invalid arguments");
+    std::memcpy(row_ptr, g->g.row_ptr.data(), sizeof(int64_t) * (g->g.n + 1));
+    if (g->g.m) std::memcpy(col, g->g.col.data(), sizeof(int32_t) * g->g.m);
+    if (w && g->g.weighted && g->g.m)
+      std::memcpy(w, g->g.w.data(), sizeof(uint32_t) * g->g.m);
+  });
+}
+
+void gsw_host_graph_destroy(gsw_host_graph *g) { delete g; }
+
+gsw_status gsw_degree_stats(const int64_t *row_ptr, int64_t n, double *out6) {
+  return guarded([&] {
+    GSW_REQUIRE(row_ptr && out6 && n > 0, GSW_E_INVALID, "invalid arguments");
+    graph_attr_t a;
+    a.fill(row_ptr, n);
+    out6[0] = a.avg_deg;
+    out6[1] = a.std_deg;
+    out6[2] = a.max_deg;
+    out6[3] = a.range;
+    out6[4] = a.GI;
+    out6[5] = a.Her;
+  });
+}
+
+// ---- device graph ----------------------------------------------------------
+gsw_status gsw_graph_create_csr(const int64_t *row_ptr, const int32_t *col,
+                                const uint32_t *w, int64_t n, int64_t m,
+                                int32_t directed, gsw_graph **out) {
+  return guarded([&] {
+    GSW_REQUIRE(out && row_ptr && (col || m == 0) && n >= 1 && m >= 0 &&
+                    n <= INT32_MAX,
+                GSW_E_INVALID, "invalid arguments");
+    GSW_REQUIRE(!directed, GSW_E_UNSUPPORTED,
+                "gsw_graph_create_csr takes symmetric graphs; use "
+                "gsw_host_graph_from_edges for directed input");
+    GSW_REQUIRE(row_ptr[0] == 0 && row_ptr[n] == m, GSW_E_INVALID,
+                "row_ptr must start at 0 and end at m");
+    for (int64_t v = 0; v < n; ++v)
+      GSW_REQUIRE(row_ptr[v] <= row_ptr[v + 1], GSW_E_INVALID,
+                  "row_ptr must be non-decreasing");
+    for (int64_t j = 0; j < m; ++j)
+      GSW_REQUIRE(col[j] >= 0 && col[j] < n, GSW_E_INVALID,
+                  "column index out of range");
+    require_gpu();
+    auto G = std::make_unique<gsw_graph>();
+    G->hg.n = n;
+    G->hg.m = m;
+    G->hg.directed = false;
+    G->hg.weighted = (w != nullptr);
+    G->hg.row_ptr.assign(row_ptr, row_ptr + n + 1);
+    G->hg.col.assign(col, col + m);
+    if (w) G->hg.w.assign(w, w + m);
+    try {
+      upload_graph(G.get());
+    } catch (...) {
+      G->release();
+      throw;
+    }
+    *out = G.release();
+  });
+}
+
+gsw_status gsw_graph_from_host(const gsw_host_graph *hg, gsw_graph **out) {
+  return guarded([&] {
+    GSW_REQUIRE(out && hg && hg->g.n >= 1, GSW_E_INVALID, "invalid arguments");
+    require_gpu();
+    auto G = std::make_unique<gsw_graph>();
+    G->hg = hg->g;
+    try {
+      upload_graph(G.get());
+    } catch (...) {
+      G->release();
+      throw;
+    }
+    *out = G.release();
+  });
+}
+
+gsw_status gsw_graph_destroy(gsw_graph *g) {
+  return guarded([&] {
+    if (!g) return;
+    gsw::sync();
+    g->release();
+    delete g;
+  });
+}
+
+gsw_status gsw_last_records(const gsw_graph *g, gsw_level_record *out,
+                            int32_t cap, int32_t *n) {
+  return guarded([&] {
+    GSW_REQUIRE(g && n && (out || cap == 0), GSW_E_INVALID, "invalid arguments");
+    const int32_t k = (int32_t)MIN((int64_t)cap, (int64_t)g->records.size());
+    if (k) std::memcpy(out, g->records.data(), sizeof(gsw_level_record) * k);
+    *n = k;
+  });
+}
+
+// ---- BFS -------------------------------------------------------------------
+gsw_status gsw_bfs(gsw_graph *G, int32_t root, const gsw_run_opts *opts,
+                   int32_t *levels_out, gsw_run_stats *stats) {
+  return guarded([&] {
+    GSW_REQUIRE(G, GSW_E_INVALID, "null graph");
+    GSW_REQUIRE(root >= 0 && root < G->hg.n, GSW_E_INVALID, "root out of range");
+    const run_opts_t o = to_opts(opts);
+    if (!G->bfs) {
+      G->bfs = std::make_unique<app_ctx_t<BFS>>();
+      G->bfs->as.build(G->hg.n, Normal);
+      G->bfs->f.data.build(G->hg.n);
+    }
+    auto &c = *G->bfs;
+    arch_once(c);
+    prepare_engine(c, o, !opts || opts->profile);
+    feature_t &fets = c.en.fets;
+    fets.centric = VC;
+    fets.pattern = Idem;
+    fets.fromall = false;
+    fets.toall = false;
+    fets.use_root = root;
+    fets.set_graph((double)G->hg.n, (double)(G->hg.m / 2), G->attr);
+    c.en.conf.conf_pruning = true;
+    view_t<GU> gv{&G->hg, G->attr, 0.0};
+    init_conf(c.en.stats, fets, c.en.conf, gv, c.f);
+
+    hipStream_t st = gsw::stream();
+    H_ERR(hipEventRecord(G->ev0, st));
+    c.as.reset();
+    c.as.init(root);
+    hipLaunchKernelGGL(k_init_bfs, dim3(gsw::grid_for(G->hg.n)), dim3(gsw::BLOCK),
+                       0, st, c.f.data.dg_wa, G->hg.n, root);
+    GU g = G->dg;
+    int levels = run_vc(c.en, c.as, g, c.f, o, [](int) { return false; });
+    H_ERR(hipEventRecord(G->ev1, st));
+    const double ms = elapsed(G);
+    collect_records(G, c.en.executor, c.as);
+    if (levels_out)
+      H_ERR(hipMemcpy(levels_out, c.f.data.dg_wa, sizeof(int32_t) * G->hg.n,
+                      hipMemcpyDeviceToHost));
+    if (stats) {
+      stats->device_ms = ms;
+      stats->levels = levels;
+      stats->n_records = (int32_t)G->records.size();
+      stats->reached = 0;
+    }
+  });
+}
+
+// ---- SSSP ------------------------------------------------------------------
+gsw_status gsw_sssp(gsw_graph *G, int32_t root, const gsw_run_opts *opts,
+                    uint32_t *dist_out, gsw_run_stats *stats) {
+  return guarded([&] {
+    GSW_REQUIRE(G, GSW_E_INVALID, "null graph");
+    GSW_REQUIRE(G->has_w, GSW_E_INVALID, "SSSP needs a weighted graph");
+    GSW_REQUIRE(root >= 0 && root < G->hg.n, GSW_E_INVALID, "root out of range");
+    const run_opts_t o = to_opts(opts);
+    if (!G->sssp) {
+      G->sssp = std::make_unique<app_ctx_t<SSSP>>();
+      G->sssp->as.build(G->hg.n, Normal);
+      G->sssp->f.data.build(G->hg.n);
+    }
+    auto &c = *G->sssp;
+    arch_once(c);
+    prepare_engine(c, o, !opts || opts->profile);
+    feature_t &fets = c.en.fets;
+    fets.centric = VC;
+    fets.pattern = Mono;
+    fets.fromall = false;
+    fets.toall = true;
+    fets.use_root = root;
+    fets.set_graph((double)G->hg.n, (double)(G->hg.m / 2), G->attr,
+                   G->mean_weight);
+    c.en.conf.conf_window = true;
+    c.f.data.window = window_t{};
+    view_t<GW> gv{&G->hg, G->attr, G->mean_weight};
+    init_conf(c.en.stats, fets, c.en.conf, gv, c.f);
+
+    hipStream_t st = gsw::stream();
+    H_ERR(hipEventRecord(G->ev0, st));
+    c.as.reset();
+    c.as.init(root);
+    hipLaunchKernelGGL(k_init_sssp, dim3(gsw::grid_for(G->hg.n)),
+                       dim3(gsw::BLOCK), 0, st, c.f.data.dg_wa, c.f.data.dg_ra,
+                       G->hg.n, root);
+    GW g = G->dgw;
+    int levels = run_vc(c.en, c.as, g, c.f, o, [](int) { return false; });
+    H_ERR(hipEventRecord(G->ev1, st));
+    const double ms = elapsed(G);
+    collect_records(G, c.en.executor, c.as);
+    if (dist_out)
+      H_ERR(hipMemcpy(dist_out, c.f.data.dg_wa, sizeof(uint32_t) * G->hg.n,
+                      hipMemcpyDeviceToHost));
+    if (stats) {
+      stats->device_ms = ms;
+      stats->levels = levels;
+      stats->n_records = (int32_t)G->records.size();
+      stats->reached = 0;
+    }
+  });
+}
+
+// ---- CC --------------------------------------------------------------------
+gsw_status gsw_cc(gsw_graph *G, const gsw_run_opts *opts, int32_t *labels_out,
+                  gsw_run_stats *stats) {
+  return guarded([&] {
+    GSW_REQUIRE(G, GSW_E_INVALID, "null graph");
+    const run_opts_t o = to_opts(opts);
+    if (!G->has_coo) {
+      G->coo.build(G->hg);
+      G->has_coo = true;
+    }
+    if (!G->cc) {
+      G->cc = std::make_unique<app_ctx_t<CC>>();
+      G->cc->as.build(MAX((int64_t)1, G->coo.nedges), Normal);
+      G->cc->f.data.build(G->hg.n);
+    }
+    auto &c = *G->cc;
+    arch_once(c);
+    prepare_engine(c, o, !opts || opts->profile);
+    feature_t &fets = c.en.fets;
+    fets.centric = EC;
+    fets.pattern = ASSO;
+    fets.set_graph((double)G->hg.n, (double)G->coo.nedges, G->attr);
+    view_t<GC> gv{&G->hg, G->attr, 0.0};
+    init_conf(c.en.stats, fets, c.en.conf, gv, c.f);
+
+    hipStream_t st = gsw::stream();
+    H_ERR(hipEventRecord(G->ev0, st));
+    c.as.reset();
+    c.as.init(ALL_ACTIVE);
+    hipLaunchKernelGGL(k_init_iota, dim3(gsw::grid_for(G->hg.n)),
+                       dim3(gsw::BLOCK), 0, st, c.f.data.dg_wa, G->hg.n);
+    GC g = G->coo;
+    int level = 0;
+    if (g.nedges > 0) {
+      for (level = 0; level < o.max_levels; ++level) {
+        c.en.inspector.inspect(c.as, g, c.f, c.en.stats, fets, c.en.conf);
+        if (c.as.finish(g, c.f, c.en.conf)) break;
+        c.en.selector.select(c.en.stats, fets, c.en.conf);
+        c.en.executor.filter(c.as, g, c.f, c.en.stats, fets, c.en.conf);
+        g.update_level();
+        c.en.executor.expand(c.as, g, c.f, c.en.stats, fets, c.en.conf);
+        p_jump(c.f, G->hg.n, G->d_flag, G->h_flag);
+        if (c.as.finish(g, c.f, c.en.conf)) break;
+      }
+    }
+    H_ERR(hipEventRecord(G->ev1, st));
+    const double ms = elapsed(G);
+    collect_records(G, c.en.executor, c.as);
+    if (labels_out)
+      H_ERR(hipMemcpy(labels_out, c.f.data.dg_wa, sizeof(int32_t) * G->hg.n,
+                      hipMemcpyDeviceToHost));
+    if (stats) {
+      stats->device_ms = ms;
+      stats->levels = level;
+      stats->n_records = (int32_t)G->records.size();
+      stats->reached = 0;
+    }
+  });
+}
+
+// ---- PageRank --------------------------------------------------------------
+gsw_status gsw_pagerank(gsw_graph *G, float damping, int32_t iters, float eps,
+                        const gsw_run_opts *opts, float *pr_out,
+                        gsw_run_stats *stats) {
+  return guarded([&] {
+    GSW_REQUIRE(G, GSW_E_INVALID, "null graph");
+    GSW_REQUIRE(iters >= 0 && damping >= 0 && damping <= 1 && eps >= 0,
+                GSW_E_INVALID, "invalid PageRank parameters");
+    const run_opts_t o = to_opts(opts);
+    if (!G->pr) {
+      G->pr = std::make_unique<app_ctx_t<PR>>();
+      G->pr->as.build(G->hg.n, Normal);
+      G->pr->f.data.build(G->hg.n);
+    }
+    auto &c = *G->pr;
+    arch_once(c);
+    prepare_engine(c, o, !opts || opts->profile);
+    c.f.damp = damping;
+    c.f.eps = eps;
+    feature_t &fets = c.en.fets;
+    fets.centric = VC;
+    fets.pattern = ASSO;
+    fets.fromall = true;
+    fets.toall = true;
+    fets.set_graph((double)G->hg.n, (double)(G->hg.m / 2), G->attr);
+    view_t<GU> gv{&G->hg, G->attr, 0.0};
+    init_conf(c.en.stats, fets, c.en.conf, gv, c.f);
+
+    hipStream_t st = gsw::stream();
+    H_ERR(hipEventRecord(G->ev0, st));
+    c.as.reset();
+    c.as.init(ALL_ACTIVE);
+    hipLaunchKernelGGL(k_init_pr, dim3(gsw::grid_for(G->hg.n)), dim3(gsw::BLOCK),
+                       0, st, c.f.data.dg_wa, c.f.data.dg_ra, G->dg.dg_row_ptr,
+                       G->hg.n, damping);
+    GU g = G->dg;
+    const int cap = iters;
+    int levels = run_vc(c.en, c.as, g, c.f, o,
+                        [cap](int level) { return level >= cap; });
+    hipLaunchKernelGGL(k_pr_output, dim3(gsw::grid_for(G->hg.n)),
+                       dim3(gsw::BLOCK), 0, st, c.f.data.dg_ra,
+                       G->dg.dg_row_ptr, G->hg.n, G->d_out_f);
+    H_ERR(hipEventRecord(G->ev1, st));
+    const double ms = elapsed(G);
+    collect_records(G, c.en.executor, c.as);
+    if (pr_out)
+      H_ERR(hipMemcpy(pr_out, G->d_out_f, sizeof(float) * G->hg.n,
+                      hipMemcpyDeviceToHost));
+    if (stats) {
+      stats->device_ms = ms;
+      stats->levels = levels;
+      stats->n_records = (int32_t)G->records.size();
+      stats->reached = 0;
+    }
+  });
+}
+
+// ---- selector trees ----------------------------------------------------------
+gsw_status gsw_tree_eval(const char *name, const double *fv, int32_t nfv,
+                         int32_t *cls) {
+  return guarded([&] {
+    GSW_REQUIRE(name && fv && cls, GSW_E_INVALID, "invalid arguments");
+    using namespace gsw_model;
+    struct {
+      const char *n;
+      const tree_t *t;
+    } table[] = {
+        {"select_dir_idem", &k_select_dir_idem},
+        {"select_dir_asso", &k_select_dir_asso},
+        {"select_dir_asso_backup", &k_select_dir_asso_backup},
+        {"select_fmt", &k_select_fmt},
+        {"select_fusion", &k_select_fusion},
+        {"select_lb_idem", &k_select_lb_idem},
+        {"select_lb_asso_forward", &k_select_lb_asso_forward},
+        {"select_lb_asso_backward", &k_select_lb_asso_backward},
+        {"select_lb_asso_forward_backup", &k_select_lb_asso_forward_backup},
+        {"select_lb_asso_backward_backup", &k_select_lb_asso_backward_backup},
+        {"select_stepping", &k_select_stepping},
+    };
+    for (auto &e : table) {
+      if (!std::strcmp(e.n, name)) {
+        GSW_REQUIRE(nfv >= e.t->n_features, GSW_E_INVALID,
+                    "feature vector too short for this tree");
+        *cls = eval_tree(*e.t, fv);
+        return;
+      }
+    }
+    throw gsw::error(GSW_E_INVALID, std::string("unknown tree ") + name);
+  });
+}
+
+// ---- primitives --------------------------------------------------------------
+namespace {
+__global__ void k_wave_scan64(const int *in, int *out) {
+  out[threadIdx.x] = gsw::wave_inclusive_scan(in[threadIdx.x]);
+}
+}  // namespace
+
+gsw_status gsw_wave_scan64(const int32_t *in, int32_t *out) {
+  return guarded([&] {
+    GSW_REQUIRE(in && out, GSW_E_INVALID, "invalid arguments");
+    require_gpu();
+    int *d = nullptr;
+    H_ERR(hipMalloc((void **)&d, sizeof(int) * 128));
+    H_ERR(hipMemcpy(d, in, sizeof(int) * 64, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_wave_scan64, dim3(1), dim3(64), 0, gsw::stream(), d,
+                       d + 64);
+    H_ERR(hipGetLastError());
+    H_ERR(hipMemcpy(out, d + 64, sizeof(int) * 64, hipMemcpyDeviceToHost));
+    hipFree(d);
+  });
+}
+
+gsw_status gsw_scan_exclusive_i64(const int64_t *in, int64_t n, int64_t *out) {
+  return guarded([&] {
+    GSW_REQUIRE(in && out && n >= 0, GSW_E_INVALID, "invalid arguments");
+    require_gpu();
+    int64_t *d = nullptr, *bs = nullptr;
+    const int64_t nb = (n + gsw::SCAN_TILE - 1) / gsw::SCAN_TILE;
+    H_ERR(hipMalloc((void **)&d, sizeof(int64_t) * (n + 1)));
+    H_ERR(hipMalloc((void **)&bs, sizeof(int64_t) * (nb + 2)));
+    if (n) H_ERR(hipMemcpy(d, in, sizeof(int64_t) * n, hipMemcpyHostToDevice));
+    gsw::scan_exclusive_inplace(d, n, bs);
+    H_ERR(hipMemcpy(out, d, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
+    hipFree(d);
+    hipFree(bs);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,130 @@
+// Per-level execution configuration chosen by the selector.
+//
+// Field names follow config_t (src/abstraction/config.cuh:9-123). Additions:
+// conf_sorted (sorted vs unsorted queue frontier), the TWC bin thresholds,
+// and `force`, the per-run form of the reference's `--configs=Dir-Fmt-LB-
+// Fusion` override (src/utils/cmdline.cuh:51-94) that the selector applies
+// after its own decision.
+#pragma once
+#include <iostream>
+
+#include "../utils/common.h"
+
+struct ins_t {
+  bool has_direction = false, has_asfmt = false, has_lb = false;
+  bool has_fusion = false, has_sorted = false;
+  Direction d = Push;
+  ASFmt asf = Queue;
+  LB lb = WM;
+  Fusion fusion = Standalone;
+  bool sorted = false;
+};
+
+struct config_t {
+  Direction conf_dir = Push;
+  Status conf_target = Active;
+  CullMode conf_cm = Interleave;
+  ASFmt conf_asfmt = Queue;
+  LB conf_lb = WM;
+  Centric conf_centric = VC;
+  QueueMode conf_qmode = Normal;
+
+  bool conf_fromall = false;
+  bool conf_toall = false;
+  bool conf_fixed = false;
+  bool conf_inherit = false;
+  bool conf_2d_partition = false;
+
+  bool conf_first_round = true;
+  bool conf_switch_to_fusion = false;
+  bool conf_switch_to_standalone = false;
+  bool conf_pruning = false;
+  bool conf_once = false;
+  bool conf_ignore_u_state = false;
+  bool conf_fuse_inspect = false;
+  bool conf_lite = false;
+  bool conf_window = false;
+  bool conf_compensation = false;
+  bool conf_super_fusion = false;
+  bool conf_fusion = false;
+  bool conf_sorted = false;
+  int conf_idle = 0;
+  int ctanum = 256;
+  int thdnum = gsw::BLOCK;
+  // TWC bins: degree < twc_small -> thread, < twc_large -> wavefront,
+  // otherwise workgroup.
+  int twc_small = 32;
+  int twc_large = 2048;
+  ins_t force;
+
+  void reset() {
+    conf_first_round = true;
+    conf_switch_to_fusion = false;
+    conf_switch_to_standalone = false;
+    conf_fuse_inspect = false;
+    conf_idle = 0;
+  }
+
+  __host__ __device__ __forceinline__ Status want() const { return conf_target; }
+  __host__ __device__ __forceinline__ bool pruning() const { return conf_pruning; }
+  __host__ __device__ __forceinline__ bool fuse_inspect() const {
+    return conf_fuse_inspect;
+  }
+  __host__ __device__ __forceinline__ bool ignore_u_state() const {
+    return conf_ignore_u_state;
+  }
+
+  static const char *lb_name(LB lb) {
+    switch (lb) {
+      case TM: return "TM";
+      case WM: return "WM";
+      case CM: return "CM";
+      case STRICT: return "STRICT";
+      case TWOD: return "2D-partition";
+      case ELB: return "ELB";
+      case TWC: return "TWC";
+    }
+    return "Unknown";
+  }
+
+  void show_hints(std::ostream &out = std::cout) const {
+    out << "[" << (conf_dir == Push ? "Push|" : "Pull|")
+        << (conf_asfmt == Queue ? (conf_sorted ? "SortedQueue|" : "Queue|")
+                                : "Bitmap|")
+        << lb_name(conf_lb) << (conf_fuse_inspect ? "|Fused" : "|Standalone")
+        << "] ";
+  }
+};
+
+// Parse "Dir-Fmt-LB-Fusion" (any field may be "x"), as prase_configs does
+// (src/utils/cmdline.cuh:51-94); Fmt additionally accepts "SortedQueue".
+inline ins_t parse_configs(const std::string &s) {
+  ins_t ins;
+  std::string parts[4];
+  int k = 0;
+  std::string cur;
+  for (char c : s) {
+    if (c == '-') {
+      if (k < 4) parts[k++] = cur;
+      cur.clear();
+    } else {
+      cur += c;
+    }
+  }
+  if (k < 4) parts[k++] = cur;
+  if (parts[0] == "Push") { ins.has_direction = true; ins.d = Push; }
+  else if (parts[0] == "Pull") { ins.has_direction = true; ins.d = Pull; }
+  if (parts[1] == "Queue") { ins.has_asfmt = true; ins.asf = Queue; ins.has_sorted = true; ins.sorted = false; }
+  else if (parts[1] == "SortedQueue") { ins.has_asfmt = true; ins.asf = Queue; ins.has_sorted = true; ins.sorted = true; }
+  else if (parts[1] == "Bitmap") { ins.has_asfmt = true; ins.asf = Bitmap; }
+  const std::string &l = parts[2];
+  if (l == "TM") { ins.has_lb = true; ins.lb = TM; }
+  else if (l == "WM") { ins.has_lb = true; ins.lb = WM; }
+  else if (l == "CM") { ins.has_lb = true; ins.lb = CM; }
+  else if (l == "STRICT") { ins.has_lb = true; ins.lb = STRICT; }
+  else if (l == "TWC") { ins.has_lb = true; ins.lb = TWC; }
+  else if (l == "TWOD") { ins.has_lb = true; ins.lb = TWOD; }
+  if (parts[3] == "Fused") { ins.has_fusion = true; ins.fusion = Fused; }
+  else if (parts[3] == "Standalone") { ins.has_fusion = true; ins.fusion = Standalone; }
+  return ins;
+}

@@ -1,0 +1,337 @@
+// executor_t: frontier generation and expand dispatch.
+//
+// filter() follows executor_t::filter + __filter (src/abstraction/
+// executor.cuh:21-72, src/kernel_libs/filter.cuh:197-252): set the frontier
+// format, compact bitmap -> queue when a queue is needed and it was not
+// produced by a fused expand, swap the queues, size the grid and run the
+// fusion compensation pass. expand() follows executor_t::expand
+// (executor.cuh:74-148): one launch of the chosen {direction, LB, fusion}
+// variant, timing history, and the fused -> standalone fallback when an
+// expand takes more than 5x the average.
+//
+// Host<->device traffic per level: standalone levels sync once (the
+// inspector's counters); fused levels sync once after the expand (the next
+// queue's length, which also tells convergence). Queue lengths are otherwise
+// known on the host from the inspector's counts.
+#pragma once
+#include <vector>
+
+#include "../data_structures/active_set.h"
+#include "../kernels/expand_pull.h"
+#include "../kernels/expand_push.h"
+#include "../kernels/filter.h"
+#include "../kernels/misc.h"
+#include "config.h"
+#include "features.h"
+#include "inspector.h"
+#include "statistics.h"
+
+// One record per expand, for logs, training data and roofline accounting.
+struct level_record_t {
+  int level;
+  int dir, fmt, lb, fused, sorted;
+  long long frontier;      // input frontier length (queue) or n (bitmap)
+  long long active, inactive;
+  long long edges;         // edges inspected by the expand (device counter)
+  long long verts;         // frontier vertices expanded
+  float ms;                // expand time (hipEvent)
+};
+
+struct executor_t {
+  bool profile = true;
+  std::vector<level_record_t> log;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, int>> ev_pairs;  // per log entry
+  int ev_next = 0;
+  inspector_t *insp = nullptr;  // for the raw frontier counts
+
+  void bind(inspector_t &i) { insp = &i; }
+
+  void clear_log() {
+    log.clear();
+    ev_pairs.clear();
+    ev_next = 0;
+  }
+
+  void release() {
+    for (auto e : ev_pool) hipEventDestroy(e);
+    ev_pool.clear();
+    clear_log();
+  }
+
+  hipEvent_t next_event() {
+    if (ev_next == (int)ev_pool.size()) {
+      hipEvent_t e;
+      H_ERR(hipEventCreate(&e));
+      ev_pool.push_back(e);
+    }
+    return ev_pool[ev_next++];
+  }
+
+  // Resolve event timings of all logged expands (syncs the stream).
+  void finalize_log(active_set_t &as) {
+    gsw::sync();
+    for (size_t i = 0; i < log.size(); ++i) {
+      if (ev_pairs[i].first < 0) continue;
+      float ms = 0;
+      H_ERR(hipEventElapsedTime(&ms, ev_pool[ev_pairs[i].first],
+                                ev_pool[ev_pairs[i].second]));
+      log[i].ms = ms;
+    }
+    if (!log.empty() && as.dg_work_base) {
+      std::vector<unsigned long long> w(log.size() * WORK_N);
+      H_ERR(hipMemcpy(w.data(), as.dg_work_base,
+                      sizeof(unsigned long long) * w.size(),
+                      hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < log.size(); ++i) {
+        log[i].edges = (long long)w[i * WORK_N + WORK_EDGES];
+        log[i].verts = (long long)w[i * WORK_N + WORK_VERTS];
+      }
+    }
+  }
+
+  // ------------------------------------------------------------- filter
+  template <typename G, typename F>
+  void filter(active_set_t &as, G &g, F &f, stat_t &stat, feature_t &fets,
+              config_t &conf) {
+    const double s = stat.last_filter_time;
+    as.set_fmt(conf.conf_asfmt);
+    if (conf.conf_asfmt == Queue) {
+      bool tofilter = !conf.conf_fuse_inspect;
+      if (conf.conf_switch_to_fusion) {
+        tofilter = true;
+        conf.conf_switch_to_fusion = false;
+      }
+      if (conf.conf_first_round) tofilter = true;
+      if (tofilter) {
+        if (!(as.queue.mode == Cached && as.queue.traceback))
+          run_filter(as, conf);
+      } else if (!conf.conf_pruning) {
+        gsw::fill_async<uint64_t>(as.bitmap.visited.dg_bits, 0,
+                                  as.bitmap.visited.n_words + 1);
+      }
+      as.swap_queues();
+    }
+    conf.thdnum = gsw::BLOCK;
+    if (!conf.conf_fuse_inspect) {
+      const double wl = (conf.conf_dir == Push) ? fets.push_workload
+                                                : fets.pull_workload;
+      conf.ctanum = (int)MIN(256.0, wl / conf.thdnum / 2 + 1);
+      if (conf.ctanum < 4)
+        conf.conf_idle += 1;
+      else
+        conf.conf_idle = 0;
+      if (conf.ctanum < 5 && conf.conf_lb == STRICT && !conf.force.has_lb)
+        conf.conf_lb = CM;
+    }
+    if constexpr (std::is_same<G, device_graph_t<CSR, typename G::edge_t>>::value) {
+      if (conf.conf_compensation && conf.conf_centric == VC &&
+          as.fmt == Queue) {
+        hipLaunchKernelGGL((gsw::k_compensation<G, F>),
+                           dim3(gsw::grid_for(as.get_size_host())),
+                           dim3(gsw::BLOCK), 0, gsw::stream(), as, g, f);
+        H_ERR(hipGetLastError());
+      }
+    }
+    const double e = gsw_now_ms();
+    stat.level = g.level;
+    stat.last_filter_time = e - s;
+    stat.avg_filter_time =
+        (stat.avg_filter_time * g.level + stat.last_filter_time) / (g.level + 1);
+    fets.last_filter_time = stat.last_filter_time;
+    fets.avg_filter_time = stat.avg_filter_time;
+  }
+
+  void run_filter(active_set_t &as, const config_t &conf) {
+    const int64_t n = as.size;
+    const int64_t nw = (n + 63) >> 6;
+    const Status target = conf.conf_target;
+    queue_t &q = as.queue;
+    const int out_slot = (q.mode == Normal) ? (q.cursor ^ 1) : (q.cursor + 1);
+    int32_t *out = (q.mode == Normal) ? q.buf[q.cursor ^ 1] : q.buf[0] + q.offset_out;
+    int32_t *out_size = q.sizes + out_slot;
+    const long long expect =
+        (target == Active) ? insp->known_active
+        : (target == Inactive) ? insp->known_inactive
+                               : insp->known_active + insp->known_inactive;
+    GSW_REQUIRE(expect <= (q.mode == Normal ? q.capacity : q.capacity - q.offset_out),
+                gsw::E_OVERFLOW, "frontier does not fit the queue");
+    gsw::set_one_async<int32_t>(out_size, 0);
+    if (!conf.conf_sorted) {
+      hipLaunchKernelGGL(gsw::k_filter_unsorted,
+                         dim3(gsw::grid_for(nw, gsw::BLOCK, 1024)),
+                         dim3(gsw::BLOCK), 0, gsw::stream(), as, target, n, out,
+                         out_size);
+    } else {
+      const int nb = (int)MAX(1, MIN(1024, (nw + gsw::BLOCK - 1) / gsw::BLOCK));
+      const int64_t wpb = (nw + nb - 1) / nb;
+      as.ensure_block_count(nb);
+      hipLaunchKernelGGL(gsw::k_filter_sorted_count, dim3(nb), dim3(gsw::BLOCK),
+                         0, gsw::stream(), as, target, n, wpb, as.dg_block_count);
+      hipLaunchKernelGGL(gsw::k_filter_sorted_write, dim3(nb), dim3(gsw::BLOCK),
+                         0, gsw::stream(), as, target, n, wpb, as.dg_block_count,
+                         out, out_size);
+    }
+    H_ERR(hipGetLastError());
+    q.host_sizes[out_slot] = (int)expect;
+  }
+
+  // ------------------------------------------------------------- expand
+  template <typename G, typename F>
+  void expand(active_set_t &as, G &g, F &f, stat_t &stat, feature_t &fets,
+              config_t &conf) {
+    const double s = gsw_now_ms();
+    level_record_t rec{};
+    rec.level = g.level;
+    rec.dir = conf.conf_dir;
+    rec.fmt = as.fmt;
+    rec.lb = conf.conf_lb;
+    rec.fused = conf.conf_fuse_inspect;
+    rec.sorted = conf.conf_sorted;
+    rec.frontier = as.get_size_host();
+    rec.active = insp ? insp->known_active : 0;
+    rec.inactive = insp ? insp->known_inactive : 0;
+    as.select_work_row((int)log.size());
+    int e0 = -1, e1 = -1;
+    if (profile) {
+      e0 = ev_next;
+      H_ERR(hipEventRecord(next_event(), gsw::stream()));
+    }
+    dispatch(as, g, f, conf);
+    if (profile) {
+      e1 = ev_next;
+      H_ERR(hipEventRecord(next_event(), gsw::stream()));
+    }
+    log.push_back(rec);
+    ev_pairs.push_back({e0, e1});
+
+    if (conf.conf_fuse_inspect && conf.conf_centric == VC) {
+      // the next queue's length: grid size for the next level, convergence
+      const int out_slot = (as.queue.mode == Normal) ? (as.queue.cursor ^ 1)
+                                                     : (as.queue.cursor + 1);
+      int nq = as.read_qsize_device(out_slot);
+      if (insp) insp->known_active = nq;
+      as.check_overflow_cached();
+      if (nq == 0) as.halt_host();
+    }
+    const double e = gsw_now_ms();
+    const int L = g.level > 0 ? g.level : 1;
+    stat.last_expand_time = e - s;
+    stat.last_time = stat.last_filter_time + stat.last_expand_time;
+    stat.avg_time = (stat.avg_time * (L - 1) + stat.last_time) / L;
+    stat.avg_expand_time = (stat.avg_expand_time * (L - 1) + stat.last_expand_time) / L;
+    fets.last_expand_time = stat.last_expand_time;
+    if (fets.pattern == Idem && fets.avg_expand_time > 0 &&
+        fets.last_expand_time > 5 * fets.avg_expand_time &&
+        conf.conf_fuse_inspect && !conf.force.has_fusion) {
+      conf.conf_fuse_inspect = false;
+      as.swap_queues();
+    }
+    fets.avg_expand_time = stat.avg_expand_time;
+    fets.last_time = stat.last_time;
+    fets.avg_time = stat.avg_time;
+    conf.conf_first_round = false;
+  }
+
+  template <typename G, typename F>
+  void dispatch(active_set_t &as, G &g, F &f, config_t &conf);
+};
+
+// grid for a frontier of N items processed `per` items per workgroup
+inline int expand_grid(int64_t N, int per) {
+  int64_t g = (N + per - 1) / per;
+  return (int)MAX(1, MIN((int64_t)gsw::MAX_GRID, g));
+}
+
+template <typename G, typename F>
+void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf) {
+  using namespace gsw;
+  hipStream_t st = stream();
+  if constexpr (std::is_same<G, device_graph_t<COO, typename G::edge_t>>::value) {
+    const int64_t N = as.get_size_host();
+    hipLaunchKernelGGL((k_expand_ec<G, F>), dim3(grid_for(N)), dim3(BLOCK), 0,
+                       st, as, g, f);
+    H_ERR(hipGetLastError());
+    return;
+  } else {
+    const int64_t N = as.get_size_host();
+    if (N == 0) return;
+    const bool push = conf.conf_dir == Push;
+    const bool fused = conf.conf_fuse_inspect;
+    LB lb = conf.conf_lb;
+    if (lb == TWOD || lb == ELB) lb = STRICT;
+    if (push) {
+      auto launch = [&](auto body) {
+        using Bd = decltype(body);
+        switch (lb) {
+          case TM:
+            hipLaunchKernelGGL(k_push_tm<Bd>, dim3(expand_grid(N, BLOCK)),
+                               dim3(BLOCK), 0, st, body);
+            break;
+          case WM:
+            hipLaunchKernelGGL(k_push_wm<Bd>, dim3(expand_grid(N, BLOCK)),
+                               dim3(BLOCK), 0, st, body);
+            break;
+          case CM:
+            hipLaunchKernelGGL(k_push_cm<Bd>, dim3(expand_grid(N, BLOCK)),
+                               dim3(BLOCK), 0, st, body);
+            break;
+          case TWC: {
+            as.bins.ensure(as.size);
+            fill_async<int32_t>(as.bins.sizes, 0, 4);
+            hipLaunchKernelGGL(k_twc_bin<G>, dim3(expand_grid(N, BLOCK)),
+                               dim3(BLOCK), 0, st, as, g, conf.conf_target,
+                               false, conf.twc_small, conf.twc_large);
+            hipLaunchKernelGGL(k_push_twc<Bd>, dim3(expand_grid(N, BLOCK)),
+                               dim3(BLOCK), 0, st, body);
+            break;
+          }
+          default: {  // STRICT
+            as.workset.ensure(as.size);
+            hipLaunchKernelGGL(k_strict_prepare<G>, dim3(grid_for(N)),
+                               dim3(BLOCK), 0, st, as, g, conf.conf_target,
+                               false, as.workset.dg_scan);
+            scan_exclusive_inplace(as.workset.dg_scan, N, as.workset.dg_bsum);
+            hipLaunchKernelGGL(k_push_strict<Bd>, dim3(MAX_GRID), dim3(BLOCK),
+                               0, st, body, (const int64_t *)as.workset.dg_scan);
+            break;
+          }
+        }
+        H_ERR(hipGetLastError());
+      };
+      if (fused)
+        launch(push_body<G, F, true>{as, g, f, conf});
+      else
+        launch(push_body<G, F, false>{as, g, f, conf});
+    } else {
+      using Bd = pull_body<G, F>;
+      Bd body{as, g, f, conf};
+      switch (lb) {
+        case TM:
+          hipLaunchKernelGGL(k_pull_tm<Bd>, dim3(expand_grid(N, BLOCK)),
+                             dim3(BLOCK), 0, st, body);
+          break;
+        case WM:
+        case TWC:
+          hipLaunchKernelGGL(k_pull_wm<Bd>, dim3(expand_grid(N, BLOCK)),
+                             dim3(BLOCK), 0, st, body);
+          break;
+        case CM:
+          hipLaunchKernelGGL(k_pull_cm<Bd>, dim3(expand_grid(N, BLOCK)),
+                             dim3(BLOCK), 0, st, body);
+          break;
+        default: {  // STRICT
+          as.workset.ensure(as.size);
+          hipLaunchKernelGGL(k_strict_prepare<G>, dim3(grid_for(N)),
+                             dim3(BLOCK), 0, st, as, g, conf.conf_target, true,
+                             as.workset.dg_scan);
+          scan_exclusive_inplace(as.workset.dg_scan, N, as.workset.dg_bsum);
+          hipLaunchKernelGGL(k_pull_strict<Bd>, dim3(MAX_GRID), dim3(BLOCK), 0,
+                             st, body, (const int64_t *)as.workset.dg_scan);
+          break;
+        }
+      }
+      H_ERR(hipGetLastError());
+    }
+  }
+}

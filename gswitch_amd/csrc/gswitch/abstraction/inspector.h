@@ -1,0 +1,142 @@
+// inspector_t: the per-level "inspect" actor.
+// Follows inspector_t (src/abstraction/inspector.cuh:112-244): skip the pass
+// in fused mode and in the first round (features are synthesised from the
+// root), otherwise run the inspector kernel, read the six counters back
+// (one pinned copy + one stream sync) and turn them into features; declare
+// convergence when nothing is active (or, for idempotent algorithms, when
+// nothing is left inactive).
+#pragma once
+#include <chrono>
+
+#include "../data_structures/active_set.h"
+#include "../kernels/inspect.h"
+#include "features.h"
+#include "statistics.h"
+
+inline double gsw_now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch())
+      .count();
+}
+
+struct inspector_t {
+  // raw counts of the current frontier, used to size queues without a sync
+  long long known_active = 0, known_inactive = 0;
+
+  template <typename E, typename F>
+  void inspect(active_set_t &as, device_graph_t<CSR, E> &g, F &f,
+               stat_t &stat, feature_t &fets, config_t &conf) {
+    stat.last_filter_time = gsw_now_ms();
+    f.data.window.adjust(fets.active_vertex, fets.cur_avg_deg_active,
+                         conf.conf_fusion);
+    bool need = !conf.conf_fuse_inspect;
+    init_fets(fets, (double)g.nvertexs, (double)g.nedges);
+    if (need && conf.conf_first_round) {
+      set_first_fets(as, fets, conf, (double)g.nvertexs, (double)g.nedges);
+      need = false;
+    }
+    if (need) {
+      stat.clean();
+      hipLaunchKernelGGL((gsw::k_inspect_vc<device_graph_t<CSR, E>, F>),
+                         dim3(gsw::grid_for((g.nvertexs + 63) / 64 * 64)),
+                         dim3(gsw::BLOCK), 0, gsw::stream(), as, g, f,
+                         stat.dg_runtime_info);
+      H_ERR(hipGetLastError());
+      stat.collect();
+      set_fets(as, stat, fets, conf, (double)g.nvertexs, (double)g.nedges);
+    }
+    fets.flatten();
+  }
+
+  template <typename E, typename F>
+  void inspect(active_set_t &as, device_graph_t<COO, E> &g, F &f,
+               stat_t &stat, feature_t &fets, config_t &conf) {
+    stat.last_filter_time = gsw_now_ms();
+    init_fets(fets, (double)g.nvertexs, (double)g.nedges);
+    if (conf.conf_first_round) {
+      conf.conf_first_round = false;
+      fets.active_vertex = (double)g.nedges;
+      fets.active_vertex_ratio = 1;
+      known_active = g.nedges;
+      known_inactive = 0;
+      return;
+    }
+    stat.clean();
+    hipLaunchKernelGGL((gsw::k_inspect_ec<device_graph_t<COO, E>, F>),
+                       dim3(gsw::grid_for((g.nedges + 63) / 64 * 64)),
+                       dim3(gsw::BLOCK), 0, gsw::stream(), as, g, f,
+                       stat.dg_runtime_info);
+    H_ERR(hipGetLastError());
+    stat.collect();
+    set_fets(as, stat, fets, conf, (double)g.nvertexs, (double)g.nedges);
+    fets.flatten();
+  }
+
+  void init_fets(feature_t &fets, double n, double m) {
+    fets.active_vertex = 0;
+    fets.active_vertex_ratio = 0;
+    fets.unactive_vertex = n;
+    fets.unactive_vertex_ratio = 1;
+    fets.growing_rate = fets.push_workload;
+    fets.push_workload = 0;
+    fets.pull_workload = m;
+    fets.push_workload_ratio = 0;
+    fets.pull_workload_ratio = 1;
+    fets.cur_avg_deg_active = 0;
+    fets.cur_avg_deg_unactive = fets.avg_deg;
+  }
+
+  void set_first_fets(active_set_t &as, feature_t &fets, const config_t &conf,
+                      double n, double m) {
+    if (fets.use_root < 0) {
+      fets.active_vertex = n;
+      fets.unactive_vertex = 0;
+      fets.push_workload = m;
+      fets.pull_workload = 0;
+    } else {
+      fets.active_vertex = 1;
+      fets.unactive_vertex = n - 1;
+      fets.push_workload = fets.first_workload;
+      fets.pull_workload = m - fets.first_workload;
+    }
+    known_active = (long long)fets.active_vertex;
+    known_inactive = (long long)fets.unactive_vertex;
+    fets.active_vertex_ratio = DIV(fets.active_vertex, n);
+    fets.unactive_vertex_ratio = DIV(fets.unactive_vertex, n);
+    fets.push_workload_ratio = DIV(fets.push_workload, m);
+    fets.pull_workload_ratio = DIV(fets.pull_workload, m);
+    fets.cur_avg_deg_active = DIV(fets.push_workload, fets.active_vertex);
+    fets.cur_avg_deg_unactive = DIV(fets.pull_workload, fets.unactive_vertex);
+    fets.cur_max_deg_active = fets.push_workload;
+    fets.cur_max_deg_unactive = fets.avg_deg;
+  }
+
+  void set_fets(active_set_t &as, stat_t &stat, feature_t &fets,
+                const config_t &conf, double n, double m) {
+    known_active = stat.active_num();
+    known_inactive = stat.unactive_num();
+    fets.active_vertex = (double)stat.active_num();
+    fets.unactive_vertex = (double)stat.unactive_num();
+    fets.active_vertex_ratio = DIV(fets.active_vertex, n);
+    fets.unactive_vertex_ratio = DIV(fets.unactive_vertex, n);
+    bool done = fets.active_vertex <= 0 ||
+                (fets.pattern == Idem && fets.unactive_vertex <= 0);
+    if (done) {
+      if (conf.conf_qmode == Cached) {
+        // keep the Cached cursor where a fused run would leave it
+        as.queue.host_sizes[as.queue.cursor + 1] = 0;
+        as.swap_queues();
+      }
+      as.halt_host();
+    }
+    fets.growing_rate = DIV((double)stat.push_workload(), fets.growing_rate);
+    fets.push_workload = (double)stat.push_workload();
+    fets.pull_workload = (double)stat.pull_workload();
+    fets.push_workload_ratio = DIV(fets.push_workload, m);
+    fets.pull_workload_ratio = DIV(fets.pull_workload, m);
+    fets.cur_avg_deg_active = DIV(fets.push_workload, fets.active_vertex);
+    fets.cur_avg_deg_unactive = DIV(fets.pull_workload, fets.unactive_vertex);
+    fets.cur_max_deg_active = (double)stat.max_deg_active();
+    fets.cur_max_deg_unactive = (double)stat.max_deg_unactive();
+  }
+};

@@ -1,0 +1,200 @@
+// selector_t: the rule-based per-level variant choice.
+//
+// Decision logic follows selector_t (src/abstraction/selector.cuh:20-304):
+// fusion triggers for idempotent algorithms (potential < 0.001 or three idle
+// levels), fused levels run Push/Queue/WM, otherwise direction, frontier
+// format and load balance come from the trained trees (now node tables,
+// model/trees_table.h) on the same feature ratios, followed by the forced
+// overrides. Class -> LB mapping is the reference's: 0 -> WM, 1 -> CM,
+// >= 2 -> STRICT (selector.cuh:126-131). The frontier target status follows
+// from (direction, from-all, to-all) as at selector.cuh:292-302.
+#pragma once
+#include "../model/trees_table.h"
+#include "config.h"
+#include "features.h"
+#include "statistics.h"
+
+struct selector_t {
+  void select(const stat_t &stats, feature_t &fets, config_t &conf) {
+    select_based_on_model(stats, fets, conf);
+  }
+
+  static int tree(const gsw_model::tree_t &t, const double *fv) {
+    return gsw_model::eval_tree(t, fv);
+  }
+
+  void trigger_fusion(const feature_t &fets, config_t &conf) {
+    if (!conf.conf_fuse_inspect) {
+      const double potential = fets.push_workload_ratio + fets.pull_workload_ratio;
+      if (potential < 0.001) {
+        conf.conf_fuse_inspect = true;
+        conf.conf_switch_to_fusion = true;
+      }
+      if (fets.growing_rate > 2) conf.conf_idle = 0;
+      if (conf.conf_idle >= 3) {
+        conf.conf_fuse_inspect = true;
+        conf.conf_switch_to_fusion = true;
+        conf.conf_idle = 0;
+      }
+    }
+  }
+
+  void fusion(const feature_t &fets, config_t &conf) {
+    conf.conf_dir = Push;
+    conf.conf_asfmt = Queue;
+    conf.conf_cm = Interleave;
+    conf.conf_lb = WM;
+    conf.conf_fuse_inspect = true;
+    conf.conf_centric = fets.centric;
+    conf.conf_fromall = fets.fromall;
+    conf.conf_toall = fets.toall;
+  }
+
+  static void dir_features(const feature_t &fets, double *v) {
+    const double *fv = fets.fv;
+    v[0] = fv[8] / (fv[7] + 0.1);
+    v[1] = fv[10] / (fv[9] + 0.1);
+    v[2] = fv[16] / (fv[15] + 0.1);
+    v[3] = fv[18] / (fv[17] + 0.1);
+    v[4] = fv[9] / (fv[15] + 0.1);
+    v[5] = fv[10] / (fv[16] + 0.1);
+    v[6] = fv[9] / (fv[17] + 0.1);
+    v[7] = fv[10] / (fv[18] + 0.1);
+  }
+
+  static LB lb_of_class(int l) { return l == 0 ? WM : (l == 1 ? CM : STRICT); }
+
+  void select_idem(feature_t &fets, config_t &conf) {
+    using namespace gsw_model;
+    double v[16];
+    dir_features(fets, v);
+    conf.conf_dir = tree(k_select_dir_idem, v) == 1 ? Pull : Push;
+    if (fets.push_workload_ratio * 5 < fets.pull_workload_ratio) conf.conf_dir = Push;
+    if (conf.force.has_direction) conf.conf_dir = conf.force.d;
+
+    if (conf.conf_dir == Pull) {
+      conf.conf_asfmt = Bitmap;
+      if (fets.fv[12] < 0.01) conf.conf_asfmt = Queue;
+    } else {
+      v[0] = fets.fv[11];
+      conf.conf_asfmt = tree(k_select_fmt, v) ? Bitmap : Queue;
+    }
+    if (conf.force.has_asfmt) conf.conf_asfmt = conf.force.asf;
+
+    if (conf.conf_dir == Pull) {
+      conf.conf_lb = TM;
+    } else {
+      int k = 0;
+      for (int i = 0; i < 7; ++i) v[k++] = fets.fv[i];
+      for (int i = 7; i < 18; i += 2) v[k++] = fets.fv[i];
+      conf.conf_lb = lb_of_class(tree(k_select_lb_idem, v));
+    }
+    if (conf.force.has_lb) conf.conf_lb = conf.force.lb;
+  }
+
+  void select_ASSO(feature_t &fets, config_t &conf) {
+    using namespace gsw_model;
+    double v[16];
+    dir_features(fets, v);
+    conf.conf_dir = tree(k_select_dir_asso, v) == 1 ? Pull : Push;
+    if (conf.force.has_direction) conf.conf_dir = conf.force.d;
+    if (conf.conf_fromall && conf.conf_toall) {
+      for (int i = 0; i < 7; ++i) v[i] = fets.fv[i];
+      conf.conf_dir = tree(k_select_dir_asso_backup, v) ? Pull : Push;
+      if (conf.force.has_direction) conf.conf_dir = conf.force.d;
+    }
+
+    v[0] = (conf.conf_dir == Push) ? fets.fv[11] : fets.fv[12];
+    conf.conf_asfmt = tree(k_select_fmt, v) ? Bitmap : Queue;
+    if (fets.fromall && fets.toall && !conf.conf_fixed) conf.conf_asfmt = Bitmap;
+    if (conf.force.has_asfmt) conf.conf_asfmt = conf.force.asf;
+
+    int k = 0;
+    for (int i = 0; i < 7; ++i) v[k++] = fets.fv[i];
+    int l;
+    const bool both = conf.conf_fromall && conf.conf_toall && !conf.conf_fixed;
+    if (conf.conf_dir == Pull) {
+      if (both) {
+        l = tree(k_select_lb_asso_backward_backup, v);
+      } else {
+        for (int i = 8; i <= 18; i += 2) v[k++] = fets.fv[i];
+        l = tree(k_select_lb_asso_backward, v);
+      }
+    } else {
+      if (both) {
+        l = tree(k_select_lb_asso_backward_backup, v);
+      } else {
+        for (int i = 7; i < 18; i += 2) v[k++] = fets.fv[i];
+        l = tree(k_select_lb_asso_forward, v);
+      }
+    }
+    conf.conf_lb = lb_of_class(l);
+    if (conf.force.has_lb) conf.conf_lb = conf.force.lb;
+  }
+
+  void select_Mono(feature_t &fets, config_t &conf) {
+    using namespace gsw_model;
+    conf.conf_dir = Push;
+    if (conf.force.has_direction) conf.conf_dir = conf.force.d;
+    conf.conf_asfmt = Queue;
+    if (conf.force.has_asfmt) conf.conf_asfmt = conf.force.asf;
+    double v[16];
+    int k = 0;
+    for (int i = 0; i < 7; ++i) v[k++] = fets.fv[i];
+    for (int i = 7; i < 18; i += 2) v[k++] = fets.fv[i];
+    conf.conf_lb = lb_of_class(tree(k_select_lb_asso_forward, v));
+    if (conf.force.has_lb) conf.conf_lb = conf.force.lb;
+  }
+
+  static void set_target(config_t &conf) {
+    if (conf.conf_dir == Push)
+      conf.conf_target = conf.conf_fromall ? All : Active;
+    else
+      conf.conf_target = conf.conf_toall ? All : Inactive;
+  }
+
+  void select_based_on_model(const stat_t &stats, feature_t &fets,
+                             config_t &conf) {
+    if (fets.centric == EC) {
+      conf.conf_target = Active;
+      conf.conf_asfmt = (fets.active_vertex_ratio > 0.8) ? Bitmap : Queue;
+      if (conf.force.has_asfmt) conf.conf_asfmt = conf.force.asf;
+      conf.conf_fuse_inspect = false;
+      return;
+    }
+    // from-all & to-all: keep the first configuration for every level
+    if (conf.conf_toall && conf.conf_fromall && !conf.conf_fixed) {
+      conf.conf_inherit = true;
+      return;
+    }
+    conf.conf_centric = fets.centric;
+    conf.conf_fromall = fets.fromall;
+    conf.conf_toall = fets.toall;
+
+    if (fets.pattern == Idem)
+      trigger_fusion(fets, conf);
+    else
+      conf.conf_fuse_inspect = conf.conf_fusion;
+    if (conf.force.has_fusion) {
+      const bool want = conf.force.fusion == Fused;
+      if (want && !conf.conf_fuse_inspect) conf.conf_switch_to_fusion = true;
+      conf.conf_fuse_inspect = want;
+    }
+    if (conf.conf_fuse_inspect) {
+      fusion(fets, conf);
+      // forced variants may pin the LB of fused levels too
+      if (conf.force.has_lb) conf.conf_lb = conf.force.lb;
+      if (conf.force.has_sorted) conf.conf_sorted = conf.force.sorted;
+      set_target(conf);
+      return;
+    }
+    if (fets.pattern == Idem)
+      select_idem(fets, conf);
+    else if (fets.pattern == ASSO)
+      select_ASSO(fets, conf);
+    else
+      select_Mono(fets, conf);
+    if (conf.force.has_sorted) conf.conf_sorted = conf.force.sorted;
+    set_target(conf);
+  }
+};

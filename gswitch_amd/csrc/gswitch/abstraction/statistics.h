@@ -1,0 +1,62 @@
+// Inspector counters (stat_t, src/abstraction/statistics.cuh:9-55).
+// Six device counters reduced by the inspector: active, inactive,
+// push_workload (sum of out-degrees of active vertices), pull_workload (sum
+// of in-degrees of inactive ones), max degree of an active / an inactive
+// vertex. Counters are int64 (the reference's int32 overflows past 2^31
+// scanned edges) and the maxima are true maxima (the reference block-sums
+// them, inspector.cuh:101-109).
+#pragma once
+#include "../utils/runtime.h"
+
+struct stat_t {
+  enum { N = 8 };
+  long long *dg_runtime_info = nullptr;
+  long long *h_runtime_info = nullptr;
+  int size = 6;
+
+  // history (wall-clock ms per phase)
+  double last_time = 0, avg_time = 0;
+  double last_filter_time = 0, last_expand_time = 0;
+  double avg_filter_time = 0, avg_expand_time = 0;
+  int level = 0;
+
+  void build(int _size = 6) {
+    size = _size;
+    if (!dg_runtime_info) {
+      H_ERR(hipMalloc((void **)&dg_runtime_info, sizeof(long long) * N));
+      H_ERR(hipHostMalloc((void **)&h_runtime_info, sizeof(long long) * N));
+    }
+    for (int i = 0; i < N; ++i) h_runtime_info[i] = 0;
+    clean();
+  }
+  void release() {
+    if (dg_runtime_info) hipFree(dg_runtime_info);
+    if (h_runtime_info) hipHostFree(h_runtime_info);
+    dg_runtime_info = nullptr;
+    h_runtime_info = nullptr;
+  }
+  void reset() {
+    last_time = avg_time = 0;
+    last_filter_time = last_expand_time = 0;
+    avg_filter_time = avg_expand_time = 0;
+    level = 0;
+  }
+  // enqueue the device->host copy; the caller synchronises
+  void collect_async(int num = N) {
+    H_ERR(hipMemcpyAsync(h_runtime_info, dg_runtime_info,
+                         sizeof(long long) * num, hipMemcpyDeviceToHost,
+                         gsw::stream()));
+  }
+  void collect(int num = N) {
+    collect_async(num);
+    gsw::sync();
+  }
+  void clean() { gsw::fill_async<long long>(dg_runtime_info, 0, N); }
+
+  long long active_num() const { return h_runtime_info[0]; }
+  long long unactive_num() const { return h_runtime_info[1]; }
+  long long push_workload() const { return h_runtime_info[2]; }
+  long long pull_workload() const { return h_runtime_info[3]; }
+  long long max_deg_active() const { return h_runtime_info[4]; }
+  long long max_deg_unactive() const { return h_runtime_info[5]; }
+};

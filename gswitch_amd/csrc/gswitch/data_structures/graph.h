@@ -1,0 +1,189 @@
+// Graphs on the device.
+//
+// device_graph_t<CSR, E> keeps the reference's names and accessors
+// (src/data_structures/graph.cuh:44-175: dg_adj_list, dg_edgedata,
+// get_out_degree, get_in_degree, get_in_start_pos, get_level, update_level)
+// but stores int64 offsets row_ptr[n+1] instead of int32 start_pos[n] +
+// odegree[n] texture objects; degrees come from adjacent offsets.
+// device_graph_t<COO, E> is the edge-centric form (graph.cuh:12-41): one
+// packed (v0 << 32 | v1) per undirected edge.
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <type_traits>
+#include <vector>
+
+#include "../../host/host_graph.h"
+#include "../utils/common.h"
+
+typedef int64_t packed_t;
+
+// Degree statistics used as selector features. Restates
+// graph_attr_t::fill (src/utils/fileIO.cuh:35-67): mean, sample standard
+// deviation, range and max of the out-degrees, Gini index GI and normalised
+// degree entropy Her over the ascending-sorted degrees.
+struct graph_attr_t {
+  double avg_deg = 0, std_deg = 0, max_deg = 0, range = 0, GI = 0, Her = 0;
+
+  void fill(const int64_t *row_ptr, int64_t nv) {
+    if (nv <= 0) return;
+    std::vector<double> d((size_t)nv);
+    for (int64_t i = 0; i < nv; ++i) d[i] = (double)(row_ptr[i + 1] - row_ptr[i]);
+    std::sort(d.begin(), d.end());
+    double s = 0, s2 = 0;
+    for (double x : d) {
+      s += x;
+      s2 += x * x;
+    }
+    std_deg = nv > 1 ? std::sqrt((s2 - s * s / nv) / (nv - 1)) : 0.0;
+    avg_deg = s / nv;
+    range = d[nv - 1] - d[0];
+    max_deg = d[nv - 1];
+    double gi = 0, her = 0;
+    for (int64_t i = 1; i <= nv; ++i) {
+      double du = d[i - 1];
+      if (s > 0) gi += (nv - i + 0.5) * du / nv / s;
+      if (du != 0) her += du * (std::log(s) - std::log(du)) / s;
+    }
+    GI = 1 - 2 * gi;
+    Her = nv > 1 ? her / std::log((double)nv) : 0.0;
+  }
+};
+
+template <GraphFmt fmt, typename E> struct device_graph_t {};
+
+template <typename E> struct device_graph_t<CSR, E> {
+  typedef E edge_t;
+  static constexpr bool has_edata = !std::is_same<E, Empty>::value;
+
+  int64_t nvertexs = 0;
+  int64_t nedges = 0;  // directed CSR entries
+  int level = 0;
+  bool directed = false;
+
+  int64_t *dg_row_ptr = nullptr;
+  int32_t *dg_adj_list = nullptr;
+  E *dg_edgedata = nullptr;
+  // reversed graph (directed only; aliases the forward arrays otherwise)
+  int64_t *dgr_row_ptr = nullptr;
+  int32_t *dgr_adj_list = nullptr;
+  E *dgr_edgedata = nullptr;
+
+  // Upload a host CSR. `w` is converted to E when E is not Empty.
+  int64_t build(const gsw_host::HostGraph &hg) {
+    nvertexs = hg.n;
+    nedges = hg.m;
+    directed = hg.directed;
+    level = 0;
+    int64_t bytes = 0;
+    H_ERR(hipMalloc((void **)&dg_row_ptr, sizeof(int64_t) * (nvertexs + 1)));
+    H_ERR(hipMemcpy(dg_row_ptr, hg.row_ptr.data(),
+                    sizeof(int64_t) * (nvertexs + 1), hipMemcpyHostToDevice));
+    H_ERR(hipMalloc((void **)&dg_adj_list, sizeof(int32_t) * (nedges + 1)));
+    if (nedges)
+      H_ERR(hipMemcpy(dg_adj_list, hg.col.data(), sizeof(int32_t) * nedges,
+                      hipMemcpyHostToDevice));
+    bytes += sizeof(int64_t) * (nvertexs + 1) + sizeof(int32_t) * nedges;
+    if (has_edata) {
+      GSW_REQUIRE(hg.weighted, gsw::E_INVALID, "graph has no edge weights");
+      bytes += upload_w(hg.w, &dg_edgedata);
+    }
+    if (directed) {
+      H_ERR(hipMalloc((void **)&dgr_row_ptr, sizeof(int64_t) * (nvertexs + 1)));
+      H_ERR(hipMemcpy(dgr_row_ptr, hg.r_row_ptr.data(),
+                      sizeof(int64_t) * (nvertexs + 1), hipMemcpyHostToDevice));
+      H_ERR(hipMalloc((void **)&dgr_adj_list, sizeof(int32_t) * (nedges + 1)));
+      if (nedges)
+        H_ERR(hipMemcpy(dgr_adj_list, hg.r_col.data(), sizeof(int32_t) * nedges,
+                        hipMemcpyHostToDevice));
+      bytes += sizeof(int64_t) * (nvertexs + 1) + sizeof(int32_t) * nedges;
+      if (has_edata) bytes += upload_w(hg.r_w, &dgr_edgedata);
+    } else {
+      dgr_row_ptr = dg_row_ptr;
+      dgr_adj_list = dg_adj_list;
+      dgr_edgedata = dg_edgedata;
+    }
+    return bytes;
+  }
+
+  void release() {
+    if (directed) {
+      hipFree(dgr_row_ptr);
+      hipFree(dgr_adj_list);
+      if (has_edata) hipFree(dgr_edgedata);
+    }
+    hipFree(dg_row_ptr);
+    hipFree(dg_adj_list);
+    if (has_edata) hipFree(dg_edgedata);
+    dg_row_ptr = dgr_row_ptr = nullptr;
+    dg_adj_list = dgr_adj_list = nullptr;
+    dg_edgedata = dgr_edgedata = nullptr;
+  }
+
+  __device__ __forceinline__ int64_t get_start_pos(int v) const {
+    return dg_row_ptr[v];
+  }
+  __device__ __forceinline__ int get_out_degree(int v) const {
+    return (int)(dg_row_ptr[v + 1] - dg_row_ptr[v]);
+  }
+  __device__ __forceinline__ int get_in_degree(int v) const {
+    return (int)(dgr_row_ptr[v + 1] - dgr_row_ptr[v]);
+  }
+  __device__ __forceinline__ int64_t get_in_start_pos(int v) const {
+    return dgr_row_ptr[v];
+  }
+  __device__ __forceinline__ E *fetch_edata(int64_t eid) const {
+    if constexpr (has_edata) return dg_edgedata + eid;
+    else return nullptr;
+  }
+  __device__ __forceinline__ E *fetch_edata_r(int64_t eid) const {
+    if constexpr (has_edata) return dgr_edgedata + eid;
+    else return nullptr;
+  }
+  __host__ __device__ __forceinline__ int get_level() const { return level; }
+  __host__ __device__ inline void update_level(int inc = 1) { level += inc; }
+  __host__ __device__ inline void reset_level() { level = 0; }
+
+ private:
+  int64_t upload_w(const std::vector<uint32_t> &w, E **dst) {
+    std::vector<E> tmp(w.size());
+    for (size_t i = 0; i < w.size(); ++i) tmp[i] = (E)w[i];
+    H_ERR(hipMalloc((void **)dst, sizeof(E) * (tmp.size() + 1)));
+    if (!tmp.empty())
+      H_ERR(hipMemcpy(*dst, tmp.data(), sizeof(E) * tmp.size(),
+                      hipMemcpyHostToDevice));
+    return (int64_t)(sizeof(E) * tmp.size());
+  }
+};
+
+template <typename E> struct device_graph_t<COO, E> {
+  typedef E edge_t;
+  int64_t nvertexs = 0;
+  int64_t nedges = 0;  // undirected edges, one packed pair each
+  int level = 0;
+  packed_t *dg_coo = nullptr;
+
+  // One packed (u << 32 | v), u < v, per undirected edge of the CSR.
+  int64_t build(const gsw_host::HostGraph &hg) {
+    nvertexs = hg.n;
+    std::vector<packed_t> pk;
+    pk.reserve((size_t)hg.m / 2 + 1);
+    for (int64_t u = 0; u < hg.n; ++u)
+      for (int64_t j = hg.row_ptr[u]; j < hg.row_ptr[u + 1]; ++j)
+        if (hg.directed || hg.col[j] > u)
+          pk.push_back(((packed_t)u << 32) | (uint32_t)hg.col[j]);
+    nedges = (int64_t)pk.size();
+    H_ERR(hipMalloc((void **)&dg_coo, sizeof(packed_t) * (nedges + 1)));
+    if (nedges)
+      H_ERR(hipMemcpy(dg_coo, pk.data(), sizeof(packed_t) * nedges,
+                      hipMemcpyHostToDevice));
+    return (int64_t)sizeof(packed_t) * nedges;
+  }
+  void release() {
+    hipFree(dg_coo);
+    dg_coo = nullptr;
+  }
+  __host__ __device__ __forceinline__ int get_level() const { return level; }
+  __host__ __device__ inline void update_level(int inc = 1) { level += inc; }
+  __host__ __device__ inline void reset_level() { level = 0; }
+};

@@ -1,0 +1,303 @@
+// Pull (bottom-up) expand kernels.
+//
+// Edge body as in the reference's __rexpand_VC_* kernels
+// (src/kernel_libs/expand_VC_{TM,WM,CM,STRICT}.cuh): for a target v and an
+// in-edge u -> v, if the source is active (or the algorithm is from-all) the
+// message emit(u, e) is combined with comp; idempotent algorithms (pruning)
+// stop at the first contributing in-edge (early exit).
+//   TM     thread per target, serial in-edge loop, combined in registers
+//   WM     wavefront per 64 targets, 64 in-edges per step; messages of one
+//          target are merged by a segmented shuffle reduction and kept in LDS
+//          until the wavefront moves on, then applied with comp
+//   CM     workgroup per 256 targets; per-wavefront segments are applied with
+//          compAtomic (a target may span wavefronts)
+//   STRICT edge-balanced over the targets' in-edges (same partition as the
+//          push STRICT); segments applied with compAtomic; early exit through
+//          the visited bitmap
+#pragma once
+#include "expand_push.h"
+
+namespace gsw {
+
+template <typename T> __device__ __forceinline__ T shfl_down_any(T x, int d) {
+  constexpr int W = (int)((sizeof(T) + 3) / 4);
+  union U {
+    T t;
+    int i[W];
+    __device__ U() {}
+  } a, b;
+  a.t = x;
+#pragma unroll
+  for (int k = 0; k < W; ++k) b.i[k] = __shfl_down(a.i[k], d, 64);
+  return b.t;
+}
+
+template <typename G, typename F> struct pull_body {
+  using WA = typename F::wa_t;
+  active_set_t as;
+  G g;
+  F f;
+  config_t conf;
+
+  __device__ __forceinline__ const int64_t *rows() const { return g.dgr_row_ptr; }
+  __device__ __forceinline__ const int32_t *cols() const { return g.dgr_adj_list; }
+  __device__ __forceinline__ bool source_ok(int u) const {
+    return conf.conf_fromall || as.bitmap.is_active(u);
+  }
+  __device__ __forceinline__ WA msg(int u, int64_t e) {
+    return f.emit(u, g.fetch_edata_r(e), g);
+  }
+  // Segmented reduction over lanes whose owner ids are non-decreasing:
+  // afterwards the first lane of each owner run holds the combined message.
+  __device__ __forceinline__ void seg_reduce(int owner, bool &vote, WA &m) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+      const WA om = shfl_down_any(m, d);
+      const int oo = __shfl_down(owner, d, 64);
+      const int ov = __shfl_down((int)vote, d, 64);
+      if (lane + d < WAVE && oo == owner && ov) {
+        if (vote)
+          f.comp(&m, om, g);
+        else
+          m = om;
+        vote = true;
+      }
+    }
+  }
+  __device__ __forceinline__ bool seg_head(int owner) const {
+    const int prev = __shfl_up(owner, 1, 64);
+    return owner >= 0 && (lane_id() == 0 || prev != owner);
+  }
+};
+
+// ---------------------------------------------------------------- TM
+template <class B>
+__global__ void __launch_bounds__(BLOCK) k_pull_tm(B body) {
+  using WA = typename B::WA;
+  const int64_t N = body.as.get_size();
+  const Status want = body.conf.want();
+  const bool prune = body.conf.conf_pruning;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  unsigned long long ne = 0, nv = 0;
+  for (int64_t idx = tid; idx < N; idx += stride) {
+    const int v = body.as.fetch(idx, want);
+    if (v < 0) continue;
+    ++nv;
+    const int64_t s = body.rows()[v], e = body.rows()[v + 1];
+    WA acc;
+    bool any = false;
+    int64_t j = s;
+    for (; j < e; ++j) {
+      const int u = body.cols()[j];
+      if (!body.source_ok(u)) continue;
+      const WA m = body.msg(u, j);
+      if (!any) {
+        acc = *body.f.wa_of(v);
+        any = true;
+      }
+      body.f.comp(&acc, m, body.g);
+      if (prune) {
+        ++j;
+        break;
+      }
+    }
+    ne += j - s;
+    if (any) *body.f.wa_of(v) = acc;
+  }
+  add_work(body.as.dg_work, ne, nv);
+}
+
+// ---------------------------------------------------------------- WM
+template <class B>
+__global__ void __launch_bounds__(BLOCK) k_pull_wm(B body) {
+  using WA = typename B::WA;
+  __shared__ int s_v[BLOCK];
+  __shared__ int s_scan[BLOCK];
+  __shared__ int s_flag[BLOCK];  // bit0: s_acc holds a message, bit1: done
+  __shared__ int64_t s_start[BLOCK];
+  __shared__ WA s_acc[BLOCK];
+  const int lane = lane_id();
+  const int off = threadIdx.x & ~(WAVE - 1);
+  const int64_t N = body.as.get_size();
+  const Status want = body.conf.want();
+  const bool prune = body.conf.conf_pruning;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> LOG_WAVE;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> LOG_WAVE;
+  unsigned long long ne = 0, nv = 0;
+  for (int64_t base = wave << 6; base < N; base += nwaves << 6) {
+    const int64_t idx = base + lane;
+    const int v = (idx < N) ? body.as.fetch(idx, want) : -1;
+    int64_t start = 0;
+    int deg = 0;
+    if (v >= 0) {
+      start = body.rows()[v];
+      deg = (int)(body.rows()[v + 1] - start);
+      ++nv;
+    }
+    int tot;
+    const int ex = wave_exclusive_scan(deg, tot);
+    s_v[off + lane] = v;
+    s_scan[off + lane] = ex;
+    s_start[off + lane] = start;
+    s_flag[off + lane] = 0;
+    wave_sync();
+    for (int j0 = 0; j0 < tot; j0 += WAVE) {
+      const int j = j0 + lane;
+      int o = -1;
+      bool vote = false;
+      WA m{};
+      if (j < tot) {
+        o = owner_search(&s_scan[off], WAVE, j);
+        if (!(prune && (s_flag[off + o] & 2))) {
+          const int64_t e = s_start[off + o] + (j - s_scan[off + o]);
+          const int u = body.cols()[e];
+          ++ne;
+          if (body.source_ok(u)) {
+            vote = true;
+            m = body.msg(u, e);
+          }
+        }
+      }
+      body.seg_reduce(o, vote, m);
+      if (body.seg_head(o) && vote) {
+        const int k = off + o;
+        if (s_flag[k] & 1)
+          body.f.comp(&s_acc[k], m, body.g);
+        else
+          s_acc[k] = m;
+        s_flag[k] |= prune ? 3 : 1;
+      }
+      wave_sync();
+      if (prune) {
+        // every target of this wavefront already found a parent
+        const bool open = (v >= 0) && deg > 0 && !(s_flag[off + lane] & 2);
+        if (ballot(open) == 0) break;
+      }
+    }
+    if (v >= 0 && (s_flag[off + lane] & 1)) {
+      WA acc = *body.f.wa_of(v);
+      body.f.comp(&acc, s_acc[off + lane], body.g);
+      *body.f.wa_of(v) = acc;
+    }
+    wave_sync();
+  }
+  add_work(body.as.dg_work, ne, nv);
+}
+
+// ---------------------------------------------------------------- CM
+template <class B>
+__global__ void __launch_bounds__(BLOCK) k_pull_cm(B body) {
+  using WA = typename B::WA;
+  __shared__ int s_v[BLOCK];
+  __shared__ int s_scan[BLOCK];
+  __shared__ int s_done[BLOCK];
+  __shared__ int64_t s_start[BLOCK];
+  __shared__ int s_red[BLOCK / WAVE];
+  const int64_t N = body.as.get_size();
+  const Status want = body.conf.want();
+  const bool prune = body.conf.conf_pruning;
+  unsigned long long ne = 0, nv = 0;
+  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < N;
+       base += (int64_t)gridDim.x * BLOCK) {
+    const int64_t idx = base + threadIdx.x;
+    const int v = (idx < N) ? body.as.fetch(idx, want) : -1;
+    int64_t start = 0;
+    int deg = 0;
+    if (v >= 0) {
+      start = body.rows()[v];
+      deg = (int)(body.rows()[v + 1] - start);
+      ++nv;
+    }
+    int tot;
+    const int ex = block_exclusive_scan(deg, tot, s_red);
+    s_v[threadIdx.x] = v;
+    s_scan[threadIdx.x] = ex;
+    s_start[threadIdx.x] = start;
+    s_done[threadIdx.x] = 0;
+    __syncthreads();
+    for (int j0 = 0; j0 < tot; j0 += BLOCK) {
+      const int j = j0 + threadIdx.x;
+      int o = -1;
+      bool vote = false;
+      WA m{};
+      if (j < tot) {
+        o = owner_search(s_scan, BLOCK, j);
+        if (!(prune && s_done[o])) {
+          const int64_t e = s_start[o] + (j - s_scan[o]);
+          const int u = body.cols()[e];
+          ++ne;
+          if (body.source_ok(u)) {
+            vote = true;
+            m = body.msg(u, e);
+          }
+        }
+      }
+      body.seg_reduce(o, vote, m);
+      if (body.seg_head(o) && vote) {
+        body.f.compAtomic(body.f.wa_of(s_v[o]), m, body.g);
+        if (prune) s_done[o] = 1;
+      }
+      __syncthreads();
+    }
+  }
+  add_work(body.as.dg_work, ne, nv);
+}
+
+// ---------------------------------------------------------------- STRICT
+template <class B>
+__global__ void __launch_bounds__(BLOCK)
+    k_pull_strict(B body, const int64_t *__restrict__ scan) {
+  using WA = typename B::WA;
+  __shared__ int64_t s_scan[BLOCK + 1];
+  __shared__ int64_t s_start[BLOCK + 1];
+  __shared__ int s_v[BLOCK + 1];
+  const int64_t N = body.as.get_size();
+  const Status want = body.conf.want();
+  const bool prune = body.conf.conf_pruning;
+  const int64_t total = scan[N];
+  const int64_t per = (total + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * per;
+  const int64_t hi = MIN(total, lo + per);
+  unsigned long long ne = 0;
+  if (lo >= hi) return;
+  int64_t k0 = last_le(scan, N + 1, lo);
+  for (int64_t c = lo; c < hi; c += BLOCK) {
+    strict_window(body, scan, N, k0, want, body.rows(), s_scan, s_v, s_start);
+    __syncthreads();
+    const int64_t slot = c + threadIdx.x;
+    int o = -1;
+    bool vote = false;
+    WA m{};
+    if (slot < hi) {
+      o = owner_search(s_scan, BLOCK + 1, slot);
+      const int64_t r = slot - s_scan[o];
+      const int64_t deg = s_scan[o + 1] - s_scan[o] - 1;
+      const int v = s_v[o];
+      if (v >= 0 && r < deg && !(prune && body.as.bitmap.visited.test(v))) {
+        const int64_t e = s_start[o] + r;
+        const int u = body.cols()[e];
+        ++ne;
+        if (body.source_ok(u)) {
+          vote = true;
+          m = body.msg(u, e);
+        }
+      }
+    }
+    body.seg_reduce(o, vote, m);
+    if (body.seg_head(o) && vote) {
+      const int v = s_v[o];
+      if (!prune || body.as.bitmap.visited.test_and_set(v))
+        body.f.compAtomic(body.f.wa_of(v), m, body.g);
+    }
+    const int64_t nc = c + BLOCK;
+    const int64_t nk = (nc < s_scan[BLOCK]) ? k0 + owner_search(s_scan, BLOCK + 1, nc)
+                                            : k0 + BLOCK;
+    __syncthreads();
+    k0 = nk;
+  }
+  add_work(body.as.dg_work, ne, 0);
+}
+
+}  // namespace gsw

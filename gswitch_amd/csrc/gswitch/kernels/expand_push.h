@@ -1,0 +1,371 @@
+// Push (top-down) expand kernels.
+//
+// The edge body is the reference's (standalone: __expand_VC_{TM,WM,CM,
+// STRICT}; fused: *_fused / *_wtf in src/kernel_libs/expand_VC_*.cuh):
+//   standalone: [pruning: first visit of u] -> [not to-all: u inactive]
+//               -> compAtomic(wa(u), emit(v, e))
+//   fused:      [not to-all: cond(u, msg)] -> compAtomic -> exact de-dup on
+//               the visited bitmap -> append u to the next queue
+// What differs per load-balancing strategy is only how (v, u, e) triples are
+// spread over lanes. Strategies, rebuilt for 64-lane wavefronts:
+//   TM     thread per frontier vertex, serial edge loop
+//   WM     wavefront per 64 frontier vertices: DPP scan of degrees, LDS-staged
+//          (vertex, offset, prefix), 64 edges per step, owner found by binary
+//          search over the 64 prefixes in LDS
+//   CM     workgroup per 256 frontier vertices, same scheme with a block scan
+//   STRICT edge-balanced: int64 prefix of (deg + 1) over the frontier, every
+//          workgroup owns an equal slice of slots; per 256-slot chunk the 257
+//          covering prefixes are staged in LDS (the +1 guarantees a chunk
+//          never spans more than 257 vertices); the owner of the next chunk
+//          start is found in the current window, so only a workgroup's first
+//          chunk searches global memory (merge-path style)
+//   TWC    degree bins: small -> thread, medium -> wavefront,
+//          large -> workgroup, all three processed in one launch
+#pragma once
+#include "../abstraction/config.h"
+#include "../data_structures/active_set.h"
+#include "../data_structures/functor.h"
+#include "../utils/wave.h"
+#include "scan.h"
+
+namespace gsw {
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// per-wave edge / vertex work counters (one atomic per wavefront)
+__device__ __forceinline__ void add_work(unsigned long long *work,
+                                         unsigned long long edges,
+                                         unsigned long long verts) {
+  edges = wave_sum_any(edges);
+  verts = wave_sum_any(verts);
+  if (lane_id() == 0) {
+    if (edges) atomicAdd(work + WORK_EDGES, edges);
+    if (verts) atomicAdd(work + WORK_VERTS, verts);
+  }
+}
+
+template <typename G, typename F, bool FUSED> struct push_body {
+  static constexpr bool fused = FUSED;
+  active_set_t as;
+  G g;
+  F f;
+  config_t conf;
+
+  __device__ __forceinline__ const int64_t *rows() const { return g.dg_row_ptr; }
+  __device__ __forceinline__ const int32_t *cols() const { return g.dg_adj_list; }
+
+  // true when u must be appended to the next queue (fused only)
+  __device__ __forceinline__ bool operator()(int v, int u, int64_t e) {
+    if constexpr (!FUSED) {
+      bool go = true;
+      if (conf.conf_pruning) go = as.bitmap.mark_duplicate_lite(u);
+      if (go && !conf.conf_toall) go = as.bitmap.is_inactive(u);
+      if (go) {
+        auto msg = f.emit(v, g.fetch_edata(e), g);
+        f.compAtomic(f.wa_of(u), msg, g);
+      }
+      return false;
+    } else {
+      if (conf.conf_pruning && as.bitmap.visited.test(u)) return false;
+      auto msg = f.emit(v, g.fetch_edata(e), g);
+      bool go = true;
+      if (!conf.conf_toall) go = f.cond(u, msg, g);
+      if (go) go = f.compAtomic(f.wa_of(u), msg, g);
+      if (go) go = as.bitmap.mark_duplicate_atomic(u);
+      return go;
+    }
+  }
+
+  __device__ __forceinline__ void emit_q(bool q, int u) {
+    if constexpr (FUSED) as.queue.push(q, u);
+  }
+};
+
+// ---------------------------------------------------------------- TM
+template <class B>
+__global__ void __launch_bounds__(BLOCK) k_push_tm(B body) {
+  const int64_t N = body.as.get_size();
+  const Status want = body.conf.want();
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  unsigned long long ne = 0, nv = 0;
+  for (int64_t idx = tid; idx < N; idx += stride) {
+    const int v = body.as.fetch(idx, want);
+    if (v < 0) continue;
+    const int64_t s = body.rows()[v], e = body.rows()[v + 1];
+    ne += e - s;
+    ++nv;
+    for (int64_t j = s; j < e; ++j) {
+      const int u = body.cols()[j];
+      body.emit_q(body(v, u, j), u);
+    }
+  }
+  add_work(body.as.dg_work, ne, nv);
+}
+
+// ---------------------------------------------------------------- WM
+template <class B>
+__global__ void __launch_bounds__(BLOCK) k_push_wm(B body) {
+  __shared__ int s_v[BLOCK];
+  __shared__ int s_scan[BLOCK];
+  __shared__ int64_t s_start[BLOCK];
+  const int lane = lane_id();
+  const int off = threadIdx.x & ~(WAVE - 1);
+  const int64_t N = body.as.get_size();
+  const Status want = body.conf.want();
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> LOG_WAVE;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> LOG_WAVE;
+  unsigned long long ne = 0, nv = 0;
+  for (int64_t base = wave << 6; base < N; base += nwaves << 6) {
+    const int64_t idx = base + lane;
+    const int v = (idx < N) ? body.as.fetch(idx, want) : -1;
+    int64_t start = 0;
+    int deg = 0;
+    if (v >= 0) {
+      start = body.rows()[v];
+      deg = (int)(body.rows()[v + 1] - start);
+      ++nv;
+    }
+    int tot;
+    const int ex = wave_exclusive_scan(deg, tot);
+    s_v[off + lane] = v;
+    s_scan[off + lane] = ex;
+    s_start[off + lane] = start;
+    wave_sync();
+    ne += deg;
+    for (int j0 = 0; j0 < tot; j0 += WAVE) {
+      const int j = j0 + lane;
+      int u = -1;
+      bool q = false;
+      if (j < tot) {
+        const int o = owner_search(&s_scan[off], WAVE, j);
+        const int64_t e = s_start[off + o] + (j - s_scan[off + o]);
+        u = body.cols()[e];
+        q = body(s_v[off + o], u, e);
+      }
+      body.emit_q(q, u);
+    }
+    wave_sync();
+  }
+  add_work(body.as.dg_work, ne, nv);
+}
+
+// ---------------------------------------------------------------- CM
+template <class B>
+__global__ void __launch_bounds__(BLOCK) k_push_cm(B body) {
+  __shared__ int s_v[BLOCK];
+  __shared__ int s_scan[BLOCK];
+  __shared__ int64_t s_start[BLOCK];
+  __shared__ int s_red[BLOCK / WAVE];
+  const int64_t N = body.as.get_size();
+  const Status want = body.conf.want();
+  unsigned long long ne = 0, nv = 0;
+  for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < N;
+       base += (int64_t)gridDim.x * BLOCK) {
+    const int64_t idx = base + threadIdx.x;
+    const int v = (idx < N) ? body.as.fetch(idx, want) : -1;
+    int64_t start = 0;
+    int deg = 0;
+    if (v >= 0) {
+      start = body.rows()[v];
+      deg = (int)(body.rows()[v + 1] - start);
+      ++nv;
+    }
+    int tot;
+    const int ex = block_exclusive_scan(deg, tot, s_red);
+    s_v[threadIdx.x] = v;
+    s_scan[threadIdx.x] = ex;
+    s_start[threadIdx.x] = start;
+    __syncthreads();
+    ne += deg;
+    for (int j0 = 0; j0 < tot; j0 += BLOCK) {
+      const int j = j0 + threadIdx.x;
+      int u = -1;
+      bool q = false;
+      if (j < tot) {
+        const int o = owner_search(s_scan, BLOCK, j);
+        const int64_t e = s_start[o] + (j - s_scan[o]);
+        u = body.cols()[e];
+        q = body(s_v[o], u, e);
+      }
+      body.emit_q(q, u);
+    }
+    __syncthreads();
+  }
+  add_work(body.as.dg_work, ne, nv);
+}
+
+// ---------------------------------------------------------------- STRICT
+// slot counts (deg + 1) of the frontier items; pull selects in-degrees
+template <typename G>
+__global__ void __launch_bounds__(BLOCK)
+    k_strict_prepare(active_set_t as, G g, Status want, bool pull,
+                     int64_t *d) {
+  const int64_t N = as.get_size();
+  const int64_t *rows = pull ? g.dgr_row_ptr : g.dg_row_ptr;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int v = as.fetch(i, want);
+    d[i] = 1 + ((v >= 0) ? (rows[v + 1] - rows[v]) : 0);
+  }
+}
+
+// Stage the 257 prefixes starting at item k0 (clamped to N) into LDS.
+template <class B>
+__device__ __forceinline__ void strict_window(B &body, const int64_t *scan,
+                                              int64_t N, int64_t k0,
+                                              Status want, const int64_t *rows,
+                                              int64_t *s_scan, int *s_v,
+                                              int64_t *s_start) {
+  for (int t = threadIdx.x; t <= BLOCK; t += blockDim.x) {
+    const int64_t k = k0 + t;
+    s_scan[t] = scan[k < N ? k : N];
+    int v = -1;
+    int64_t st = 0;
+    if (k < N) {
+      v = body.as.fetch(k, want);
+      if (v >= 0) st = rows[v];
+    }
+    s_v[t] = v;
+    s_start[t] = st;
+  }
+}
+
+template <class B>
+__global__ void __launch_bounds__(BLOCK)
+    k_push_strict(B body, const int64_t *__restrict__ scan) {
+  __shared__ int64_t s_scan[BLOCK + 1];
+  __shared__ int64_t s_start[BLOCK + 1];
+  __shared__ int s_v[BLOCK + 1];
+  const int64_t N = body.as.get_size();
+  const Status want = body.conf.want();
+  const int64_t total = scan[N];
+  const int64_t per = (total + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * per;
+  const int64_t hi = MIN(total, lo + per);
+  unsigned long long ne = 0;
+  if (lo >= hi) return;
+  int64_t k0 = last_le(scan, N + 1, lo);
+  for (int64_t c = lo; c < hi; c += BLOCK) {
+    strict_window(body, scan, N, k0, want, body.rows(), s_scan, s_v, s_start);
+    __syncthreads();
+    const int64_t slot = c + threadIdx.x;
+    int u = -1;
+    bool q = false;
+    if (slot < hi) {
+      const int o = owner_search(s_scan, BLOCK + 1, slot);
+      const int64_t r = slot - s_scan[o];
+      const int64_t deg = s_scan[o + 1] - s_scan[o] - 1;
+      const int v = s_v[o];
+      if (v >= 0 && r < deg) {
+        const int64_t e = s_start[o] + r;
+        u = body.cols()[e];
+        q = body(v, u, e);
+        ++ne;
+      }
+    }
+    body.emit_q(q, u);
+    // owner of the next chunk start, from the staged window
+    const int64_t nc = c + BLOCK;
+    const int64_t nk = (nc < s_scan[BLOCK]) ? k0 + owner_search(s_scan, BLOCK + 1, nc)
+                                            : k0 + BLOCK;
+    __syncthreads();
+    k0 = nk;
+  }
+  add_work(body.as.dg_work, ne, 0);
+}
+
+// ---------------------------------------------------------------- TWC
+template <typename G>
+__global__ void __launch_bounds__(BLOCK)
+    k_twc_bin(active_set_t as, G g, Status want, bool pull, int small,
+              int large) {
+  const int64_t N = as.get_size();
+  const int64_t *rows = pull ? g.dgr_row_ptr : g.dg_row_ptr;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i0 = tid - lane_id(); i0 < N; i0 += stride) {
+    const int64_t i = i0 + lane_id();
+    int v = (i < N) ? as.fetch(i, want) : -1;
+    int b = -1;
+    if (v >= 0) {
+      const int64_t d = rows[v + 1] - rows[v];
+      b = (d < small) ? 0 : (d < large) ? 1 : 2;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int slot = wave_append(b == k, as.bins.sizes + k);
+      if (b == k) as.bins.bin[k][slot] = v;
+    }
+  }
+}
+
+template <class B>
+__global__ void __launch_bounds__(BLOCK) k_push_twc(B body) {
+  const int lane = lane_id();
+  const int32_t *sizes = body.as.bins.sizes;
+  unsigned long long ne = 0, nv = 0;
+  // large: one workgroup per vertex
+  const int nl = sizes[2];
+  for (int i = blockIdx.x; i < nl; i += gridDim.x) {
+    const int v = body.as.bins.bin[2][i];
+    const int64_t s = body.rows()[v], e = body.rows()[v + 1];
+    if (threadIdx.x == 0) {
+      ne += e - s;
+      ++nv;
+    }
+    for (int64_t j0 = s; j0 < e; j0 += BLOCK) {
+      const int64_t j = j0 + threadIdx.x;
+      int u = -1;
+      bool q = false;
+      if (j < e) {
+        u = body.cols()[j];
+        q = body(v, u, j);
+      }
+      body.emit_q(q, u);
+    }
+  }
+  // medium: one wavefront per vertex
+  const int nm = sizes[1];
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> LOG_WAVE;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> LOG_WAVE;
+  for (int64_t i = wave; i < nm; i += nwaves) {
+    const int v = body.as.bins.bin[1][i];
+    const int64_t s = body.rows()[v], e = body.rows()[v + 1];
+    if (lane == 0) {
+      ne += e - s;
+      ++nv;
+    }
+    for (int64_t j0 = s; j0 < e; j0 += WAVE) {
+      const int64_t j = j0 + lane;
+      int u = -1;
+      bool q = false;
+      if (j < e) {
+        u = body.cols()[j];
+        q = body(v, u, j);
+      }
+      body.emit_q(q, u);
+    }
+  }
+  // small: one thread per vertex
+  const int ns = sizes[0];
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = tid; i < ns; i += stride) {
+    const int v = body.as.bins.bin[0][i];
+    const int64_t s = body.rows()[v], e = body.rows()[v + 1];
+    ne += e - s;
+    ++nv;
+    for (int64_t j = s; j < e; ++j) {
+      const int u = body.cols()[j];
+      body.emit_q(body(v, u, j), u);
+    }
+  }
+  add_work(body.as.dg_work, ne, nv);
+}
+
+}  // namespace gsw

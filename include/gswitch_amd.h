@@ -1,0 +1,170 @@
+/*
+ * gswitch_amd — C-ABI of the MI355X traversal engine (libgswitch_amd.so).
+ *
+ * The reference exposes a header-only C++ template API and no C-ABI
+ * (its apps #include "gswitch.h" and subclass Functor). This header is the
+ * drop-in boundary for callers that bind over an FFI: plain pointers and
+ * sizes, opaque handles with explicit destroy, a status code on every call
+ * and a thread-local message from gsw_last_error(). Each entry point names
+ * the reference interface it replaces (paths relative to the reference
+ * tree). Host pointers are read/written synchronously; device work is
+ * ordered on the library's own stream. Handles are thread-compatible (one
+ * thread per handle), not thread-safe.
+ */
+#ifndef GSWITCH_AMD_H
+#define GSWITCH_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t gsw_status;
+enum {
+  GSW_OK = 0,
+  GSW_E_INVALID = 1,     /* bad argument (null pointer, id out of range) */
+  GSW_E_HIP = 2,         /* HIP runtime error, or no GPU present */
+  GSW_E_ALLOC = 3,       /* device allocation failed */
+  GSW_E_OVERFLOW = 4,    /* a frontier queue would overflow its capacity */
+  GSW_E_UNSUPPORTED = 5, /* variant or option not available for this call */
+  GSW_E_INTERNAL = 6
+};
+
+typedef struct gsw_host_graph gsw_host_graph; /* host CSR owned by the lib */
+typedef struct gsw_graph gsw_graph;           /* device-resident graph     */
+
+/* ---- library ---------------------------------------------------------- */
+const char *gsw_version(void);
+/* Message of the last failed call on this thread ("" if none). Replaces the
+ * reference's print-and-exit error handling (H_ERR, src/utils/utils.cuh:70-78). */
+const char *gsw_last_error(void);
+/* Number of visible GPUs (0 without a GPU; no other call then succeeds). */
+int32_t gsw_device_count(void);
+
+/* ---- host ingest (src/utils/fileIO.cuh) -------------------------------
+ * CSR build semantics of host_graph_t<CSR>::build + rinse
+ * (src/utils/fileIO.cuh:210-332, 399-443): self-loops dropped, undirected
+ * edges stored both ways, duplicates removed, adjacency sorted; vertex ids
+ * are kept (no isolated-vertex renumbering). Weighted duplicates keep the
+ * smallest weight. */
+gsw_status gsw_host_graph_from_edges(int64_t n, int64_t ne, const int32_t *src,
+                                     const int32_t *dst, const uint32_t *w,
+                                     int32_t directed, gsw_host_graph **out);
+/* Seeded synthetic generators (counter-based SplitMix64 draws). R-MAT:
+ * edge_factor * 2^scale samples with quadrant probabilities (a, b, c,
+ * 1-a-b-c). Weighted graphs get one uint32 weight in [1, 64] per undirected
+ * edge, shared by both directions (as gen_weight, src/utils/fileIO.cuh:148-166,
+ * shared at :288, :295). */
+gsw_status gsw_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b,
+                        double c, uint64_t seed, int32_t weighted,
+                        gsw_host_graph **out);
+gsw_status gsw_gen_grid2d(int64_t rows, int64_t cols, double p_delete,
+                          double long_range_frac, uint64_t seed,
+                          int32_t weighted, gsw_host_graph **out);
+gsw_status gsw_gen_er(int64_t n, double mean_degree, uint64_t seed,
+                      int32_t weighted, gsw_host_graph **out);
+gsw_status gsw_gen_ba(int64_t n, int32_t m, uint64_t seed, int32_t weighted,
+                      gsw_host_graph **out);
+gsw_status gsw_host_graph_info(const gsw_host_graph *g, int64_t *n, int64_t *m,
+                               int32_t *directed, int32_t *weighted);
+/* Copy out row_ptr[n+1], col[m] and (if weighted and w != NULL) w[m]. */
+gsw_status gsw_host_graph_export(const gsw_host_graph *g, int64_t *row_ptr,
+                                 int32_t *col, uint32_t *w);
+void gsw_host_graph_destroy(gsw_host_graph *g);
+/* Degree statistics used as selector features: out[0..5] = avg, std, max,
+ * range, Gini, entropy (graph_attr_t::fill, src/utils/fileIO.cuh:35-67). */
+gsw_status gsw_degree_stats(const int64_t *row_ptr, int64_t n, double *out6);
+
+/* ---- device graph (device_graph_t<CSR>, src/data_structures/graph.cuh:44-175;
+ *      build_graph, graph.cuh:268-280) ----------------------------------- */
+gsw_status gsw_graph_create_csr(const int64_t *row_ptr, const int32_t *col,
+                                const uint32_t *w, int64_t n, int64_t m,
+                                int32_t directed, gsw_graph **out);
+gsw_status gsw_graph_from_host(const gsw_host_graph *hg, gsw_graph **out);
+gsw_status gsw_graph_destroy(gsw_graph *g);
+
+/* ---- runs ---------------------------------------------------------------
+ * Options. `configs` pins the selector's choices like the reference's
+ * --configs=Dir-Fmt-LB-Fusion (src/utils/cmdline.cuh:51-94); any field may
+ * be "x"; Fmt also accepts "SortedQueue", LB also "TWC". */
+typedef struct {
+  const char *configs;
+  int32_t max_levels; /* safety cap on loop iterations */
+  int32_t verbose;    /* per-level log on stderr */
+  int32_t twc_small;  /* TWC bins: degree < twc_small -> thread */
+  int32_t twc_large;  /*           degree < twc_large -> wavefront */
+  int32_t profile;    /* time every expand with hipEvents */
+  int32_t reserved[3];
+} gsw_run_opts;
+void gsw_run_opts_init(gsw_run_opts *o);
+
+typedef struct {
+  double device_ms;      /* hipEvent: frontier init -> results ready */
+  int32_t levels;        /* level-loop iterations */
+  int32_t n_records;     /* expands logged (gsw_last_records) */
+  int64_t reached;       /* vertices with a result (BFS/SSSP), components (CC) */
+} gsw_run_stats;
+
+/* One record per expand launch (the reference's --verbose line,
+ * src/abstraction/executor.cuh:138-146, plus counters). */
+typedef struct {
+  int32_t level;
+  int8_t dir;    /* 0 Push, 1 Pull */
+  int8_t fmt;    /* 0 Queue, 1 Bitmap */
+  int8_t lb;     /* 0 TM, 1 WM, 2 CM, 3 STRICT, 4 TWOD, 5 ELB, 6 TWC */
+  int8_t fused;  /* 1 if inspect+filter were fused into the expand */
+  int8_t sorted; /* queue format: 1 sorted, 0 unsorted */
+  int8_t pad[3];
+  int64_t frontier; /* input frontier length */
+  int64_t active;   /* active vertices (edges for CC) at this level */
+  int64_t inactive;
+  int64_t edges;    /* edges inspected by the expand */
+  int64_t verts;    /* frontier vertices expanded */
+  float ms;         /* expand duration (0 unless profile) */
+  float pad2;
+} gsw_level_record;
+
+/* BFS levels (-1 unreached) from `root`. Replaces run_bfs
+ * (application/bfs.cu:78-102) with the BFS functor of bfs.cu:16-36. */
+gsw_status gsw_bfs(gsw_graph *g, int32_t root, const gsw_run_opts *opts,
+                   int32_t *levels_out, gsw_run_stats *stats);
+/* SSSP distances (UINT32_MAX unreached); needs a weighted graph. Replaces
+ * run_sssp (application/sssp.cu:106-126), functor sssp.cu:17-53, stepping
+ * window src/data_structures/window.cuh:44-123. */
+gsw_status gsw_sssp(gsw_graph *g, int32_t root, const gsw_run_opts *opts,
+                    uint32_t *dist_out, gsw_run_stats *stats);
+/* Connected components, label = smallest vertex id in the component.
+ * Replaces run_cc (application/cc.cu:82-108): edge-centric hooking
+ * (cc.cu:15-32) + pointer jumping (src/utils/intrinsics.cuh:328-362). */
+gsw_status gsw_cc(gsw_graph *g, const gsw_run_opts *opts, int32_t *labels_out,
+                  gsw_run_stats *stats);
+/* PageRank, un-normalised: pr_0 = 1-d, pr_{k+1}(v) = 1-d + d*sum pr_k(u)/deg(u),
+ * `iters` iterations; eps > 0 additionally deactivates vertices whose
+ * relative change is below eps as the reference does. Replaces
+ * run_pagerank (application/pagerank.cu:81-103), functor pagerank.cu:18-42. */
+gsw_status gsw_pagerank(gsw_graph *g, float damping, int32_t iters, float eps,
+                        const gsw_run_opts *opts, float *pr_out,
+                        gsw_run_stats *stats);
+/* Records of the last run on `g`; *n = number copied (<= cap). */
+gsw_status gsw_last_records(const gsw_graph *g, gsw_level_record *out,
+                            int32_t cap, int32_t *n);
+
+/* ---- rule-based selector (src/model/select_*.h) ------------------------
+ * Evaluate one of the reference's trained decision trees by name
+ * ("select_dir_idem", "select_lb_idem", ...) on a feature vector. */
+gsw_status gsw_tree_eval(const char *name, const double *fv, int32_t nfv,
+                         int32_t *cls);
+
+/* ---- primitives (replace moderngpu's mgpu::scan / sorted_search used at
+ *      src/kernel_libs/expand_VC_STRICT.cuh:540-552) --------------------- */
+/* out[i] = sum(in[0..i)), out[n] = total; host buffers, device scan. */
+gsw_status gsw_scan_exclusive_i64(const int64_t *in, int64_t n, int64_t *out);
+/* wave64 DPP inclusive scan of 64 ints (host buffers); diagnostics. */
+gsw_status gsw_wave_scan64(const int32_t *in, int32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSWITCH_AMD_H */

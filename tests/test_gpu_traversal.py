@@ -1,0 +1,161 @@
+"""GPU parity: every traversal variant against the CPU oracle.
+
+BFS levels, SSSP distances and CC labels must be bit-exact whatever variant
+the selector picks or the test forces; PageRank (FP32 output, FP64
+accumulation) must be within 1e-5 max relative error of the FP64 oracle.
+Variants are forced with the reference's --configs syntax
+(Dir-Fmt-LB-Fusion, src/utils/cmdline.cuh:51-94).
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from conftest import csr_from_edges
+
+pytestmark = pytest.mark.gpu
+
+DIRS = ["Push", "Pull"]
+FMTS = ["Queue", "SortedQueue", "Bitmap"]
+LBS = ["TM", "WM", "CM", "STRICT", "TWC"]
+FUS = ["Standalone", "Fused"]
+ALL_VARIANTS = ["-".join(v) for v in itertools.product(DIRS, FMTS, LBS, FUS)]
+PUSH_VARIANTS = [v for v in ALL_VARIANTS if v.startswith("Push")]
+
+PR_TOL = 1e-5
+
+
+def graphs(gsw):
+    out = {}
+    out["rmat12"] = gsw.rmat(12, 16, seed=1)
+    out["grid"] = gsw.grid2d(40, 50, 0.1, 0.01, 3)
+    out["er"] = gsw.erdos_renyi(3000, 3.0, 5)
+    out["ba"] = gsw.barabasi_albert(2000, 3, 6)
+    return out
+
+
+@pytest.fixture(scope="module")
+def gset(gsw):
+    hs = graphs(gsw)
+    return {k: (h, gsw.Graph(h)) for k, h in hs.items()}
+
+
+@pytest.fixture(scope="module")
+def wset(gsw):
+    out = {}
+    for k, h in {"rmat12w": gsw.rmat(12, 16, seed=2, weighted=True),
+                 "gridw": gsw.grid2d(30, 40, 0.1, 0.01, 3, weighted=True),
+                 "erw": gsw.erdos_renyi(2000, 4.0, 8, weighted=True)}.items():
+        out[k] = (h, gsw.Graph(h))
+    return out
+
+
+# ---------------------------------------------------------------- primitives
+def test_wave_scan64(gsw):
+    rng = np.random.default_rng(0)
+    x = rng.integers(-1000, 1000, 64).astype(np.int32)
+    assert np.array_equal(gsw.wave_scan64(x), np.cumsum(x).astype(np.int32))
+
+
+@pytest.mark.parametrize("n", [0, 1, 2047, 2048, 2049, 100000, 3000001])
+def test_device_scan(gsw, n):
+    rng = np.random.default_rng(n)
+    x = rng.integers(0, 1 << 20, n).astype(np.int64)
+    out = gsw.scan_exclusive_i64(x)
+    ref = np.concatenate([[0], np.cumsum(x)])
+    assert np.array_equal(out, ref)
+
+
+# ---------------------------------------------------------------- BFS
+@pytest.mark.parametrize("gname", ["rmat12", "grid", "er", "ba"])
+def test_bfs_default_selector(gsw, orc, gset, gname):
+    h, g = gset[gname]
+    for root in [h.max_degree_vertex(), 0, h.n - 1]:
+        got = g.bfs(root)
+        assert np.array_equal(got, orc.bfs(h.row_ptr, h.col, root)), root
+
+
+@pytest.mark.parametrize("variant", ALL_VARIANTS)
+def test_bfs_forced_variant(gsw, orc, gset, variant):
+    for gname in ["rmat12", "grid"]:
+        h, g = gset[gname]
+        root = h.max_degree_vertex()
+        got = g.bfs(root, configs=variant)
+        ref = orc.bfs(h.row_ptr, h.col, root)
+        assert np.array_equal(got, ref), "%s on %s" % (variant, gname)
+
+
+def test_bfs_records_and_edge_cases(gsw, orc):
+    rp, col = csr_from_edges(6, [(0, 1), (1, 2)])
+    g = gsw.Graph(row_ptr=rp, col=col)
+    assert list(g.bfs(0)) == [0, 1, 2, -1, -1, -1]
+    assert list(g.bfs(4)) == [-1, -1, -1, -1, 0, -1]  # isolated root
+    recs = g.records()
+    assert all(r["frontier"] >= 0 for r in recs)
+    with pytest.raises(gsw.GswError):
+        g.bfs(6)
+    g.close()
+
+
+def test_bfs_c1_rmat16(gsw, orc):
+    """R-MAT scale 16, EF 16, seed 1, source = highest-degree vertex."""
+    h = gsw.rmat(16, 16, 0.57, 0.19, 0.19, seed=1)
+    g = gsw.Graph(h)
+    root = h.max_degree_vertex()
+    ref = orc.bfs(h.row_ptr, h.col, root)
+    assert np.array_equal(g.bfs(root), ref)
+    for v in ["Push-Queue-WM-Fused", "Pull-Bitmap-TM-Standalone",
+              "Push-SortedQueue-STRICT-Standalone", "Push-Queue-TWC-Standalone"]:
+        assert np.array_equal(g.bfs(root, configs=v), ref), v
+
+
+# ---------------------------------------------------------------- SSSP
+@pytest.mark.parametrize("gname", ["rmat12w", "gridw", "erw"])
+def test_sssp_default(gsw, orc, wset, gname):
+    h, g = wset[gname]
+    for root in [h.max_degree_vertex(), 0]:
+        assert np.array_equal(g.sssp(root), orc.spfa(h.row_ptr, h.col, h.w, root))
+
+
+@pytest.mark.parametrize("variant", ALL_VARIANTS)
+def test_sssp_forced_variant(gsw, orc, wset, variant):
+    h, g = wset["rmat12w"]
+    root = h.max_degree_vertex()
+    assert np.array_equal(g.sssp(root, configs=variant),
+                          orc.spfa(h.row_ptr, h.col, h.w, root)), variant
+
+
+def test_sssp_requires_weights(gsw, gset):
+    h, g = gset["er"]
+    with pytest.raises(gsw.GswError):
+        g.sssp(0)
+
+
+# ---------------------------------------------------------------- CC
+@pytest.mark.parametrize("gname", ["rmat12", "grid", "er", "ba"])
+@pytest.mark.parametrize("variant", [None, "x-Queue-x-x", "x-Bitmap-x-x"])
+def test_cc(gsw, orc, gset, gname, variant):
+    h, g = gset[gname]
+    assert np.array_equal(g.cc(configs=variant), orc.cc(h.row_ptr, h.col))
+
+
+# ---------------------------------------------------------------- PageRank
+def max_rel(a, b):
+    return float(np.max(np.abs(a.astype(np.float64) - b) / np.abs(b)))
+
+
+@pytest.mark.parametrize("variant", [None] + ["%s-Bitmap-%s-Standalone" % (d, l)
+                                              for d in DIRS for l in LBS])
+def test_pagerank(gsw, orc, gset, variant):
+    for gname in ["rmat12", "ba"]:
+        h, g = gset[gname]
+        pr = g.pagerank(0.85, 50, configs=variant)
+        ref = orc.pagerank(h.row_ptr, h.col, 0.85, 50)
+        assert max_rel(pr, ref) <= PR_TOL, (variant, gname, max_rel(pr, ref))
+
+
+def test_pagerank_iteration_counts(gsw, orc, gset):
+    h, g = gset["er"]
+    for k in (0, 1, 2, 5):
+        pr = g.pagerank(0.85, k)
+        assert max_rel(pr, orc.pagerank(h.row_ptr, h.col, 0.85, k)) <= PR_TOL, k

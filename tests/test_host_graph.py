@@ -1,0 +1,149 @@
+"""Host ingest: generators and the CSR builder, restated independently in
+numpy from their documented definitions (counter-based SplitMix64 draws,
+R-MAT quadrant thresholds, grid edge numbering), plus the degree features
+restated from graph_attr_t::fill (src/utils/fileIO.cuh:35-67)."""
+import numpy as np
+import pytest
+
+from conftest import csr_from_edges
+
+U64 = np.uint64
+
+
+def mix64(z):
+    with np.errstate(over="ignore"):
+        z = z.astype(U64)
+        z = (z ^ (z >> U64(30))) * U64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> U64(27))) * U64(0x94D049BB133111EB)
+        return z ^ (z >> U64(31))
+
+
+def stream_key(seed, stream):
+    with np.errstate(over="ignore"):
+        return mix64(mix64(np.array([seed], U64)) + U64(stream) * U64(0xD1B54A32D192ED03))[0]
+
+
+def draw(key, ctr):
+    with np.errstate(over="ignore"):
+        return mix64(U64(key) + (np.asarray(ctr, U64) + U64(1)) * U64(0x9E3779B97F4A7C15))
+
+
+def rmat_np(scale, ef, a, b, c, seed):
+    ne = ef << scale
+    key = stream_key(seed, 1)
+    t1 = np.uint64(int(np.floor(a * 2**32)))
+    t2 = np.uint64(int(np.floor((a + b) * 2**32)))
+    t3 = np.uint64(int(np.floor((a + b + c) * 2**32)))
+    per = (scale + 1) // 2
+    i = np.arange(ne, dtype=U64)
+    u = np.zeros(ne, U64)
+    v = np.zeros(ne, U64)
+    for k in range(scale):
+        r = draw(key, i * U64(per) + U64(k >> 1))
+        x = (r >> U64(32)) if (k & 1) else (r & U64(0xFFFFFFFF))
+        ub = (x >= t2).astype(U64)
+        vb = (((x >= t1) & (x < t2)) | (x >= t3)).astype(U64)
+        u = (u << U64(1)) | ub
+        v = (v << U64(1)) | vb
+    return u.astype(np.int64), v.astype(np.int64)
+
+
+def pair_weight_np(wkey, lo, hi, lim=64):
+    r = draw(wkey, (lo.astype(U64) << U64(32)) | hi.astype(U64))
+    return (1 + (((r >> U64(32)) * U64(lim)) >> U64(32))).astype(np.uint32)
+
+
+@pytest.mark.parametrize("scale,seed", [(8, 1), (11, 7)])
+def test_rmat_matches_numpy_restatement(gsw, scale, seed):
+    h = gsw.rmat(scale, 16, 0.57, 0.19, 0.19, seed)
+    u, v = rmat_np(scale, 16, 0.57, 0.19, 0.19, seed)
+    rp, col = csr_from_edges(1 << scale, np.stack([u, v], 1))
+    assert np.array_equal(h.row_ptr, rp)
+    assert np.array_equal(h.col, col)
+
+
+def test_rmat_weights_symmetric_and_restated(gsw):
+    h = gsw.rmat(9, 8, 0.57, 0.19, 0.19, 3, weighted=True)
+    src = np.repeat(np.arange(h.n), np.diff(h.row_ptr))
+    lo, hi = np.minimum(src, h.col), np.maximum(src, h.col)
+    assert np.array_equal(h.w, pair_weight_np(stream_key(3, 2), lo, hi))
+    assert h.w.min() >= 1 and h.w.max() <= 64
+
+
+def test_grid_structure(gsw):
+    R, Cc = 20, 30
+    full = gsw.grid2d(R, Cc, 0.0, 0.0, 3)
+    assert full.m == 2 * (R * (Cc - 1) + (R - 1) * Cc)
+    # deletions follow the documented edge numbering
+    h = gsw.grid2d(R, Cc, 0.1, 0.0, 3)
+    key = stream_key(3, 3)
+    thr = int(0.1 * 2**64)
+    kept = []
+    for r in range(R):
+        for c in range(Cc - 1):
+            if int(draw(key, r * (Cc - 1) + c)) >= thr:
+                kept.append((r * Cc + c, r * Cc + c + 1))
+    for r in range(R - 1):
+        for c in range(Cc):
+            if int(draw(key, R * (Cc - 1) + r * Cc + c)) >= thr:
+                kept.append((r * Cc + c, (r + 1) * Cc + c))
+    rp, col = csr_from_edges(R * Cc, kept)
+    assert np.array_equal(h.row_ptr, rp) and np.array_equal(h.col, col)
+    lr = gsw.grid2d(R, Cc, 0.1, 0.05, 3)
+    assert lr.m >= h.m
+
+
+def test_er_and_ba_properties(gsw):
+    h = gsw.erdos_renyi(5000, 8.0, 5)
+    assert abs(h.m / h.n - 8.0) < 0.3
+    ba = gsw.barabasi_albert(3000, 4, 6)
+    deg = ba.degrees
+    assert deg.min() >= 4 and deg.max() > 40  # preferential attachment hubs
+    assert ba.m == 2 * (10 + (3000 - 5) * 4)  # clique(5) + 4 per new vertex
+    again = gsw.barabasi_albert(3000, 4, 6)
+    assert np.array_equal(ba.col, again.col)
+
+
+def test_csr_builder_semantics(gsw):
+    rng = np.random.default_rng(0)
+    e = rng.integers(0, 300, size=(4000, 2))
+    h = gsw.from_edges(300, e[:, 0], e[:, 1])
+    rp, col = csr_from_edges(300, e)
+    assert np.array_equal(h.row_ptr, rp) and np.array_equal(h.col, col)
+    # weighted duplicates keep the smallest weight
+    w = rng.integers(1, 100, size=4000)
+    hw = gsw.from_edges(300, e[:, 0], e[:, 1], w)
+    rp2, col2, w2 = csr_from_edges(300, e, w)
+    assert np.array_equal(hw.col, col2) and np.array_equal(hw.w, w2)
+
+
+def test_csr_builder_directed(gsw):
+    h = gsw.from_edges(4, [0, 0, 2, 2], [1, 2, 3, 3], directed=True)
+    assert list(h.row_ptr) == [0, 2, 2, 3, 3]
+    assert list(h.col) == [1, 2, 3]
+
+
+def test_csr_builder_rejects_bad_ids(gsw):
+    with pytest.raises(gsw.GswError):
+        gsw.from_edges(3, [0, 5], [1, 1])
+
+
+def test_degree_stats_restated(gsw):
+    h = gsw.rmat(10, 8, seed=4)
+    d = np.sort(np.diff(h.row_ptr)).astype(np.float64)
+    n, s = len(d), d.sum()
+    i = np.arange(1, n + 1)
+    gi = 1 - 2 * np.sum((n - i + 0.5) * d / n / s)
+    nz = d > 0
+    her = np.sum(d[nz] * (np.log(s) - np.log(d[nz])) / s) / np.log(n)
+    st = gsw.degree_stats(h.row_ptr)
+    assert st["avg"] == pytest.approx(d.mean())
+    assert st["std"] == pytest.approx(d.std(ddof=1))
+    assert st["max"] == d.max() and st["range"] == d.max() - d.min()
+    assert st["GI"] == pytest.approx(gi)
+    assert st["Her"] == pytest.approx(her)
+    # closed forms: a regular graph has Gini 0 and entropy 1
+    ring = csr_from_edges(50, [(i, (i + 1) % 50) for i in range(50)])
+    st = gsw.degree_stats(ring[0])
+    assert st["GI"] == pytest.approx(0.0, abs=1e-12)
+    assert st["Her"] == pytest.approx(1.0)
