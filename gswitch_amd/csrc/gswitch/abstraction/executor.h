@@ -260,6 +260,13 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf) {
     const bool fused = conf.conf_fuse_inspect;
     LB lb = conf.conf_lb;
     if (lb == TWOD || lb == ELB) lb = STRICT;
+    // Scratch must exist before any kernel body copies `as` by value.
+    if (lb == TWC && push) as.bins.ensure(as.size);
+    if (lb == STRICT) as.workset.ensure(as.size);
+    GSW_REQUIRE(!(lb == TWC && push) || (as.bins.sizes && as.bins.bin[2]),
+                E_INTERNAL_CHECK, "TWC bins missing");
+    GSW_REQUIRE(lb != STRICT || (as.workset.dg_scan && as.workset.dg_bsum),
+                E_INTERNAL_CHECK, "STRICT workset missing");
     if (push) {
       auto launch = [&](auto body) {
         using Bd = decltype(body);
@@ -277,7 +284,6 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf) {
                                dim3(BLOCK), 0, st, body);
             break;
           case TWC: {
-            as.bins.ensure(as.size);
             fill_async<int32_t>(as.bins.sizes, 0, 4);
             hipLaunchKernelGGL(k_twc_bin<G>, dim3(expand_grid(N, BLOCK)),
                                dim3(BLOCK), 0, st, as, g, conf.conf_target,
@@ -287,7 +293,6 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf) {
             break;
           }
           default: {  // STRICT
-            as.workset.ensure(as.size);
             hipLaunchKernelGGL(k_strict_prepare<G>, dim3(grid_for(N)),
                                dim3(BLOCK), 0, st, as, g, conf.conf_target,
                                false, as.workset.dg_scan);
@@ -321,7 +326,6 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf) {
                              dim3(BLOCK), 0, st, body);
           break;
         default: {  // STRICT
-          as.workset.ensure(as.size);
           hipLaunchKernelGGL(k_strict_prepare<G>, dim3(grid_for(N)),
                              dim3(BLOCK), 0, st, as, g, conf.conf_target, true,
                              as.workset.dg_scan);

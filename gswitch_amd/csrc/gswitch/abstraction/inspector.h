@@ -31,6 +31,12 @@ struct inspector_t {
                          conf.conf_fusion);
     bool need = !conf.conf_fuse_inspect;
     init_fets(fets, (double)g.nvertexs, (double)g.nedges);
+    if (conf.conf_first_round) {
+      // the initial frontier is known exactly (root or everything), also
+      // when the first level is fused and features are not synthesised
+      known_active = fets.use_root < 0 ? g.nvertexs : 1;
+      known_inactive = g.nvertexs - known_active;
+    }
     if (need && conf.conf_first_round) {
       set_first_fets(as, fets, conf, (double)g.nvertexs, (double)g.nedges);
       need = false;

@@ -49,6 +49,7 @@ enum : int {
   E_ALLOC = 3,
   E_OVERFLOW = 4,
   E_UNSUPPORTED = 5,
+  E_INTERNAL_CHECK = 6,
 };
 
 inline void hip_check(hipError_t e, const char *what, const char *file,
