@@ -93,6 +93,7 @@ _SIGS = {
                                   C.POINTER(C.c_int32)]),
     "gsw_scan_exclusive_i64": (C.c_int32, [_P, C.c_int64, _P]),
     "gsw_wave_scan64": (C.c_int32, [_P, _P]),
+    "gsw_stream_copy": (C.c_int32, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(_lib, _name)
@@ -283,7 +284,7 @@ class Graph:
         return pr
 
     def records(self):
-        cap = 1 << 16
+        cap = max(1, self.last_stats.n_records if self.last_stats else 1 << 16)
         buf = (LevelRecord * cap)()
         n = C.c_int32()
         _check(_lib.gsw_last_records(self._h, buf, cap, C.byref(n)))
@@ -311,3 +312,10 @@ def wave_scan64(x):
     out = np.empty(64, np.int32)
     _check(_lib.gsw_wave_scan64(_ptr(x), _ptr(out)))
     return out
+
+
+def stream_copy(nbytes=1 << 30, reps=10):
+    """Best (read + write) GB/s of a plain device copy of `nbytes`."""
+    g = C.c_double()
+    _check(_lib.gsw_stream_copy(int(nbytes), int(reps), C.byref(g)))
+    return g.value

@@ -1,7 +1,7 @@
 """Build libgswitch_amd.so (HIP, gfx950) and the CPU oracle.
 
-    python -m gswitch_amd.build            # library + oracle
-    python -m gswitch_amd.build --lib      # library only
+    python gswitch_amd/build.py            # library + oracle
+    python gswitch_amd/build.py --lib      # library only
 
 The library is compiled in-tree with hipcc for --offload-arch=gfx950 so the
 built .so travels with the source tree. Packed-FP32 instructions are disabled

@@ -58,6 +58,9 @@ struct SSSP : Functor<VC, uint32_t, uint32_t, uint32_t> {
     return false;
   }
   __device__ bool compAtomic(uint32_t *d, uint32_t msg, GW g) {
+    // distances only decrease, so a plain read that is already <= msg
+    // proves the atomic would change nothing
+    if (*d <= msg) return false;
     const uint32_t old = atomicMin(d, msg);
     return old > msg && data.window.in((double)msg);
   }

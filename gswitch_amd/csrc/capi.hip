@@ -728,6 +728,51 @@ gsw_status gsw_wave_scan64(const int32_t *in, int32_t *out) {
   });
 }
 
+namespace {
+__global__ void __launch_bounds__(gsw::BLOCK)
+    k_stream_copy(const int4 *__restrict__ src, int4 *__restrict__ dst,
+                  int64_t n16) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace
+
+gsw_status gsw_stream_copy(int64_t bytes, int32_t reps, double *gbps) {
+  return guarded([&] {
+    GSW_REQUIRE(gbps && bytes >= 16 && reps >= 1, GSW_E_INVALID,
+                "invalid arguments");
+    require_gpu();
+    const int64_t n16 = bytes / 16;
+    int4 *a = nullptr, *b = nullptr;
+    H_ERR(hipMalloc((void **)&a, n16 * 16));
+    H_ERR(hipMalloc((void **)&b, n16 * 16));
+    H_ERR(hipMemset(a, 1, n16 * 16));
+    hipEvent_t e0, e1;
+    H_ERR(hipEventCreate(&e0));
+    H_ERR(hipEventCreate(&e1));
+    hipStream_t st = gsw::stream();
+    const int grid = 256 * 8 * 4;
+    double best = 0;
+    for (int r = 0; r < reps; ++r) {
+      H_ERR(hipEventRecord(e0, st));
+      hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(gsw::BLOCK), 0, st, a,
+                         b, n16);
+      H_ERR(hipEventRecord(e1, st));
+      H_ERR(hipEventSynchronize(e1));
+      float ms = 0;
+      H_ERR(hipEventElapsedTime(&ms, e0, e1));
+      const double g = 2.0 * (double)n16 * 16 / (ms * 1e-3) / 1e9;
+      best = g > best ? g : best;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    *gbps = best;
+  });
+}
+
 gsw_status gsw_scan_exclusive_i64(const int64_t *in, int64_t n, int64_t *out) {
   return guarded([&] {
     GSW_REQUIRE(in && out && n >= 0, GSW_E_INVALID, "invalid arguments");

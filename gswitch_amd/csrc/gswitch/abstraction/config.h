@@ -55,6 +55,10 @@ struct config_t {
   // otherwise workgroup.
   int twc_small = 32;
   int twc_large = 2048;
+  // skew guard: a TM/WM/CM choice moves to STRICT when a frontier vertex has
+  // more edges than this (one lane / wavefront / workgroup would otherwise
+  // walk a hub alone); 0 disables it. Forced LBs are never changed.
+  int skew_limit = 4096;
   ins_t force;
 
   void reset() {

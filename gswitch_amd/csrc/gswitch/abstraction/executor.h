@@ -293,12 +293,16 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf) {
             break;
           }
           default: {  // STRICT
-            hipLaunchKernelGGL(k_strict_prepare<G>, dim3(grid_for(N)),
-                               dim3(BLOCK), 0, st, as, g, conf.conf_target,
-                               false, as.workset.dg_scan);
-            scan_exclusive_inplace(as.workset.dg_scan, N, as.workset.dg_bsum);
+            const int64_t *scan = nullptr;  // bitmap: implicit rows[v] + v
+            if (as.fmt == Queue) {
+              hipLaunchKernelGGL(k_strict_prepare<G>, dim3(grid_for(N)),
+                                 dim3(BLOCK), 0, st, as, g, conf.conf_target,
+                                 false, as.workset.dg_scan);
+              scan_exclusive_inplace(as.workset.dg_scan, N, as.workset.dg_bsum);
+              scan = as.workset.dg_scan;
+            }
             hipLaunchKernelGGL(k_push_strict<Bd>, dim3(MAX_GRID), dim3(BLOCK),
-                               0, st, body, (const int64_t *)as.workset.dg_scan);
+                               0, st, body, scan);
             break;
           }
         }
@@ -326,12 +330,16 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf) {
                              dim3(BLOCK), 0, st, body);
           break;
         default: {  // STRICT
-          hipLaunchKernelGGL(k_strict_prepare<G>, dim3(grid_for(N)),
-                             dim3(BLOCK), 0, st, as, g, conf.conf_target, true,
-                             as.workset.dg_scan);
-          scan_exclusive_inplace(as.workset.dg_scan, N, as.workset.dg_bsum);
+          const int64_t *scan = nullptr;  // bitmap: implicit rows[v] + v
+          if (as.fmt == Queue) {
+            hipLaunchKernelGGL(k_strict_prepare<G>, dim3(grid_for(N)),
+                               dim3(BLOCK), 0, st, as, g, conf.conf_target,
+                               true, as.workset.dg_scan);
+            scan_exclusive_inplace(as.workset.dg_scan, N, as.workset.dg_bsum);
+            scan = as.workset.dg_scan;
+          }
           hipLaunchKernelGGL(k_pull_strict<Bd>, dim3(MAX_GRID), dim3(BLOCK), 0,
-                             st, body, (const int64_t *)as.workset.dg_scan);
+                             st, body, scan);
           break;
         }
       }

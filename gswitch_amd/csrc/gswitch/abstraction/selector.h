@@ -146,6 +146,18 @@ struct selector_t {
     if (conf.force.has_lb) conf.conf_lb = conf.force.lb;
   }
 
+  // Move TM/WM/CM to the edge-balanced STRICT when the frontier holds a
+  // vertex whose degree exceeds conf.skew_limit. Idempotent pulls (early
+  // exit per target) keep their choice.
+  static void skew_guard(const feature_t &fets, config_t &conf) {
+    if (conf.force.has_lb || conf.skew_limit <= 0) return;
+    if (conf.conf_lb == STRICT || conf.conf_lb == TWC) return;
+    if (conf.conf_dir == Pull && conf.conf_pruning) return;
+    const double maxd = (conf.conf_dir == Push) ? fets.cur_max_deg_active
+                                                : fets.cur_max_deg_unactive;
+    if (maxd > conf.skew_limit) conf.conf_lb = STRICT;
+  }
+
   static void set_target(config_t &conf) {
     if (conf.conf_dir == Push)
       conf.conf_target = conf.conf_fromall ? All : Active;
@@ -185,6 +197,7 @@ struct selector_t {
       // forced variants may pin the LB of fused levels too
       if (conf.force.has_lb) conf.conf_lb = conf.force.lb;
       if (conf.force.has_sorted) conf.conf_sorted = conf.force.sorted;
+      skew_guard(fets, conf);
       set_target(conf);
       return;
     }
@@ -195,6 +208,7 @@ struct selector_t {
     else
       select_Mono(fets, conf);
     if (conf.force.has_sorted) conf.conf_sorted = conf.force.sorted;
+    skew_guard(fets, conf);
     set_target(conf);
   }
 };

@@ -246,54 +246,84 @@ __global__ void __launch_bounds__(BLOCK) k_pull_cm(B body) {
 }
 
 // ---------------------------------------------------------------- STRICT
+// A segment that holds every in-edge of its target inside this wavefront's
+// slot range is combined with a plain comp; only targets that straddle
+// wavefronts (or workgroups) pay compAtomic.
 template <class B>
 __global__ void __launch_bounds__(BLOCK)
     k_pull_strict(B body, const int64_t *__restrict__ scan) {
   using WA = typename B::WA;
-  __shared__ int64_t s_scan[BLOCK + 1];
-  __shared__ int64_t s_start[BLOCK + 1];
-  __shared__ int s_v[BLOCK + 1];
+  __shared__ int64_t s_scan[SCHUNK + 1];
+  __shared__ int64_t s_start[SCHUNK + 1];
+  __shared__ int s_v[SCHUNK + 1];
+  const slots_t sl{scan, body.rows()};
   const int64_t N = body.as.get_size();
   const Status want = body.conf.want();
   const bool prune = body.conf.conf_pruning;
-  const int64_t total = scan[N];
+  const int64_t total = sl.at(N);
   const int64_t per = (total + gridDim.x - 1) / gridDim.x;
   const int64_t lo = (int64_t)blockIdx.x * per;
   const int64_t hi = MIN(total, lo + per);
   unsigned long long ne = 0;
   if (lo >= hi) return;
-  int64_t k0 = last_le(scan, N + 1, lo);
-  for (int64_t c = lo; c < hi; c += BLOCK) {
-    strict_window(body, scan, N, k0, want, body.rows(), s_scan, s_v, s_start);
+  const int wofs = threadIdx.x & ~(WAVE - 1);
+  int64_t k0 = sl.last_le(N + 1, lo);
+  for (int64_t c = lo; c < hi; c += SCHUNK) {
+    strict_window(body, sl, N, k0, want, s_scan, s_v, s_start);
     __syncthreads();
-    const int64_t slot = c + threadIdx.x;
-    int o = -1;
-    bool vote = false;
-    WA m{};
-    if (slot < hi) {
-      o = owner_search(s_scan, BLOCK + 1, slot);
-      const int64_t r = slot - s_scan[o];
-      const int64_t deg = s_scan[o + 1] - s_scan[o] - 1;
-      const int v = s_v[o];
-      if (v >= 0 && r < deg && !(prune && body.as.bitmap.visited.test(v))) {
-        const int64_t e = s_start[o] + r;
-        const int u = body.cols()[e];
-        ++ne;
-        if (body.source_ok(u)) {
-          vote = true;
-          m = body.msg(u, e);
+    int o[SITEMS], u[SITEMS];
+    int64_t e[SITEMS];
+#pragma unroll
+    for (int j = 0; j < SITEMS; ++j) {
+      const int64_t slot = c + j * BLOCK + threadIdx.x;
+      o[j] = -1;
+      u[j] = -1;
+      if (slot < hi) {
+        o[j] = owner_search(s_scan, SCHUNK + 1, slot);
+        const int64_t r = slot - s_scan[o[j]];
+        const int64_t deg = s_scan[o[j] + 1] - s_scan[o[j]] - 1;
+        const int v = s_v[o[j]];
+        if (v >= 0 && r < deg && !(prune && body.as.bitmap.visited.test(v))) {
+          e[j] = s_start[o[j]] + r;
+          u[j] = 0;
         }
       }
     }
-    body.seg_reduce(o, vote, m);
-    if (body.seg_head(o) && vote) {
-      const int v = s_v[o];
-      if (!prune || body.as.bitmap.visited.test_and_set(v))
-        body.f.compAtomic(body.f.wa_of(v), m, body.g);
+#pragma unroll
+    for (int j = 0; j < SITEMS; ++j)
+      if (u[j] == 0) u[j] = body.cols()[e[j]];
+      else u[j] = -1;
+#pragma unroll
+    for (int j = 0; j < SITEMS; ++j) {
+      bool vote = false;
+      WA m{};
+      if (u[j] >= 0) {
+        ++ne;
+        if (body.source_ok(u[j])) {
+          vote = true;
+          m = body.msg(u[j], e[j]);
+        }
+      }
+      body.seg_reduce(o[j], vote, m);
+      if (body.seg_head(o[j]) && vote) {
+        const int v = s_v[o[j]];
+        // this wavefront's slots for item j: [wlo, min(wlo + 64, hi))
+        const int64_t wlo = c + j * BLOCK + wofs;
+        const int64_t whi = MIN(wlo + WAVE, hi);
+        const bool whole = s_scan[o[j]] >= wlo && s_scan[o[j] + 1] - 1 <= whi;
+        if (prune) {
+          if (body.as.bitmap.visited.test_and_set(v))
+            body.f.compAtomic(body.f.wa_of(v), m, body.g);
+        } else if (whole) {
+          WA acc = *body.f.wa_of(v);
+          body.f.comp(&acc, m, body.g);
+          *body.f.wa_of(v) = acc;
+        } else {
+          body.f.compAtomic(body.f.wa_of(v), m, body.g);
+        }
+      }
     }
-    const int64_t nc = c + BLOCK;
-    const int64_t nk = (nc < s_scan[BLOCK]) ? k0 + owner_search(s_scan, BLOCK + 1, nc)
-                                            : k0 + BLOCK;
+    const int64_t nk = strict_next_owner(s_scan, k0, c + SCHUNK);
     __syncthreads();
     k0 = nk;
   }

@@ -214,65 +214,111 @@ __global__ void __launch_bounds__(BLOCK)
   }
 }
 
-// Stage the 257 prefixes starting at item k0 (clamped to N) into LDS.
+// Slot prefix of the STRICT partition: an explicit int64 scan of (deg + 1)
+// over the frontier items (queue frontiers), or, for bitmap frontiers where
+// item k is vertex k, the implicit prefix rows[k] + k (no scan launches).
+struct slots_t {
+  const int64_t *scan;  // explicit prefix, or nullptr
+  const int64_t *rows;  // offsets of the traversed direction
+  __device__ __forceinline__ int64_t at(int64_t k) const {
+    return scan ? scan[k] : rows[k] + k;
+  }
+  // largest k in [0, len) with at(k) <= x (at(0) <= x)
+  __device__ __forceinline__ int64_t last_le(int64_t len, int64_t x) const {
+    int64_t lo = 0, hi = len;
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (at(mid) <= x)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    return lo;
+  }
+};
+
+// A STRICT chunk is SITEMS slots per lane: slots c + j*BLOCK + lane, so each
+// lane keeps SITEMS independent loads in flight and every load instruction
+// of a wavefront stays coalesced. The window holds the SCHUNK + 1 prefixes
+// that can own the chunk's slots (each item owns >= 1 slot).
+constexpr int SITEMS = 4;
+constexpr int SCHUNK = BLOCK * SITEMS;
+
+// Stage the SCHUNK + 1 prefixes starting at item k0 (clamped to N) into LDS.
 template <class B>
-__device__ __forceinline__ void strict_window(B &body, const int64_t *scan,
+__device__ __forceinline__ void strict_window(B &body, const slots_t &sl,
                                               int64_t N, int64_t k0,
-                                              Status want, const int64_t *rows,
-                                              int64_t *s_scan, int *s_v,
-                                              int64_t *s_start) {
-  for (int t = threadIdx.x; t <= BLOCK; t += blockDim.x) {
+                                              Status want, int64_t *s_scan,
+                                              int *s_v, int64_t *s_start) {
+  for (int t = threadIdx.x; t <= SCHUNK; t += blockDim.x) {
     const int64_t k = k0 + t;
-    s_scan[t] = scan[k < N ? k : N];
+    s_scan[t] = sl.at(k < N ? k : N);
     int v = -1;
     int64_t st = 0;
     if (k < N) {
       v = body.as.fetch(k, want);
-      if (v >= 0) st = rows[v];
+      if (v >= 0) st = sl.rows[v];
     }
     s_v[t] = v;
     s_start[t] = st;
   }
 }
 
+// Owner of the next chunk start: inside the staged window, or exactly the
+// item right after it (every item owns at least one slot).
+__device__ __forceinline__ int64_t strict_next_owner(const int64_t *s_scan,
+                                                     int64_t k0, int64_t nc) {
+  return (nc < s_scan[SCHUNK]) ? k0 + owner_search(s_scan, SCHUNK + 1, nc)
+                               : k0 + SCHUNK;
+}
+
 template <class B>
 __global__ void __launch_bounds__(BLOCK)
     k_push_strict(B body, const int64_t *__restrict__ scan) {
-  __shared__ int64_t s_scan[BLOCK + 1];
-  __shared__ int64_t s_start[BLOCK + 1];
-  __shared__ int s_v[BLOCK + 1];
+  __shared__ int64_t s_scan[SCHUNK + 1];
+  __shared__ int64_t s_start[SCHUNK + 1];
+  __shared__ int s_v[SCHUNK + 1];
+  const slots_t sl{scan, body.rows()};
   const int64_t N = body.as.get_size();
   const Status want = body.conf.want();
-  const int64_t total = scan[N];
+  const int64_t total = sl.at(N);
   const int64_t per = (total + gridDim.x - 1) / gridDim.x;
   const int64_t lo = (int64_t)blockIdx.x * per;
   const int64_t hi = MIN(total, lo + per);
   unsigned long long ne = 0;
   if (lo >= hi) return;
-  int64_t k0 = last_le(scan, N + 1, lo);
-  for (int64_t c = lo; c < hi; c += BLOCK) {
-    strict_window(body, scan, N, k0, want, body.rows(), s_scan, s_v, s_start);
+  int64_t k0 = sl.last_le(N + 1, lo);
+  for (int64_t c = lo; c < hi; c += SCHUNK) {
+    strict_window(body, sl, N, k0, want, s_scan, s_v, s_start);
     __syncthreads();
-    const int64_t slot = c + threadIdx.x;
-    int u = -1;
-    bool q = false;
-    if (slot < hi) {
-      const int o = owner_search(s_scan, BLOCK + 1, slot);
-      const int64_t r = slot - s_scan[o];
-      const int64_t deg = s_scan[o + 1] - s_scan[o] - 1;
-      const int v = s_v[o];
-      if (v >= 0 && r < deg) {
-        const int64_t e = s_start[o] + r;
-        u = body.cols()[e];
-        q = body(v, u, e);
-        ++ne;
+    int u[SITEMS], v[SITEMS];
+    int64_t e[SITEMS];
+#pragma unroll
+    for (int j = 0; j < SITEMS; ++j) {
+      const int64_t slot = c + j * BLOCK + threadIdx.x;
+      v[j] = -1;
+      if (slot < hi) {
+        const int o = owner_search(s_scan, SCHUNK + 1, slot);
+        const int64_t r = slot - s_scan[o];
+        const int64_t deg = s_scan[o + 1] - s_scan[o] - 1;
+        if (s_v[o] >= 0 && r < deg) {
+          v[j] = s_v[o];
+          e[j] = s_start[o] + r;
+        }
       }
     }
-    body.emit_q(q, u);
-    // owner of the next chunk start, from the staged window
-    const int64_t nc = c + BLOCK;
-    const int64_t nk = (nc < s_scan[BLOCK]) ? k0 + owner_search(s_scan, BLOCK + 1, nc)
-                                            : k0 + BLOCK;
+#pragma unroll
+    for (int j = 0; j < SITEMS; ++j) u[j] = (v[j] >= 0) ? body.cols()[e[j]] : -1;
+#pragma unroll
+    for (int j = 0; j < SITEMS; ++j) {
+      bool q = false;
+      if (v[j] >= 0) {
+        q = body(v[j], u[j], e[j]);
+        ++ne;
+      }
+      body.emit_q(q, u[j]);
+    }
+    const int64_t nk = strict_next_owner(s_scan, k0, c + SCHUNK);
     __syncthreads();
     k0 = nk;
   }

@@ -162,6 +162,11 @@ gsw_status gsw_tree_eval(const char *name, const double *fv, int32_t nfv,
 gsw_status gsw_scan_exclusive_i64(const int64_t *in, int64_t n, int64_t *out);
 /* wave64 DPP inclusive scan of 64 ints (host buffers); diagnostics. */
 gsw_status gsw_wave_scan64(const int32_t *in, int32_t *out);
+/* Device-to-device copy of `bytes` with a plain 16-byte-per-lane kernel,
+ * `reps` times; *gbps = best (read + write) bandwidth. The measured HBM
+ * ceiling, and the calibration kernel for hardware byte counters; it has
+ * no counterpart in the reference. */
+gsw_status gsw_stream_copy(int64_t bytes, int32_t reps, double *gbps);
 
 #ifdef __cplusplus
 }
