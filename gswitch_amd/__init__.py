@@ -37,7 +37,7 @@ class RunOpts(C.Structure):
     _fields_ = [("configs", C.c_char_p), ("max_levels", C.c_int32),
                 ("verbose", C.c_int32), ("twc_small", C.c_int32),
                 ("twc_large", C.c_int32), ("profile", C.c_int32),
-                ("reserved", C.c_int32 * 3)]
+                ("pull_kernel", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 class RunStats(C.Structure):
@@ -203,7 +203,7 @@ def tree_eval(name, fv):
 
 # --------------------------------------------------------------- device graphs
 def run_opts(configs=None, max_levels=0, verbose=False, twc_small=0,
-             twc_large=0, profile=True):
+             twc_large=0, profile=True, pull_kernel=0):
     o = RunOpts()
     _lib.gsw_run_opts_init(C.byref(o))
     o.configs = configs.encode() if configs else None
@@ -215,6 +215,7 @@ def run_opts(configs=None, max_levels=0, verbose=False, twc_small=0,
     if twc_large:
         o.twc_large = twc_large
     o.profile = int(profile)
+    o.pull_kernel = int(pull_kernel)
     return o
 
 

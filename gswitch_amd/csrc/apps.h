@@ -130,6 +130,7 @@ struct run_opts_t {
   bool verbose = false;
   bool sorted = false;
   int twc_small = 32, twc_large = 2048;
+  bool mp_pull = true;
 };
 
 inline void apply_opts(config_t &conf, const run_opts_t &o) {
@@ -137,6 +138,7 @@ inline void apply_opts(config_t &conf, const run_opts_t &o) {
   conf.conf_sorted = o.sorted;
   conf.twc_small = o.twc_small;
   conf.twc_large = o.twc_large;
+  conf.mp_pull = o.mp_pull;
 }
 
 // Generic vertex-centric level loop (application/bfs.cu:87-98 shape).

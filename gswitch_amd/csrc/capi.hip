@@ -149,6 +149,7 @@ void upload_graph(gsw_graph *G) {
     G->dgw.dg_adj_list = G->dg.dg_adj_list;
     G->dgw.dgr_row_ptr = G->dg.dgr_row_ptr;
     G->dgw.dgr_adj_list = G->dg.dgr_adj_list;
+    G->dgw.twod = nullptr;  // TWOD slices are built for unweighted views only
     GSW_REQUIRE(!G->hg.directed, GSW_E_UNSUPPORTED,
                 "weighted directed graphs are not supported");
     H_ERR(hipMalloc((void **)&G->dgw.dg_edgedata,
@@ -174,6 +175,7 @@ run_opts_t to_opts(const gsw_run_opts *o) {
   r.verbose = o->verbose != 0;
   if (o->twc_small > 0) r.twc_small = o->twc_small;
   if (o->twc_large > 0) r.twc_large = o->twc_large;
+  r.mp_pull = o->pull_kernel == 0;
   return r;
 }
 

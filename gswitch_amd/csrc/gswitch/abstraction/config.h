@@ -59,6 +59,9 @@ struct config_t {
   // more edges than this (one lane / wavefront / workgroup would otherwise
   // walk a hub alone); 0 disables it. Forced LBs are never changed.
   int skew_limit = 4096;
+  // from-all bitmap pulls use the merge-path kernel (k_pull_mp*) instead of
+  // the segmented-reduction STRICT/TWOD kernels
+  bool mp_pull = true;
   ins_t force;
 
   void reset() {

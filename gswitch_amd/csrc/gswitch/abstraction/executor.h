@@ -21,6 +21,7 @@
 #include "../kernels/expand_push.h"
 #include "../kernels/filter.h"
 #include "../kernels/misc.h"
+#include "../kernels/twod.h"
 #include "config.h"
 #include "features.h"
 #include "inspector.h"
@@ -259,7 +260,16 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf) {
     const bool push = conf.conf_dir == Push;
     const bool fused = conf.conf_fuse_inspect;
     LB lb = conf.conf_lb;
-    if (lb == TWOD || lb == ELB) lb = STRICT;
+    // TWOD serves from-all pulls over a bitmap frontier of an unweighted
+    // graph; anything else takes the edge-balanced STRICT path
+    bool twod_ok = false;
+    if constexpr (!G::has_edata)
+      twod_ok = !push && conf.conf_fromall && as.fmt == Bitmap && g.twod;
+    if (lb == ELB || (lb == TWOD && !twod_ok)) lb = STRICT;
+    if (lb == TWOD) {
+      as.workset.ensure(as.size);
+      twod_ensure(g, TWOD_MAX, as.workset.dg_bsum);
+    }
     // Scratch must exist before any kernel body copies `as` by value.
     if (lb == TWC && push) as.bins.ensure(as.size);
     if (lb == STRICT) as.workset.ensure(as.size);
@@ -329,8 +339,23 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf) {
           hipLaunchKernelGGL(k_pull_cm<Bd>, dim3(expand_grid(N, BLOCK)),
                              dim3(BLOCK), 0, st, body);
           break;
+        case TWOD:
+          if (conf.mp_pull)
+            hipLaunchKernelGGL(k_pull_mp_twod<Bd>, dim3(MAX_GRID), dim3(BLOCK),
+                               0, st, body, *g.twod);
+          else
+            hipLaunchKernelGGL(k_pull_twod<Bd>, dim3(MAX_GRID), dim3(BLOCK), 0,
+                               st, body, *g.twod);
+          break;
         default: {  // STRICT
           const int64_t *scan = nullptr;  // bitmap: implicit rows[v] + v
+          if (conf.mp_pull && conf.conf_fromall && !conf.conf_pruning &&
+              as.fmt == Bitmap) {
+            // every in-edge contributes: the merge-path walk is cheaper
+            hipLaunchKernelGGL(k_pull_mp<Bd>, dim3(MAX_GRID), dim3(BLOCK), 0,
+                               st, body);
+            break;
+          }
           if (as.fmt == Queue) {
             hipLaunchKernelGGL(k_strict_prepare<G>, dim3(grid_for(N)),
                                dim3(BLOCK), 0, st, as, g, conf.conf_target,

@@ -149,13 +149,18 @@ struct selector_t {
   // Move TM/WM/CM to the edge-balanced STRICT when the frontier holds a
   // vertex whose degree exceeds conf.skew_limit. Idempotent pulls (early
   // exit per target) keep their choice.
+  // From-all pulls over a bitmap (PageRank-like) go to TWOD instead, whose
+  // XCD-mapped source slices keep the gathers L2-resident.
   static void skew_guard(const feature_t &fets, config_t &conf) {
     if (conf.force.has_lb || conf.skew_limit <= 0) return;
-    if (conf.conf_lb == STRICT || conf.conf_lb == TWC) return;
     if (conf.conf_dir == Pull && conf.conf_pruning) return;
     const double maxd = (conf.conf_dir == Push) ? fets.cur_max_deg_active
                                                 : fets.cur_max_deg_unactive;
-    if (maxd > conf.skew_limit) conf.conf_lb = STRICT;
+    if (maxd <= conf.skew_limit) return;
+    if (conf.conf_dir == Pull && conf.conf_fromall && conf.conf_asfmt == Bitmap)
+      conf.conf_lb = TWOD;
+    else if (conf.conf_lb == TM || conf.conf_lb == WM || conf.conf_lb == CM)
+      conf.conf_lb = STRICT;
   }
 
   static void set_target(config_t &conf) {

@@ -50,6 +50,21 @@ struct graph_attr_t {
   }
 };
 
+// 2-D (source-sliced) layout for from-all pulls, the TWOD variant. The
+// in-adjacency is split into P slices by source id, with slice boundaries at
+// equal edge counts; slice p is a full CSR (rp[p][n+1], col[p][...]) over
+// sources in [bound[p], bound[p+1]). Workgroup b of a TWOD launch serves
+// slice b % P, which is the XCD the dispatcher places it on, so each XCD's
+// gathers stay inside one slice that fits its L2 (cf. chunk_t,
+// src/utils/2d_partition.cuh:13-299, which tiles for shared memory).
+constexpr int TWOD_MAX = 8;
+struct twod_t {
+  int P = 0;
+  int64_t *rp[TWOD_MAX] = {};
+  int32_t *col[TWOD_MAX] = {};
+  int32_t bound[TWOD_MAX + 1] = {};
+};
+
 template <GraphFmt fmt, typename E> struct device_graph_t {};
 
 template <typename E> struct device_graph_t<CSR, E> {
@@ -68,6 +83,9 @@ template <typename E> struct device_graph_t<CSR, E> {
   int64_t *dgr_row_ptr = nullptr;
   int32_t *dgr_adj_list = nullptr;
   E *dgr_edgedata = nullptr;
+  // host-side handle of the TWOD slices (kernels/twod.h builds them on first
+  // use); shared by every copy of this struct, owned by the one that built it
+  twod_t *twod = nullptr;
 
   // Upload a host CSR. `w` is converted to E when E is not Empty.
   int64_t build(const gsw_host::HostGraph &hg) {
@@ -75,6 +93,7 @@ template <typename E> struct device_graph_t<CSR, E> {
     nedges = hg.m;
     directed = hg.directed;
     level = 0;
+    twod = new twod_t();
     int64_t bytes = 0;
     H_ERR(hipMalloc((void **)&dg_row_ptr, sizeof(int64_t) * (nvertexs + 1)));
     H_ERR(hipMemcpy(dg_row_ptr, hg.row_ptr.data(),
@@ -107,6 +126,14 @@ template <typename E> struct device_graph_t<CSR, E> {
   }
 
   void release() {
+    if (twod) {
+      for (int p = 0; p < twod->P; ++p) {
+        hipFree(twod->rp[p]);
+        hipFree(twod->col[p]);
+      }
+      delete twod;
+      twod = nullptr;
+    }
     if (directed) {
       hipFree(dgr_row_ptr);
       hipFree(dgr_adj_list);

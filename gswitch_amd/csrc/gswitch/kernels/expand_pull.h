@@ -249,20 +249,24 @@ __global__ void __launch_bounds__(BLOCK) k_pull_cm(B body) {
 // A segment that holds every in-edge of its target inside this wavefront's
 // slot range is combined with a plain comp; only targets that straddle
 // wavefronts (or workgroups) pay compAtomic.
+// Workgroup `bi` of `nb` processes its equal share of the slots of `sl`
+// (adjacency `cols`). `allow_whole` = false when other workgroups' slots may
+// also feed the same target (TWOD slices), so every flush is compAtomic.
 template <class B>
-__global__ void __launch_bounds__(BLOCK)
-    k_pull_strict(B body, const int64_t *__restrict__ scan) {
+__device__ __forceinline__ void pull_strict_range(B &body, const slots_t sl,
+                                                  const int32_t *cols,
+                                                  int64_t bi, int64_t nb,
+                                                  bool allow_whole) {
   using WA = typename B::WA;
   __shared__ int64_t s_scan[SCHUNK + 1];
   __shared__ int64_t s_start[SCHUNK + 1];
   __shared__ int s_v[SCHUNK + 1];
-  const slots_t sl{scan, body.rows()};
   const int64_t N = body.as.get_size();
   const Status want = body.conf.want();
   const bool prune = body.conf.conf_pruning;
   const int64_t total = sl.at(N);
-  const int64_t per = (total + gridDim.x - 1) / gridDim.x;
-  const int64_t lo = (int64_t)blockIdx.x * per;
+  const int64_t per = (total + nb - 1) / nb;
+  const int64_t lo = bi * per;
   const int64_t hi = MIN(total, lo + per);
   unsigned long long ne = 0;
   if (lo >= hi) return;
@@ -291,7 +295,7 @@ __global__ void __launch_bounds__(BLOCK)
     }
 #pragma unroll
     for (int j = 0; j < SITEMS; ++j)
-      if (u[j] == 0) u[j] = body.cols()[e[j]];
+      if (u[j] == 0) u[j] = cols[e[j]];
       else u[j] = -1;
 #pragma unroll
     for (int j = 0; j < SITEMS; ++j) {
@@ -310,7 +314,8 @@ __global__ void __launch_bounds__(BLOCK)
         // this wavefront's slots for item j: [wlo, min(wlo + 64, hi))
         const int64_t wlo = c + j * BLOCK + wofs;
         const int64_t whi = MIN(wlo + WAVE, hi);
-        const bool whole = s_scan[o[j]] >= wlo && s_scan[o[j] + 1] - 1 <= whi;
+        const bool whole = allow_whole && s_scan[o[j]] >= wlo &&
+                           s_scan[o[j] + 1] - 1 <= whi;
         if (prune) {
           if (body.as.bitmap.visited.test_and_set(v))
             body.f.compAtomic(body.f.wa_of(v), m, body.g);
@@ -328,6 +333,195 @@ __global__ void __launch_bounds__(BLOCK)
     k0 = nk;
   }
   add_work(body.as.dg_work, ne, 0);
+}
+
+// ------------------------------------------------- merge-path from-all pull
+// For from-all pulls (every in-edge contributes: PageRank-like) over all
+// targets. The slot space is the implicit rows[k] + k (deg + 1 slots per
+// target, the last one closes the row). A tile of MP_TILE slots is staged
+// in three steps:
+//  A. the MP_TILE + 1 row prefixes and the tile's contiguous edge range are
+//     read coalesced; every message emit(col[e]) is gathered into LDS, so all
+//     gathers of the tile are in flight at once;
+//  B. lane t walks its MP_R consecutive slots: one LDS read and one comp per
+//     edge; a row that starts and ends inside the lane's run is written with
+//     a plain comp (when `exclusive`), other row pieces become a head (row
+//     that started before the run) or a tail (row still open at its end);
+//  C. a head collects the tails of the lanes before it that belong to the
+//     same row; rows wholly inside the tile are written plainly, rows that
+//     cross a tile boundary (or any row, when !exclusive) use compAtomic.
+// `exclusive` = this launch is the only producer for each target.
+constexpr int MP_R = 8;
+constexpr int MP_TILE = BLOCK * MP_R;
+
+template <class B>
+__device__ __forceinline__ void pull_mp_range(B &body, const int64_t *rows,
+                                              const int32_t *cols, int64_t bi,
+                                              int64_t nb, bool exclusive) {
+  using WA = typename B::WA;
+  __shared__ int s_row[MP_TILE + 2];  // row prefix relative to the tile start
+  __shared__ WA s_msg[MP_TILE];
+  __shared__ WA s_tail[BLOCK];
+  __shared__ int s_tail_row[BLOCK];
+  __shared__ int64_t s_k0;
+  const int t = threadIdx.x;
+  const int64_t N = body.as.get_size();
+  const Status want = body.conf.want();
+  const slots_t sl{nullptr, rows};
+  const int64_t total = sl.at(N);
+  const int64_t ntiles = (total + MP_TILE - 1) / MP_TILE;
+  const int64_t tpb = (ntiles + nb - 1) / nb;
+  const int64_t tile_lo = bi * tpb, tile_hi = MIN(ntiles, tile_lo + tpb);
+  unsigned long long ne = 0;
+  if (tile_lo >= tile_hi) return;
+  if (t == 0) s_k0 = sl.last_le(N + 1, tile_lo * MP_TILE);
+  __syncthreads();
+  int64_t k0 = s_k0;
+  for (int64_t tile = tile_lo; tile < tile_hi; ++tile) {
+    const int64_t S = tile * MP_TILE;
+    const int64_t S_end = MIN(S + MP_TILE, total);
+    const int span = (int)(S_end - S);
+    // prefixes beyond the tile are clamped to MP_TILE + 1: only comparisons
+    // against slots <= MP_TILE are ever made
+    for (int j = t; j <= MP_TILE + 1; j += BLOCK) {
+      const int64_t k = k0 + j;
+      s_row[j] = (int)MIN(sl.at(k < N ? k : N) - S, (int64_t)MP_TILE + 1);
+    }
+    __syncthreads();
+    // Slot s of row k is edge s - k, so the tile's edges are the contiguous
+    // range [S - k0, S_end - k_end), k_end = owner of slot S_end (N at the
+    // end of the slot space).
+    int64_t k_end = (span < s_row[MP_TILE])
+                        ? k0 + owner_search(s_row, MP_TILE + 1, span)
+                        : k0 + MP_TILE;
+    if (S_end == total) k_end = N;
+    const int64_t e_lo = S - k0;
+    const int nedge = (int)((S_end - k_end) - e_lo);
+    for (int i = t; i < nedge; i += BLOCK) {
+      const int64_t e = e_lo + i;
+      s_msg[i] = body.msg(cols[e], e);
+    }
+    ne += (nedge + BLOCK - 1 - t) / BLOCK;
+    __syncthreads();
+    // B: sequential walk over [t*R, t*R + R)
+    const int r0 = t * MP_R, r1 = MIN(span, r0 + MP_R);
+    int tail_row = -1;
+    WA tail{};
+    int head_row = -1;
+    WA head{};
+    if (r0 < r1) {
+      int k = owner_search(s_row, MP_TILE + 1, r0);
+      int rs = s_row[k], re = s_row[k + 1] - 1;  // row start, closing slot
+      WA acc{};
+      bool has = false;
+      for (int i = r0; i < r1; ++i) {
+        if (i == re) {
+          const int64_t v = k0 + k;
+          if (rs >= r0) {  // the whole row lies in this lane's run
+            if (has && body.as.fetch(v, want) >= 0) {
+              if (exclusive) {
+                WA w = *body.f.wa_of((int)v);
+                body.f.comp(&w, acc, body.g);
+                *body.f.wa_of((int)v) = w;
+              } else {
+                body.f.compAtomic(body.f.wa_of((int)v), acc, body.g);
+              }
+            }
+          } else {
+            head_row = has ? k : -2 - k;  // -2-k: head without messages
+            head = acc;
+          }
+          ++k;
+          rs = s_row[k];
+          re = s_row[k + 1] - 1;
+          has = false;
+        } else {
+          const WA m = s_msg[i - k];  // edge (S + i) - (k0 + k), tile-relative
+          if (has)
+            body.f.comp(&acc, m, body.g);
+          else
+            acc = m;
+          has = true;
+        }
+      }
+      if (has) {  // row k is still open after the run
+        tail_row = k;
+        tail = acc;
+      }
+    }
+    s_tail_row[t] = tail_row;
+    s_tail[t] = tail;
+    __syncthreads();
+    // C: heads gather the tails before them
+    auto flush = [&](int k, WA val, bool inside) {
+      const int64_t v = k0 + k;
+      if (body.as.fetch(v, want) < 0) return;
+      if (exclusive && inside) {
+        WA w = *body.f.wa_of((int)v);
+        body.f.comp(&w, val, body.g);
+        *body.f.wa_of((int)v) = w;
+      } else {
+        body.f.compAtomic(body.f.wa_of((int)v), val, body.g);
+      }
+    };
+    if (head_row != -1) {
+      const int k = head_row >= 0 ? head_row : -2 - head_row;
+      bool has = head_row >= 0;
+      WA val = head;
+      for (int q = t - 1; q >= 0 && s_tail_row[q] == k; --q) {
+        if (has)
+          body.f.comp(&val, s_tail[q], body.g);
+        else
+          val = s_tail[q];
+        has = true;
+      }
+      if (has) flush(k, val, s_row[k] >= 0);
+    }
+    // the lane holding the tile's last slot closes a row that runs on
+    if (r0 < span && span <= r0 + MP_R && tail_row >= 0) {
+      WA val = tail;
+      for (int q = t - 1; q >= 0 && s_tail_row[q] == tail_row; --q)
+        body.f.comp(&val, s_tail[q], body.g);
+      flush(tail_row, val, false);
+    }
+    // owner of the next tile start
+    const int64_t nk = (MP_TILE < s_row[MP_TILE])
+                           ? k0 + owner_search(s_row, MP_TILE + 1, MP_TILE)
+                           : k0 + MP_TILE;
+    __syncthreads();
+    k0 = nk;
+  }
+  add_work(body.as.dg_work, ne, 0);
+}
+
+template <class B>
+__global__ void __launch_bounds__(BLOCK) k_pull_mp(B body) {
+  pull_mp_range(body, body.rows(), body.cols(), (int64_t)blockIdx.x,
+                (int64_t)gridDim.x, true);
+}
+
+template <class B>
+__global__ void __launch_bounds__(BLOCK) k_pull_mp_twod(B body, twod_t tw) {
+  const int p = (int)(blockIdx.x % (unsigned)tw.P);
+  pull_mp_range(body, tw.rp[p], tw.col[p], (int64_t)(blockIdx.x / tw.P),
+                (int64_t)(gridDim.x / tw.P), false);
+}
+
+template <class B>
+__global__ void __launch_bounds__(BLOCK)
+    k_pull_strict(B body, const int64_t *__restrict__ scan) {
+  pull_strict_range(body, slots_t{scan, body.rows()}, body.cols(),
+                    (int64_t)blockIdx.x, (int64_t)gridDim.x, true);
+}
+
+// TWOD: workgroup b serves source slice b % P (its XCD under round-robin
+// dispatch) as workgroup b / P of gridDim.x / P over that slice's slots.
+template <class B>
+__global__ void __launch_bounds__(BLOCK) k_pull_twod(B body, twod_t tw) {
+  const int p = (int)(blockIdx.x % (unsigned)tw.P);
+  pull_strict_range(body, slots_t{nullptr, tw.rp[p]}, tw.col[p],
+                    (int64_t)(blockIdx.x / tw.P), (int64_t)(gridDim.x / tw.P),
+                    false);
 }
 
 }  // namespace gsw

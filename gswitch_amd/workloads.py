@@ -28,6 +28,7 @@ def main(argv=None):
     ap.add_argument("--configs", default=None)
     ap.add_argument("--mib", type=int, default=1024)
     ap.add_argument("--records", action="store_true")
+    ap.add_argument("--pull-kernel", type=int, default=0)
     a = ap.parse_args(argv)
     if a.app == "copy":
         g = gsw.stream_copy(a.mib << 20, a.repeat)
@@ -48,7 +49,8 @@ def main(argv=None):
         elif a.app == "cc":
             g.cc(configs=a.configs, out=False)
         else:
-            g.pagerank(0.85, a.iters, configs=a.configs, out=False)
+            g.pagerank(0.85, a.iters, configs=a.configs, out=False,
+                       pull_kernel=a.pull_kernel)
         st = g.last_stats
         out = {"app": a.app, "scale": a.scale, "device_ms": st.device_ms,
                "levels": st.levels}

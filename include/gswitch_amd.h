@@ -95,7 +95,8 @@ typedef struct {
   int32_t twc_small;  /* TWC bins: degree < twc_small -> thread */
   int32_t twc_large;  /*           degree < twc_large -> wavefront */
   int32_t profile;    /* time every expand with hipEvents */
-  int32_t reserved[3];
+  int32_t pull_kernel; /* from-all pulls: 0 merge-path walk, 1 segmented reduction */
+  int32_t reserved[2];
 } gsw_run_opts;
 void gsw_run_opts_init(gsw_run_opts *o);
 

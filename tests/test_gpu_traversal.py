@@ -145,11 +145,12 @@ def max_rel(a, b):
 
 
 @pytest.mark.parametrize("variant", [None] + ["%s-Bitmap-%s-Standalone" % (d, l)
-                                              for d in DIRS for l in LBS])
-def test_pagerank(gsw, orc, gset, variant):
-    for gname in ["rmat12", "ba"]:
+                                              for d in DIRS for l in LBS + ["TWOD"]])
+@pytest.mark.parametrize("pull_kernel", [0, 1])
+def test_pagerank(gsw, orc, gset, variant, pull_kernel):
+    for gname in ["rmat12", "ba", "grid"]:
         h, g = gset[gname]
-        pr = g.pagerank(0.85, 50, configs=variant)
+        pr = g.pagerank(0.85, 50, configs=variant, pull_kernel=pull_kernel)
         ref = orc.pagerank(h.row_ptr, h.col, 0.85, 50)
         assert max_rel(pr, ref) <= PR_TOL, (variant, gname, max_rel(pr, ref))
 
