@@ -89,6 +89,8 @@ _SIGS = {
     "gsw_pagerank": (C.c_int32, [_P, C.c_float, C.c_int32, C.c_float,
                                  C.POINTER(RunOpts), _P, C.POINTER(RunStats)]),
     "gsw_last_records": (C.c_int32, [_P, _P, C.c_int32, C.POINTER(C.c_int32)]),
+    "gsw_msbfs64": (C.c_int32, [_P, _P, C.c_int32, C.POINTER(RunOpts), _P, _P,
+                                C.POINTER(RunStats)]),
     "gsw_tree_eval": (C.c_int32, [C.c_char_p, _P, C.c_int32,
                                   C.POINTER(C.c_int32)]),
     "gsw_scan_exclusive_i64": (C.c_int32, [_P, C.c_int64, _P]),
@@ -283,6 +285,18 @@ class Graph:
                                  C.byref(st)))
         self.last_stats = st
         return pr
+
+    def msbfs64(self, sources, out=True, **kw):
+        """64-source bitmask BFS: (masks uint64[n], per-source level sums)."""
+        src = np.ascontiguousarray(sources, np.int32)
+        o = run_opts(**kw)
+        st = RunStats()
+        masks = np.empty(self.n, np.uint64) if out else None
+        sums = np.zeros(64, np.uint64)
+        _check(_lib.gsw_msbfs64(self._h, _ptr(src), len(src), C.byref(o),
+                                _ptr(masks), _ptr(sums), C.byref(st)))
+        self.last_stats = st
+        return masks, sums
 
     def records(self):
         cap = max(1, self.last_stats.n_records if self.last_stats else 1 << 16)

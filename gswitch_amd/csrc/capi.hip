@@ -6,6 +6,7 @@
 #include <string>
 
 #include "apps.h"
+#include "gswitch/kernels/msbfs.h"
 #include "host/host_graph.h"
 
 using namespace gsw_apps;
@@ -116,6 +117,11 @@ struct gsw_graph {
   int32_t *h_flag = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<gsw_level_record> records;
+  // 64-source BFS state (allocated on first use)
+  uint64_t *ms_buf = nullptr;  // visit | A | B, n each
+  gsw::ms_counters_t *ms_cnt = nullptr;
+  gsw::ms_counters_t *ms_hcnt = nullptr;
+  int32_t *ms_src = nullptr;
 
   void release() {
     if (bfs) bfs->release();
@@ -130,6 +136,10 @@ struct gsw_graph {
     if (h_flag) hipHostFree(h_flag);
     if (ev0) hipEventDestroy(ev0);
     if (ev1) hipEventDestroy(ev1);
+    if (ms_buf) hipFree(ms_buf);
+    if (ms_cnt) hipFree(ms_cnt);
+    if (ms_hcnt) hipHostFree(ms_hcnt);
+    if (ms_src) hipFree(ms_src);
   }
 };
 
@@ -675,6 +685,96 @@ gsw_status gsw_pagerank(gsw_graph *G, float damping, int32_t iters, float eps,
 }
 
 // ---- selector trees ----------------------------------------------------------
+// ---- 64-source bitmask BFS ------------------------------------------------
+gsw_status gsw_msbfs64(gsw_graph *G, const int32_t *sources, int32_t nsrc,
+                       const gsw_run_opts *opts, uint64_t *mask_out,
+                       uint64_t *level_sums_out, gsw_run_stats *stats) {
+  return guarded([&] {
+    using namespace gsw;
+    GSW_REQUIRE(G && sources && nsrc >= 1 && nsrc <= 64, GSW_E_INVALID,
+                "need 1..64 sources");
+    const int64_t n = G->hg.n, m = G->hg.m;
+    for (int s = 0; s < nsrc; ++s)
+      GSW_REQUIRE(sources[s] >= 0 && sources[s] < n, GSW_E_INVALID,
+                  "source out of range");
+    const run_opts_t o = to_opts(opts);
+    if (!G->ms_buf) {
+      H_ERR(hipMalloc((void **)&G->ms_buf, sizeof(uint64_t) * 3 * (n + 1)));
+      H_ERR(hipMalloc((void **)&G->ms_cnt, sizeof(ms_counters_t)));
+      H_ERR(hipHostMalloc((void **)&G->ms_hcnt, sizeof(ms_counters_t)));
+      H_ERR(hipMalloc((void **)&G->ms_src, sizeof(int32_t) * 64));
+    }
+    uint64_t *visit = G->ms_buf, *front = G->ms_buf + (n + 1),
+             *next = G->ms_buf + 2 * (n + 1);
+    const uint64_t all = (nsrc == 64) ? ~0ull : ((1ull << nsrc) - 1);
+    const int64_t *rows = G->dg.dg_row_ptr;
+    const int32_t *cols = G->dg.dg_adj_list;
+    std::vector<uint64_t> sums(64, 0);
+    hipStream_t st = stream();
+    H_ERR(hipMemcpyAsync(G->ms_src, sources, sizeof(int32_t) * nsrc,
+                         hipMemcpyHostToDevice, st));
+    H_ERR(hipEventRecord(G->ev0, st));
+    fill_async<uint64_t>(visit, 0, 2 * (n + 1));  // visit and front
+    hipLaunchKernelGGL(k_msbfs_seed, dim3(1), dim3(64), 0, st, visit, front,
+                       (const int32_t *)G->ms_src, nsrc);
+    double front_deg = 0;
+    for (int s = 0; s < nsrc; ++s)
+      front_deg += (double)(G->hg.row_ptr[sources[s] + 1] - G->hg.row_ptr[sources[s]]);
+    G->records.clear();
+    int level = 0;
+    const int grid = grid_for((n + 63) / 64 * 64);
+    while (level < o.max_levels) {
+      ++level;
+      // direction: forced ("Push-..." / "Pull-..."), else pull once the
+      // frontier's edges exceed 1/32 of the graph
+      const bool pull = o.force.has_direction ? (o.force.d == Pull)
+                                              : front_deg > (double)m / 32.0;
+      fill_async<unsigned long long>((unsigned long long *)G->ms_cnt, 0ull,
+                                     sizeof(ms_counters_t) / 8);
+      if (pull) {
+        hipLaunchKernelGGL(k_msbfs_pull, dim3(grid), dim3(BLOCK), 0, st, rows,
+                           cols, n, all, visit, (const uint64_t *)front, next,
+                           G->ms_cnt);
+      } else {
+        fill_async<uint64_t>(next, 0, n + 1);
+        hipLaunchKernelGGL(k_msbfs_push, dim3(grid), dim3(BLOCK), 0, st, rows,
+                           cols, n, (const uint64_t *)visit,
+                           (const uint64_t *)front, next);
+        hipLaunchKernelGGL(k_msbfs_sweep, dim3(grid), dim3(BLOCK), 0, st, rows,
+                           n, visit, next, G->ms_cnt);
+      }
+      H_ERR(hipGetLastError());
+      H_ERR(hipMemcpyAsync(G->ms_hcnt, G->ms_cnt, sizeof(ms_counters_t),
+                           hipMemcpyDeviceToHost, st));
+      sync();
+      gsw_level_record rec{};
+      rec.level = level;
+      rec.dir = pull ? 1 : 0;
+      rec.fmt = pull ? 1 : 0;
+      rec.frontier = (int64_t)G->ms_hcnt->newly;
+      rec.edges = (int64_t)G->ms_hcnt->newly_deg;
+      G->records.push_back(rec);
+      if (G->ms_hcnt->newly == 0) break;
+      for (int s = 0; s < 64; ++s) sums[s] += (uint64_t)level * G->ms_hcnt->per_src[s];
+      front_deg = (double)G->ms_hcnt->newly_deg;
+      std::swap(front, next);
+    }
+    H_ERR(hipEventRecord(G->ev1, st));
+    const double ms = elapsed(G);
+    if (mask_out)
+      H_ERR(hipMemcpy(mask_out, visit, sizeof(uint64_t) * n,
+                      hipMemcpyDeviceToHost));
+    if (level_sums_out)
+      for (int s = 0; s < 64; ++s) level_sums_out[s] = sums[s];
+    if (stats) {
+      stats->device_ms = ms;
+      stats->levels = level;
+      stats->n_records = (int32_t)G->records.size();
+      stats->reached = 0;
+    }
+  });
+}
+
 gsw_status gsw_tree_eval(const char *name, const double *fv, int32_t nfv,
                          int32_t *cls) {
   return guarded([&] {

@@ -147,6 +147,14 @@ gsw_status gsw_cc(gsw_graph *g, const gsw_run_opts *opts, int32_t *labels_out,
 gsw_status gsw_pagerank(gsw_graph *g, float damping, int32_t iters, float eps,
                         const gsw_run_opts *opts, float *pr_out,
                         gsw_run_stats *stats);
+/* 64-source bitmask BFS: bit s of mask_out[v] is set iff v is reachable from
+ * sources[s] (1 <= nsrc <= 64). level_sums_out[s] (64 entries, optional) =
+ * sum over reached v of the BFS level of v from sources[s]. No counterpart
+ * in the reference, whose application/mbfs.cu:122-153 re-runs the
+ * single-source BFS; results equal nsrc independent gsw_bfs runs. */
+gsw_status gsw_msbfs64(gsw_graph *g, const int32_t *sources, int32_t nsrc,
+                       const gsw_run_opts *opts, uint64_t *mask_out,
+                       uint64_t *level_sums_out, gsw_run_stats *stats);
 /* Records of the last run on `g`; *n = number copied (<= cap). */
 gsw_status gsw_last_records(const gsw_graph *g, gsw_level_record *out,
                             int32_t cap, int32_t *n);
