@@ -37,7 +37,8 @@ class RunOpts(C.Structure):
     _fields_ = [("configs", C.c_char_p), ("max_levels", C.c_int32),
                 ("verbose", C.c_int32), ("twc_small", C.c_int32),
                 ("twc_large", C.c_int32), ("profile", C.c_int32),
-                ("pull_kernel", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("pull_kernel", C.c_int32), ("capture", C.c_int32),
+                ("grid", C.c_int32)]
 
 
 class RunStats(C.Structure):
@@ -70,6 +71,9 @@ _SIGS = {
                                C.POINTER(_P)]),
     "gsw_gen_ba": (C.c_int32, [C.c_int64, C.c_int32, C.c_uint64, C.c_int32,
                                C.POINTER(_P)]),
+    "gsw_gen_er_batch": (C.c_int32, [C.c_int32, C.c_int64, C.c_int64, C.c_double,
+                                     C.c_double, C.c_uint64, C.c_int32,
+                                     C.POINTER(_P), _P]),
     "gsw_host_graph_info": (C.c_int32, [_P, C.POINTER(C.c_int64),
                                         C.POINTER(C.c_int64),
                                         C.POINTER(C.c_int32),
@@ -91,6 +95,12 @@ _SIGS = {
     "gsw_last_records": (C.c_int32, [_P, _P, C.c_int32, C.POINTER(C.c_int32)]),
     "gsw_msbfs64": (C.c_int32, [_P, _P, C.c_int32, C.POINTER(RunOpts), _P, _P,
                                 C.POINTER(RunStats)]),
+    "gsw_bfs_multi": (C.c_int32, [_P, _P, C.c_int32, C.POINTER(RunOpts), _P,
+                                  C.POINTER(RunStats)]),
+    "gsw_autotune_bfs": (C.c_int32, [_P, _P, C.c_int32, C.c_int32,
+                                     C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_double)]),
     "gsw_tree_eval": (C.c_int32, [C.c_char_p, _P, C.c_int32,
                                   C.POINTER(C.c_int32)]),
     "gsw_scan_exclusive_i64": (C.c_int32, [_P, C.c_int64, _P]),
@@ -179,6 +189,16 @@ def barabasi_albert(n, m, seed=6, weighted=False):
     return _gen(_lib.gsw_gen_ba, n, m, seed, int(weighted))
 
 
+def erdos_renyi_batch(count, nmin=1000, nmax=200000, dmin=2.0, dmax=32.0, seed=5,
+                      weighted=False):
+    """`count` ER instances as one disjoint union: (HostGraph, offsets[count+1])."""
+    off = np.empty(count + 1, np.int64)
+    h = _P()
+    _check(_lib.gsw_gen_er_batch(count, nmin, nmax, dmin, dmax, seed, int(weighted),
+                                 C.byref(h), _ptr(off)))
+    return HostGraph(h), off
+
+
 def from_edges(n, src, dst, w=None, directed=False):
     src = np.ascontiguousarray(src, np.int32)
     dst = np.ascontiguousarray(dst, np.int32)
@@ -205,7 +225,7 @@ def tree_eval(name, fv):
 
 # --------------------------------------------------------------- device graphs
 def run_opts(configs=None, max_levels=0, verbose=False, twc_small=0,
-             twc_large=0, profile=True, pull_kernel=0):
+             twc_large=0, profile=True, pull_kernel=0, capture=0, grid=0):
     o = RunOpts()
     _lib.gsw_run_opts_init(C.byref(o))
     o.configs = configs.encode() if configs else None
@@ -218,6 +238,8 @@ def run_opts(configs=None, max_levels=0, verbose=False, twc_small=0,
         o.twc_large = twc_large
     o.profile = int(profile)
     o.pull_kernel = int(pull_kernel)
+    o.capture = int(capture)
+    o.grid = int(grid)
     return o
 
 
@@ -297,6 +319,30 @@ class Graph:
                                 _ptr(masks), _ptr(sums), C.byref(st)))
         self.last_stats = st
         return masks, sums
+
+    def bfs_multi(self, roots, configs=None, out=True, **kw):
+        """BFS levels from the nearest of `roots`; capture=L replays a captured
+        loop of 2L fused levels per launch."""
+        r = np.ascontiguousarray(roots, np.int32)
+        o = self._opts(configs, kw)
+        st = RunStats()
+        lv = np.empty(self.n, np.int32) if out else None
+        _check(_lib.gsw_bfs_multi(self._h, _ptr(r), len(r), C.byref(o), _ptr(lv),
+                                  C.byref(st)))
+        self.last_stats = st
+        return lv
+
+    def autotune_bfs(self, roots, capture=8):
+        """Fastest captured-loop shape for these roots:
+        dict(lb, twc_small, twc_large, grid, ms)."""
+        r = np.ascontiguousarray(roots, np.int32)
+        lb, s, l, g = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        ms = C.c_double()
+        _check(_lib.gsw_autotune_bfs(self._h, _ptr(r), len(r), capture, C.byref(lb),
+                                     C.byref(s), C.byref(l), C.byref(g),
+                                     C.byref(ms)))
+        return dict(lb=LB_NAMES[lb.value], twc_small=s.value, twc_large=l.value,
+                    grid=g.value, ms=ms.value)
 
     def records(self):
         cap = max(1, self.last_stats.n_records if self.last_stats else 1 << 16)

@@ -1,6 +1,7 @@
 // C-ABI of the traversal engine (declarations: include/gswitch_amd.h).
 #include "../../include/gswitch_amd.h"
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -47,6 +48,21 @@ __global__ void k_init_bfs(int *lvl, int64_t n, int root) {
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += (int64_t)gridDim.x * blockDim.x)
     lvl[v] = (v == root) ? 0 : -1;
+}
+__global__ void k_set_root_levels(int *lvl, const int32_t *roots, int64_t k) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k;
+       i += (int64_t)gridDim.x * blockDim.x)
+    lvl[roots[i]] = 0;
+}
+// End of one captured BFS level that consumed queue `cur`: the level moves
+// on only if the level produced a frontier (so *level ends as the number of
+// expanded non-empty frontiers), and the consumed size is cleared for the
+// level after next.
+__global__ void k_level_advance(int *level, int32_t *sizes, int cur) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (sizes[cur ^ 1] != 0) *level += 1;
+    sizes[cur] = 0;
+  }
 }
 __global__ void k_init_sssp(uint32_t *wa, uint32_t *ra, int64_t n, int root) {
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
@@ -122,8 +138,27 @@ struct gsw_graph {
   gsw::ms_counters_t *ms_cnt = nullptr;
   gsw::ms_counters_t *ms_hcnt = nullptr;
   int32_t *ms_src = nullptr;
+  // multi-root BFS state: roots on the device, the device level counter and
+  // the captured level loop (instantiated once per launch shape)
+  int32_t *d_roots = nullptr;
+  int64_t roots_cap = 0;
+  int32_t *d_level = nullptr;
+  hipGraph_t cap_graph = nullptr;
+  hipGraphExec_t cap_exec = nullptr;
+  int cap_key[5] = {-1, -1, -1, -1, -1};
+
+  void drop_capture() {
+    if (cap_exec) hipGraphExecDestroy(cap_exec);
+    if (cap_graph) hipGraphDestroy(cap_graph);
+    cap_exec = nullptr;
+    cap_graph = nullptr;
+    for (int &x : cap_key) x = -1;
+  }
 
   void release() {
+    drop_capture();
+    if (d_roots) hipFree(d_roots);
+    if (d_level) hipFree(d_level);
     if (bfs) bfs->release();
     if (sssp) sssp->release();
     if (cc) cc->release();
@@ -247,6 +282,181 @@ double elapsed(gsw_graph *G) {
   return ms;
 }
 
+app_ctx_t<BFS> &bfs_ctx(gsw_graph *G) {
+  if (!G->bfs) {
+    G->bfs = std::make_unique<app_ctx_t<BFS>>();
+    G->bfs->as.build(G->hg.n, Normal);
+    G->bfs->f.data.build(G->hg.n);
+  }
+  return *G->bfs;
+}
+
+// Validate, deduplicate and upload the roots of a multi-root run; returns
+// the distinct roots (ascending) and their total out-degree.
+std::vector<int32_t> upload_roots(gsw_graph *G, const int32_t *roots,
+                                  int32_t nroots, double &workload) {
+  GSW_REQUIRE(roots && nroots >= 1, GSW_E_INVALID, "no roots");
+  std::vector<int32_t> r(roots, roots + nroots);
+  for (int32_t v : r)
+    GSW_REQUIRE(v >= 0 && v < G->hg.n, GSW_E_INVALID, "root out of range");
+  std::sort(r.begin(), r.end());
+  r.erase(std::unique(r.begin(), r.end()), r.end());
+  if ((int64_t)r.size() > G->roots_cap) {
+    if (G->d_roots) H_ERR(hipFree(G->d_roots));
+    G->d_roots = nullptr;
+    H_ERR(hipMalloc((void **)&G->d_roots, sizeof(int32_t) * r.size()));
+    G->roots_cap = (int64_t)r.size();
+  }
+  H_ERR(hipMemcpy(G->d_roots, r.data(), sizeof(int32_t) * r.size(),
+                  hipMemcpyHostToDevice));
+  workload = 0;
+  for (int32_t v : r)
+    workload += (double)(G->hg.row_ptr[v + 1] - G->hg.row_ptr[v]);
+  return r;
+}
+
+// BFS levels: -1 everywhere, 0 on the roots (one root, or k device roots)
+void init_bfs_levels(gsw_graph *G, int *lvl, int root, const int32_t *d_roots,
+                     int64_t k) {
+  hipStream_t st = gsw::stream();
+  hipLaunchKernelGGL(k_init_bfs, dim3(gsw::grid_for(G->hg.n)), dim3(gsw::BLOCK),
+                     0, st, lvl, G->hg.n, d_roots ? -1 : root);
+  if (d_roots)
+    hipLaunchKernelGGL(k_set_root_levels, dim3(gsw::grid_for(k)),
+                       dim3(gsw::BLOCK), 0, st, lvl, d_roots, k);
+  H_ERR(hipGetLastError());
+}
+
+// The per-level engine loop for BFS from one root (d_roots == nullptr) or
+// from k distinct device roots.
+int bfs_engine_run(gsw_graph *G, const gsw_run_opts *opts, int32_t root,
+                   const int32_t *d_roots, int64_t k, double workload) {
+  const run_opts_t o = to_opts(opts);
+  auto &c = bfs_ctx(G);
+  arch_once(c);
+  prepare_engine(c, o, !opts || opts->profile);
+  feature_t &fets = c.en.fets;
+  fets.centric = VC;
+  fets.pattern = Idem;
+  fets.fromall = false;
+  fets.toall = false;
+  fets.use_root = root;
+  if (d_roots) {
+    fets.first_active = (double)k;
+    fets.first_workload = workload;
+  }
+  fets.set_graph((double)G->hg.n, (double)(G->hg.m / 2), G->attr);
+  c.en.conf.conf_pruning = true;
+  view_t<GU> gv{&G->hg, G->attr, 0.0};
+  init_conf(c.en.stats, fets, c.en.conf, gv, c.f);
+
+  H_ERR(hipEventRecord(G->ev0, gsw::stream()));
+  c.as.reset();
+  if (d_roots)
+    c.as.init_roots(d_roots, k);
+  else
+    c.as.init(root);
+  init_bfs_levels(G, c.f.data.dg_wa, root, d_roots, k);
+  GU g = G->dg;
+  const int levels = run_vc(c.en, c.as, g, c.f, o, [](int) { return false; });
+  H_ERR(hipEventRecord(G->ev1, gsw::stream()));
+  return levels;
+}
+
+// Multi-root BFS as a captured level loop: every level is a fused push
+// expand over the unsorted queue (WM, or TWC degree bins) followed by
+// k_level_advance; the BFS level lives in device memory, so `per_launch`
+// levels replay as one hipGraph with no host round trip between them. The
+// host reads the next queue length once per replay and stops at zero.
+int bfs_captured_run(gsw_graph *G, int64_t k, LB lb, int small, int large,
+                     int grid, int per_launch, int max_levels) {
+  GSW_REQUIRE(lb == WM || lb == TWC, GSW_E_UNSUPPORTED,
+              "captured BFS supports the WM and TWC expands");
+  GSW_REQUIRE(per_launch >= 2 && per_launch % 2 == 0 && grid >= 1 &&
+                  grid <= gsw::MAX_GRID && small >= 1 && large > small,
+              GSW_E_INVALID, "invalid captured-loop shape");
+  auto &c = bfs_ctx(G);
+  active_set_t &as = c.as;
+  GSW_REQUIRE(as.queue.mode == Normal && as.queue.buf[1], GSW_E_INTERNAL,
+              "captured BFS needs double-buffered queues");
+  GSW_REQUIRE(k <= as.queue.capacity, GSW_E_OVERFLOW, "too many roots");
+  if (lb == TWC) as.bins.ensure(as.size);
+  if (!G->d_level) H_ERR(hipMalloc((void **)&G->d_level, sizeof(int32_t) * 2));
+  hipStream_t st = gsw::stream();
+
+  H_ERR(hipEventRecord(G->ev0, st));
+  as.reset();
+  as.set_fmt(Queue);
+  H_ERR(hipMemcpyAsync(as.queue.buf[0], G->d_roots, sizeof(int32_t) * k,
+                       hipMemcpyDeviceToDevice, st));
+  gsw::set_one_async<int32_t>(as.queue.sizes, (int32_t)k);
+  gsw::set_one_async<int32_t>(G->d_level, 1);
+  init_bfs_levels(G, c.f.data.dg_wa, -1, G->d_roots, k);
+
+  const int key[5] = {(int)lb, small, large, grid, per_launch};
+  if (!G->cap_exec || std::memcmp(key, G->cap_key, sizeof(key)) != 0) {
+    G->drop_capture();
+    config_t conf{};
+    conf.conf_dir = Push;
+    conf.conf_lb = lb;
+    conf.conf_fuse_inspect = true;
+    conf.conf_pruning = true;
+    conf.conf_toall = false;
+    conf.conf_target = Active;
+    GU g = G->dg;
+    g.level_ptr = G->d_level;
+    using Bd = gsw::push_body<GU, BFS, true>;
+    H_ERR(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int j = 0; j < per_launch; ++j) {
+        active_set_t a = as;
+        a.queue.cursor = j & 1;
+        Bd body{a, g, c.f, conf};
+        if (lb == TWC) {
+          gsw::fill_async<int32_t>(as.bins.sizes, 0, 4);
+          hipLaunchKernelGGL(gsw::k_twc_bin<GU>, dim3(grid), dim3(gsw::BLOCK), 0,
+                             st, a, g, Active, false, small, large);
+          hipLaunchKernelGGL(gsw::k_push_twc<Bd>, dim3(grid), dim3(gsw::BLOCK),
+                             0, st, body);
+        } else {
+          hipLaunchKernelGGL(gsw::k_push_wm<Bd>, dim3(grid), dim3(gsw::BLOCK), 0,
+                             st, body);
+        }
+        hipLaunchKernelGGL(k_level_advance, dim3(1), dim3(64), 0, st,
+                           G->d_level, as.queue.sizes, j & 1);
+        H_ERR(hipGetLastError());
+      }
+    } catch (...) {
+      hipGraph_t dead = nullptr;
+      (void)hipStreamEndCapture(st, &dead);
+      if (dead) (void)hipGraphDestroy(dead);
+      throw;
+    }
+    H_ERR(hipStreamEndCapture(st, &G->cap_graph));
+    H_ERR(hipGraphInstantiate(&G->cap_exec, G->cap_graph, nullptr, nullptr, 0));
+    std::memcpy(G->cap_key, key, sizeof(key));
+  }
+
+  int32_t *h = G->h_flag;
+  for (int done = 0; done < max_levels; done += per_launch) {
+    H_ERR(hipGraphLaunch(G->cap_exec, st));
+    H_ERR(hipMemcpyAsync(h, as.queue.sizes, sizeof(int32_t),
+                         hipMemcpyDeviceToHost, st));
+    H_ERR(hipMemcpyAsync(h + 1, as.dg_ctrl + CTRL_OVERFLOW, sizeof(int32_t),
+                         hipMemcpyDeviceToHost, st));
+    gsw::sync();
+    GSW_REQUIRE(h[1] == 0, GSW_E_OVERFLOW, "frontier queue overflow");
+    if (h[0] == 0) break;
+  }
+  H_ERR(hipEventRecord(G->ev1, st));
+  H_ERR(hipMemcpy(h, G->d_level, sizeof(int32_t), hipMemcpyDeviceToHost));
+  return h[0];
+}
+
+int default_capture_grid(const gsw_graph *G) {
+  return gsw::grid_for(G->hg.n, gsw::BLOCK, gsw::MAX_GRID);
+}
+
 }  // namespace
 
 extern "C" {
@@ -350,6 +560,22 @@ gsw_status gsw_gen_ba(int64_t n, int32_t m, uint64_t seed, int32_t weighted,
     std::vector<int32_t> s, d;
     gsw_host::gen_ba(n, m, seed, s, d);
     finish_gen(n, s, d, seed, weighted, out);
+  });
+}
+
+gsw_status gsw_gen_er_batch(int32_t count, int64_t nmin, int64_t nmax,
+                            double dmin, double dmax, uint64_t seed,
+                            int32_t weighted, gsw_host_graph **out,
+                            int64_t *offsets_out) {
+  return guarded([&] {
+    GSW_REQUIRE(out && offsets_out && count >= 1 && nmin >= 1 && nmax >= nmin &&
+                    nmax <= INT32_MAX && dmin >= 0 && dmax >= dmin,
+                GSW_E_INVALID, "invalid ER batch parameters");
+    std::vector<int32_t> s, d;
+    std::vector<int64_t> off;
+    gsw_host::gen_er_batch(count, nmin, nmax, dmin, dmax, seed, s, d, off);
+    finish_gen(off.back(), s, d, seed, weighted, out);
+    std::memcpy(offsets_out, off.data(), sizeof(int64_t) * off.size());
   });
 }
 
@@ -471,36 +697,9 @@ gsw_status gsw_bfs(gsw_graph *G, int32_t root, const gsw_run_opts *opts,
   return guarded([&] {
     GSW_REQUIRE(G, GSW_E_INVALID, "null graph");
     GSW_REQUIRE(root >= 0 && root < G->hg.n, GSW_E_INVALID, "root out of range");
-    const run_opts_t o = to_opts(opts);
-    if (!G->bfs) {
-      G->bfs = std::make_unique<app_ctx_t<BFS>>();
-      G->bfs->as.build(G->hg.n, Normal);
-      G->bfs->f.data.build(G->hg.n);
-    }
-    auto &c = *G->bfs;
-    arch_once(c);
-    prepare_engine(c, o, !opts || opts->profile);
-    feature_t &fets = c.en.fets;
-    fets.centric = VC;
-    fets.pattern = Idem;
-    fets.fromall = false;
-    fets.toall = false;
-    fets.use_root = root;
-    fets.set_graph((double)G->hg.n, (double)(G->hg.m / 2), G->attr);
-    c.en.conf.conf_pruning = true;
-    view_t<GU> gv{&G->hg, G->attr, 0.0};
-    init_conf(c.en.stats, fets, c.en.conf, gv, c.f);
-
-    hipStream_t st = gsw::stream();
-    H_ERR(hipEventRecord(G->ev0, st));
-    c.as.reset();
-    c.as.init(root);
-    hipLaunchKernelGGL(k_init_bfs, dim3(gsw::grid_for(G->hg.n)), dim3(gsw::BLOCK),
-                       0, st, c.f.data.dg_wa, G->hg.n, root);
-    GU g = G->dg;
-    int levels = run_vc(c.en, c.as, g, c.f, o, [](int) { return false; });
-    H_ERR(hipEventRecord(G->ev1, st));
+    const int levels = bfs_engine_run(G, opts, root, nullptr, 1, 0.0);
     const double ms = elapsed(G);
+    auto &c = *G->bfs;
     collect_records(G, c.en.executor, c.as);
     if (levels_out)
       H_ERR(hipMemcpy(levels_out, c.f.data.dg_wa, sizeof(int32_t) * G->hg.n,
@@ -511,6 +710,83 @@ gsw_status gsw_bfs(gsw_graph *G, int32_t root, const gsw_run_opts *opts,
       stats->n_records = (int32_t)G->records.size();
       stats->reached = 0;
     }
+  });
+}
+
+// ---- batched / multi-root BFS ---------------------------------------------
+gsw_status gsw_bfs_multi(gsw_graph *G, const int32_t *roots, int32_t nroots,
+                         const gsw_run_opts *opts, int32_t *levels_out,
+                         gsw_run_stats *stats) {
+  return guarded([&] {
+    GSW_REQUIRE(G, GSW_E_INVALID, "null graph");
+    double workload = 0;
+    const std::vector<int32_t> r = upload_roots(G, roots, nroots, workload);
+    const int64_t k = (int64_t)r.size();
+    int levels = 0;
+    if (opts && opts->capture > 0) {
+      const run_opts_t o = to_opts(opts);
+      LB lb = WM;
+      if (o.force.has_lb) {
+        GSW_REQUIRE(o.force.lb == WM || o.force.lb == TWC, GSW_E_UNSUPPORTED,
+                    "captured BFS supports the WM and TWC expands");
+        lb = o.force.lb;
+      }
+      const int grid = opts->grid > 0 ? opts->grid : default_capture_grid(G);
+      levels = bfs_captured_run(G, k, lb, o.twc_small, o.twc_large, grid,
+                                2 * opts->capture, o.max_levels);
+      G->records.clear();
+    } else {
+      levels = bfs_engine_run(G, opts, r[0], G->d_roots, k, workload);
+      collect_records(G, G->bfs->en.executor, G->bfs->as);
+    }
+    const double ms = elapsed(G);
+    if (levels_out)
+      H_ERR(hipMemcpy(levels_out, G->bfs->f.data.dg_wa,
+                      sizeof(int32_t) * G->hg.n, hipMemcpyDeviceToHost));
+    if (stats) {
+      stats->device_ms = ms;
+      stats->levels = levels;
+      stats->n_records = (int32_t)G->records.size();
+      stats->reached = 0;
+    }
+  });
+}
+
+gsw_status gsw_autotune_bfs(gsw_graph *G, const int32_t *roots, int32_t nroots,
+                            int32_t capture, int32_t *best_lb,
+                            int32_t *best_small, int32_t *best_large,
+                            int32_t *best_grid, double *best_ms) {
+  return guarded([&] {
+    GSW_REQUIRE(G && capture > 0 && best_lb && best_small && best_large &&
+                    best_grid && best_ms,
+                GSW_E_INVALID, "invalid arguments");
+    double workload = 0;
+    const std::vector<int32_t> r = upload_roots(G, roots, nroots, workload);
+    const int64_t k = (int64_t)r.size();
+    struct cand_t { LB lb; int small, large; };
+    const cand_t cands[] = {{WM, 32, 2048},   {TWC, 16, 1024}, {TWC, 32, 2048},
+                            {TWC, 64, 4096},  {TWC, 128, 8192}};
+    const int gmax = default_capture_grid(G);
+    int grids[3] = {MAX(1, gmax / 8), MAX(1, gmax / 2), gmax};
+    double best = 1e300;
+    for (const cand_t &cd : cands)
+      for (int gi = 0; gi < 3; ++gi) {
+        if (gi > 0 && grids[gi] == grids[gi - 1]) continue;
+        double t = 1e300;
+        for (int rep = 0; rep < 3; ++rep) {
+          bfs_captured_run(G, k, cd.lb, cd.small, cd.large, grids[gi],
+                           2 * capture, 1 << 20);
+          t = MIN(t, elapsed(G));
+        }
+        if (t < best) {
+          best = t;
+          *best_lb = (int32_t)cd.lb;
+          *best_small = cd.small;
+          *best_large = cd.large;
+          *best_grid = grids[gi];
+        }
+      }
+    *best_ms = best;
   });
 }
 

@@ -36,6 +36,7 @@ struct feature_t {
   double avg_filter_time = 0, avg_expand_time = 0;
   double growing_rate = 0;
   double first_workload = 0;
+  double first_active = 1;  // roots of the first frontier (multi-root runs)
 
   // Architecture
   double cap = 0;

@@ -34,7 +34,7 @@ struct inspector_t {
     if (conf.conf_first_round) {
       // the initial frontier is known exactly (root or everything), also
       // when the first level is fused and features are not synthesised
-      known_active = fets.use_root < 0 ? g.nvertexs : 1;
+      known_active = fets.use_root < 0 ? g.nvertexs : (long long)fets.first_active;
       known_inactive = g.nvertexs - known_active;
     }
     if (need && conf.conf_first_round) {
@@ -100,8 +100,8 @@ struct inspector_t {
       fets.push_workload = m;
       fets.pull_workload = 0;
     } else {
-      fets.active_vertex = 1;
-      fets.unactive_vertex = n - 1;
+      fets.active_vertex = fets.first_active;
+      fets.unactive_vertex = n - fets.first_active;
       fets.push_workload = fets.first_workload;
       fets.pull_workload = m - fets.first_workload;
     }

@@ -60,6 +60,18 @@ struct bins_t {
   }
 };
 
+// mark roots[0..k) active (and not inactive) in both frontier bitmaps
+__global__ void __launch_bounds__(256)
+    k_set_roots(uint64_t *active, uint64_t *inactive, const int32_t *roots,
+                int64_t k) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int v = roots[i];
+    atomicOr((unsigned long long *)&active[v >> 6], 1ull << (v & 63));
+    atomicOr((unsigned long long *)&inactive[v >> 6], 1ull << (v & 63));
+  }
+}
+
 enum : int { CTRL_HALT = 0, CTRL_OVERFLOW = 1, CTRL_FLAG = 2, CTRL_N = 8 };
 enum : int { WORK_EDGES = 0, WORK_VERTS = 1, WORK_N = 4 };
 constexpr int WORK_ROWS = 16384;  // per-expand counter rows kept per traversal
@@ -152,6 +164,20 @@ struct active_set_t {
     } else {
       gsw::fill_async<uint64_t>(bitmap.active.dg_bits, 0, nw + 1);
       gsw::fill_async<uint64_t>(bitmap.inactive.dg_bits, 0, nw + 1);
+    }
+  }
+
+  // Several roots at once (a batch of disjoint instances, or multi-source
+  // traversals): `d_roots` holds k distinct vertex ids on the device.
+  void init_roots(const int32_t *d_roots, int64_t k) {
+    const int64_t nw = bitmap.active.n_words;
+    gsw::fill_async<uint64_t>(bitmap.active.dg_bits, 0, nw + 1);
+    gsw::fill_async<uint64_t>(bitmap.inactive.dg_bits, 0, nw + 1);
+    if (k > 0) {
+      hipLaunchKernelGGL(k_set_roots, dim3(gsw::grid_for(k)), dim3(256), 0,
+                         gsw::stream(), bitmap.active.dg_bits,
+                         bitmap.inactive.dg_bits, d_roots, k);
+      H_ERR(hipGetLastError());
     }
   }
 

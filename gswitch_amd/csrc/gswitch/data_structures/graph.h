@@ -167,9 +167,14 @@ template <typename E> struct device_graph_t<CSR, E> {
     if constexpr (has_edata) return dgr_edgedata + eid;
     else return nullptr;
   }
-  __host__ __device__ __forceinline__ int get_level() const { return level; }
+  // The level lives in the struct (passed by value, as in the reference) or,
+  // for captured level loops, in device memory that a kernel advances.
+  __device__ __forceinline__ int get_level() const {
+    return level_ptr ? *level_ptr : level;
+  }
   __host__ __device__ inline void update_level(int inc = 1) { level += inc; }
   __host__ __device__ inline void reset_level() { level = 0; }
+  int *level_ptr = nullptr;
 
  private:
   int64_t upload_w(const std::vector<uint32_t> &w, E **dst) {

@@ -54,7 +54,7 @@ void init_conf(stat_t &stats, feature_t &fets, config_t &conf, Gr &g, F &f) {
   stats.build();
   fets.architecture_features();
   conf.conf_inherit = false;
-  if (fets.use_root >= 0 && g.hg)
+  if (fets.use_root >= 0 && g.hg && fets.first_active <= 1)
     fets.first_workload = (double)(g.hg->row_ptr[fets.use_root + 1] -
                                    g.hg->row_ptr[fets.use_root]);
   if (fets.centric == VC) {

@@ -294,3 +294,51 @@ void gen_ba(int64_t n, int m, uint64_t seed, std::vector<int32_t> &src,
 }
 
 }  // namespace gsw_host
+
+namespace gsw_host {
+
+// Instance i: u1, u2 = the top 53 bits of draw(batch_key, 2i) and
+// draw(batch_key, 2i + 1) as uniforms in [0, 1); n_i = floor(exp(ln nmin +
+// u1 (ln nmax - ln nmin))), d_i = dmin + u2 (dmax - dmin); its edges are
+// gen_er(n_i, d_i, seed_i) with seed_i = stream_key(seed, 1000 + i), shifted
+// by the instance's vertex offset.
+void gen_er_batch(int count, int64_t nmin, int64_t nmax, double dmin,
+                  double dmax, uint64_t seed, std::vector<int32_t> &src,
+                  std::vector<int32_t> &dst, std::vector<int64_t> &offsets) {
+  if (count < 1 || nmin < 1 || nmax < nmin)
+    throw std::invalid_argument("invalid batch parameters");
+  const uint64_t key = stream_key(seed, STREAM_BATCH);
+  std::vector<int64_t> ns(count);
+  std::vector<double> ds(count);
+  offsets.assign(count + 1, 0);
+  const double l0 = std::log((double)nmin), l1 = std::log((double)nmax);
+  for (int i = 0; i < count; ++i) {
+    const double u1 = (double)(draw(key, 2 * (uint64_t)i) >> 11) * 0x1.0p-53;
+    const double u2 = (double)(draw(key, 2 * (uint64_t)i + 1) >> 11) * 0x1.0p-53;
+    ns[i] = std::max<int64_t>(1, (int64_t)std::floor(std::exp(l0 + u1 * (l1 - l0))));
+    ns[i] = std::min(ns[i], nmax);
+    ds[i] = dmin + u2 * (dmax - dmin);
+    offsets[i + 1] = offsets[i] + ns[i];
+  }
+  if (offsets[count] > (int64_t)INT32_MAX)
+    throw std::invalid_argument("batch exceeds int32 vertex ids");
+  // edge counts per instance, then every instance generated in parallel
+  std::vector<int64_t> eo(count + 1, 0);
+  for (int i = 0; i < count; ++i)
+    eo[i + 1] = eo[i] + (int64_t)std::llround((double)ns[i] * ds[i] / 2.0);
+  src.resize((size_t)eo[count]);
+  dst.resize((size_t)eo[count]);
+  const int T = num_threads();
+  run_threads(T, [&](int t) {
+    std::vector<int32_t> s, d;
+    for (int i = t; i < count; i += T) {
+      gen_er(ns[i], ds[i], stream_key(seed, 1000 + (uint64_t)i), s, d);
+      for (size_t j = 0; j < s.size(); ++j) {
+        src[eo[i] + j] = (int32_t)(s[j] + offsets[i]);
+        dst[eo[i] + j] = (int32_t)(d[j] + offsets[i]);
+      }
+    }
+  });
+}
+
+}  // namespace gsw_host

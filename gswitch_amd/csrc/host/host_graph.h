@@ -39,6 +39,7 @@ enum : uint64_t {
   STREAM_GRID_LONG = 4,
   STREAM_ER = 5,
   STREAM_BA = 6,
+  STREAM_BATCH = 7,
 };
 
 // Symmetric edge weight in [1, lim] for the canonical pair (lo, hi).
@@ -79,6 +80,11 @@ void gen_er(int64_t n, double mean_degree, uint64_t seed,
             std::vector<int32_t> &src, std::vector<int32_t> &dst);
 void gen_ba(int64_t n, int m, uint64_t seed, std::vector<int32_t> &src,
             std::vector<int32_t> &dst);
+// `count` Erdos-Renyi instances as one disjoint union (see gsw_gen_er_batch);
+// offsets receives count + 1 vertex offsets.
+void gen_er_batch(int count, int64_t nmin, int64_t nmax, double dmin,
+                  double dmax, uint64_t seed, std::vector<int32_t> &src,
+                  std::vector<int32_t> &dst, std::vector<int64_t> &offsets);
 
 int num_threads();
 

@@ -96,7 +96,10 @@ typedef struct {
   int32_t twc_large;  /*           degree < twc_large -> wavefront */
   int32_t profile;    /* time every expand with hipEvents */
   int32_t pull_kernel; /* from-all pulls: 0 merge-path walk, 1 segmented reduction */
-  int32_t reserved[2];
+  int32_t capture;     /* gsw_bfs_multi: >0 = hipGraph-captured fused level loop,
+                          2*capture levels per replay; 0 = per-level host loop */
+  int32_t grid;        /* gsw_bfs_multi captured loop: workgroups per launch
+                          (0 = sized from the graph) */
 } gsw_run_opts;
 void gsw_run_opts_init(gsw_run_opts *o);
 
@@ -155,6 +158,33 @@ gsw_status gsw_pagerank(gsw_graph *g, float damping, int32_t iters, float eps,
 gsw_status gsw_msbfs64(gsw_graph *g, const int32_t *sources, int32_t nsrc,
                        const gsw_run_opts *opts, uint64_t *mask_out,
                        uint64_t *level_sums_out, gsw_run_stats *stats);
+/* Multi-root BFS: levels from the nearest of `roots` (duplicates ignored).
+ * On a disjoint union of graphs (gsw_gen_er_batch) with one root per
+ * instance this is one batched BFS over all instances: each instance's
+ * levels equal its own single-source BFS (application/bfs.cu:78-102).
+ * opts->capture > 0 runs a hipGraph-captured fused push loop (no host sync
+ * inside a replay) with the LB of opts->configs (WM or TWC). */
+gsw_status gsw_bfs_multi(gsw_graph *g, const int32_t *roots, int32_t nroots,
+                         const gsw_run_opts *opts, int32_t *levels_out,
+                         gsw_run_stats *stats);
+/* Autotuner for the captured batched BFS: times {WM, TWC} x TWC bin
+ * thresholds x workgroups per launch on this graph and roots (best of 3)
+ * and returns the fastest (lb uses the gsw_level_record codes; feed the
+ * result back through gsw_run_opts configs/twc_small/twc_large/grid).
+ * No counterpart in the reference, whose grid sizes and thresholds are
+ * compile-time constants (src/utils/common.cuh:14-24). */
+gsw_status gsw_autotune_bfs(gsw_graph *g, const int32_t *roots, int32_t nroots,
+                            int32_t capture, int32_t *best_lb,
+                            int32_t *best_small, int32_t *best_large,
+                            int32_t *best_grid, double *best_ms);
+/* Batch of Erdos-Renyi graphs as one disjoint union: instance i has
+ * floor(exp(U[ln nmin, ln nmax])) vertices and mean degree U[dmin, dmax]
+ * (both drawn from the batch seed), its own edge stream, and vertex ids
+ * offsets_out[i] .. offsets_out[i+1]-1 (count + 1 entries). */
+gsw_status gsw_gen_er_batch(int32_t count, int64_t nmin, int64_t nmax,
+                            double dmin, double dmax, uint64_t seed,
+                            int32_t weighted, gsw_host_graph **out,
+                            int64_t *offsets_out);
 /* Records of the last run on `g`; *n = number copied (<= cap). */
 gsw_status gsw_last_records(const gsw_graph *g, gsw_level_record *out,
                             int32_t cap, int32_t *n);

@@ -147,3 +147,40 @@ def test_degree_stats_restated(gsw):
     st = gsw.degree_stats(ring[0])
     assert st["GI"] == pytest.approx(0.0, abs=1e-12)
     assert st["Her"] == pytest.approx(1.0)
+
+
+def er_batch_params(count, nmin, nmax, dmin, dmax, seed):
+    """Instance sizes, mean degrees and seeds of an ER batch, restated."""
+    key = stream_key(seed, 7)
+    ctr = np.arange(2 * count, dtype=U64)
+    u = (draw(key, ctr) >> U64(11)).astype(np.float64) * 2.0 ** -53
+    u1, u2 = u[0::2], u[1::2]
+    ln0, ln1 = np.log(nmin), np.log(nmax)
+    n = np.floor(np.exp(ln0 + u1 * (ln1 - ln0))).astype(np.int64)
+    n = np.clip(n, 1, nmax)
+    d = dmin + u2 * (dmax - dmin)
+    seeds = [int(stream_key(seed, 1000 + i)) for i in range(count)]
+    return n, d, seeds
+
+
+def test_er_batch_is_union_of_instances(gsw):
+    count, nmin, nmax = 12, 50, 4000
+    h, off = gsw.erdos_renyi_batch(count, nmin, nmax, 2.0, 32.0, seed=5)
+    n, d, seeds = er_batch_params(count, nmin, nmax, 2.0, 32.0, 5)
+    assert np.array_equal(np.diff(off), n)
+    assert off[0] == 0 and off[-1] == h.n
+    for i in range(count):
+        # instance isolation: the union's block equals the instance built alone
+        one = gsw.erdos_renyi(int(n[i]), float(d[i]), seeds[i])
+        lo, hi = int(off[i]), int(off[i + 1])
+        rp = h.row_ptr[lo:hi + 1] - h.row_ptr[lo]
+        cols = h.col[h.row_ptr[lo]:h.row_ptr[hi]]
+        assert np.array_equal(rp, one.row_ptr)
+        assert np.array_equal(cols - lo, one.col)
+        assert cols.min(initial=lo) >= lo and cols.max(initial=lo) < hi
+    again, off2 = gsw.erdos_renyi_batch(count, nmin, nmax, 2.0, 32.0, seed=5)
+    assert np.array_equal(again.col, h.col) and np.array_equal(off, off2)
+    with pytest.raises(gsw.GswError):
+        gsw.erdos_renyi_batch(0)
+    with pytest.raises(gsw.GswError):
+        gsw.erdos_renyi_batch(3, nmin=10, nmax=5)
