@@ -92,6 +92,8 @@ _SIGS = {
     "gsw_cc": (C.c_int32, [_P, C.POINTER(RunOpts), _P, C.POINTER(RunStats)]),
     "gsw_pagerank": (C.c_int32, [_P, C.c_float, C.c_int32, C.c_float,
                                  C.POINTER(RunOpts), _P, C.POINTER(RunStats)]),
+    "gsw_bc": (C.c_int32, [_P, C.c_int32, C.POINTER(RunOpts), _P, _P, _P,
+                           C.POINTER(RunStats)]),
     "gsw_last_records": (C.c_int32, [_P, _P, C.c_int32, C.POINTER(C.c_int32)]),
     "gsw_msbfs64": (C.c_int32, [_P, _P, C.c_int32, C.POINTER(RunOpts), _P, _P,
                                 C.POINTER(RunStats)]),
@@ -307,6 +309,18 @@ class Graph:
                                  C.byref(st)))
         self.last_stats = st
         return pr
+
+    def bc(self, root, configs=None, out=True, **kw):
+        """Betweenness dependencies from `root`: (levels, nsp, dep)."""
+        o = self._opts(configs, kw)
+        st = RunStats()
+        lv = np.empty(self.n, np.int32) if out else None
+        nsp = np.empty(self.n, np.float64) if out else None
+        dep = np.empty(self.n, np.float64) if out else None
+        _check(_lib.gsw_bc(self._h, int(root), C.byref(o), _ptr(lv), _ptr(nsp),
+                           _ptr(dep), C.byref(st)))
+        self.last_stats = st
+        return lv, nsp, dep
 
     def msbfs64(self, sources, out=True, **kw):
         """64-source bitmask BFS: (masks uint64[n], per-source level sums)."""

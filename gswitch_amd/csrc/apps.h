@@ -109,6 +109,78 @@ struct PR : Functor<VC, double, float, Empty> {
   }
 };
 
+// ---- betweenness centrality from one root (application/bc.cu:16-94) -----
+// Forward pass: level + number of shortest paths (nsp); backward pass:
+// Brandes dependency accumulated from the deepest level up. nsp is an FP64
+// count (the reference's int overflows on large graphs) and dep is FP64
+// (the reference's float).
+struct bc_fwd_t {
+  int level;  // -1 until reached
+  int pad;
+  double nsp;
+};
+struct bc_bwd_t {
+  double nsp;
+  double dep;
+};
+
+struct BCF : Functor<VC, bc_fwd_t, Empty, Empty> {
+  __device__ Status filter(int v, GU g) {
+    const int lvl = wa_of(v)->level;
+    if (lvl == g.get_level()) return Active;
+    if (lvl < 0) return Inactive;
+    return Fixed;
+  }
+  __device__ bc_fwd_t emit(int v, Empty *e, GU g) {
+    bc_fwd_t x = *wa_of(v);
+    x.level += 1;
+    return x;
+  }
+  __device__ bool cond(int v, bc_fwd_t m, GU g) {
+    const int lvl = wa_of(v)->level;
+    return lvl < 0 || lvl == g.get_level();
+  }
+  __device__ bool comp(bc_fwd_t *v, bc_fwd_t m, GU g) {
+    v->level = m.level;
+    v->nsp += m.nsp;
+    return true;
+  }
+  __device__ bool compAtomic(bc_fwd_t *v, bc_fwd_t m, GU g) {
+    v->level = m.level;
+    atomicAdd(&v->nsp, m.nsp);
+    return true;
+  }
+};
+
+// ra = forward level. Becoming active finalises dep = nsp * sum over
+// children of (1 + dep_c) / nsp_c.
+struct BCB : Functor<VC, bc_bwd_t, int, Empty> {
+  __device__ Status filter(int v, GU g) {
+    const int lvl = *ra_of(v);
+    if (lvl == g.get_level()) {
+      bc_bwd_t *x = wa_of(v);
+      x->dep = x->nsp * x->dep;
+      return Active;
+    }
+    if (lvl < g.get_level()) return Inactive;
+    return Fixed;
+  }
+  __device__ bc_bwd_t emit(int v, Empty *e, GU g) {
+    bc_bwd_t x = *wa_of(v);
+    x.dep = (x.dep + 1.0) / x.nsp;
+    return x;
+  }
+  __device__ bool cond(int v, bc_bwd_t m, GU g) { return *ra_of(v) == g.get_level(); }
+  __device__ bool comp(bc_bwd_t *v, bc_bwd_t m, GU g) {
+    v->dep += m.dep;
+    return true;
+  }
+  __device__ bool compAtomic(bc_bwd_t *v, bc_bwd_t m, GU g) {
+    atomicAdd(&v->dep, m.dep);
+    return true;
+  }
+};
+
 // ---- per-run engine state ------------------------------------------------
 struct engine_t {
   inspector_t inspector;
@@ -142,9 +214,10 @@ inline void apply_opts(config_t &conf, const run_opts_t &o) {
 }
 
 // Generic vertex-centric level loop (application/bfs.cu:87-98 shape).
-// `stop_after_inspect(level)` lets PageRank end on a fold pass.
+// `stop_after_inspect(level)` lets PageRank end on a fold pass. `g` keeps
+// the level the loop ended on (BC's backward pass starts from it).
 template <typename G, typename F, typename Stop>
-int run_vc(engine_t &en, active_set_t &as, G g, F &f, const run_opts_t &o,
+int run_vc(engine_t &en, active_set_t &as, G &g, F &f, const run_opts_t &o,
            Stop stop_after_inspect) {
   int level = 0;
   for (level = 0; level < o.max_levels; ++level) {

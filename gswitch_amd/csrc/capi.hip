@@ -86,6 +86,41 @@ __global__ void k_init_pr(double *wa, float *ra, const int64_t *row_ptr,
     ra[v] = od ? (1.0f - damp) / (float)od : (1.0f - damp);
   }
 }
+__global__ void k_init_bc(bc_fwd_t *wa, int64_t n, int root) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    bc_fwd_t x;
+    x.level = (v == root) ? 0 : -1;
+    x.pad = 0;
+    x.nsp = (v == root) ? 1.0 : 0.0;
+    wa[v] = x;
+  }
+}
+// Backward-pass state from the forward result (set_functor,
+// application/bc.cu:242-246): ra = level, wa = {nsp, 0}; *max_level =
+// deepest level reached.
+__global__ void k_bc_backward_init(const bc_fwd_t *fw, bc_bwd_t *bw, int *ra,
+                                   int64_t n, int *max_level) {
+  int deepest = -1;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const bc_fwd_t x = fw[v];
+    ra[v] = x.level;
+    bw[v] = bc_bwd_t{x.nsp, 0.0};
+    deepest = max(deepest, x.level);
+  }
+  deepest = gsw::wave_max_any(deepest);
+  if (gsw::lane_id() == 0) atomicMax(max_level, deepest);
+}
+__global__ void k_bc_output(const bc_fwd_t *fw, const bc_bwd_t *bw, int64_t n,
+                            int32_t *level, double *nsp, double *dep) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    level[v] = fw[v].level;
+    nsp[v] = bw[v].nsp;
+    dep[v] = bw[v].dep;
+  }
+}
 __global__ void k_pr_output(const float *ra, const int64_t *row_ptr,
                             int64_t n, float *out) {
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
@@ -128,6 +163,12 @@ struct gsw_graph {
   std::unique_ptr<app_ctx_t<SSSP>> sssp;
   std::unique_ptr<app_ctx_t<CC>> cc;
   std::unique_ptr<app_ctx_t<PR>> pr;
+  // BC: the forward functor owns the frontier and engine; the backward
+  // functor has its own vertex arrays
+  std::unique_ptr<app_ctx_t<BCF>> bc;
+  BCB bcb{};
+  bool bcb_built = false;
+  int32_t *d_bc_out = nullptr;  // level | nsp | dep staging (16 B per vertex)
   float *d_out_f = nullptr;
   int32_t *d_flag = nullptr;
   int32_t *h_flag = nullptr;
@@ -163,6 +204,9 @@ struct gsw_graph {
     if (sssp) sssp->release();
     if (cc) cc->release();
     if (pr) pr->release();
+    if (bc) bc->release();
+    if (bcb_built) bcb.data.release();
+    if (d_bc_out) hipFree(d_bc_out);
     if (has_w) hipFree(dgw.dg_edgedata);
     if (has_coo) coo.release();
     dg.release();
@@ -787,6 +831,123 @@ gsw_status gsw_autotune_bfs(gsw_graph *G, const int32_t *roots, int32_t nroots,
         }
       }
     *best_ms = best;
+  });
+}
+
+// ---- BC --------------------------------------------------------------------
+// run_bc (application/bc.cu:97-160): forward level loop with BCF, then the
+// backward loop with BCB from the deepest level up to the root. The backward
+// pass always runs standalone levels (the inspector applies the dependency
+// finalisation to every vertex of the level, including those no child
+// reached), and each level's frontier comes from the inspector's bitmaps.
+gsw_status gsw_bc(gsw_graph *G, int32_t root, const gsw_run_opts *opts,
+                  int32_t *levels_out, double *nsp_out, double *dep_out,
+                  gsw_run_stats *stats) {
+  return guarded([&] {
+    GSW_REQUIRE(G, GSW_E_INVALID, "null graph");
+    GSW_REQUIRE(root >= 0 && root < G->hg.n, GSW_E_INVALID, "root out of range");
+    GSW_REQUIRE(!G->hg.directed, GSW_E_UNSUPPORTED,
+                "BC needs an undirected graph");
+    const run_opts_t o = to_opts(opts);
+    const int64_t n = G->hg.n;
+    if (!G->bc) {
+      G->bc = std::make_unique<app_ctx_t<BCF>>();
+      G->bc->as.build(n, Normal);
+      G->bc->f.data.build(n);
+    }
+    if (!G->bcb_built) {
+      G->bcb.data.build(n);
+      G->bcb_built = true;
+    }
+    if (!G->d_bc_out)
+      H_ERR(hipMalloc((void **)&G->d_bc_out, 20 * (size_t)(n + 1)));
+    auto &c = *G->bc;
+    BCB &h = G->bcb;
+    arch_once(c);
+    hipStream_t st = gsw::stream();
+
+    // forward
+    prepare_engine(c, o, !opts || opts->profile);
+    feature_t &fets = c.en.fets;
+    fets.centric = VC;
+    fets.pattern = ASSO;
+    fets.fromall = false;
+    fets.toall = false;
+    fets.use_root = root;
+    fets.set_graph((double)n, (double)(G->hg.m / 2), G->attr);
+    view_t<GU> gv{&G->hg, G->attr, 0.0};
+    init_conf(c.en.stats, fets, c.en.conf, gv, c.f);
+    H_ERR(hipEventRecord(G->ev0, st));
+    c.as.reset();
+    c.as.init(root);
+    hipLaunchKernelGGL(k_init_bc, dim3(gsw::grid_for(n)), dim3(gsw::BLOCK), 0, st,
+                       c.f.data.dg_wa, n, root);
+    GU g = G->dg;
+    int levels = run_vc(c.en, c.as, g, c.f, o, [](int) { return false; });
+    collect_records(G, c.en.executor, c.as);
+    std::vector<gsw_level_record> fwd_records = G->records;
+
+    // backward
+    gsw::set_one_async<int32_t>(G->d_flag, -1);
+    hipLaunchKernelGGL(k_bc_backward_init, dim3(gsw::grid_for(n)),
+                       dim3(gsw::BLOCK), 0, st, c.f.data.dg_wa, h.data.dg_wa,
+                       h.data.dg_ra, n, G->d_flag);
+    H_ERR(hipGetLastError());
+    H_ERR(hipMemcpyAsync(G->h_flag, G->d_flag, sizeof(int32_t),
+                         hipMemcpyDeviceToHost, st));
+    gsw::sync();
+    const int deepest = G->h_flag[0];
+    prepare_engine(c, o, !opts || opts->profile);
+    fets.centric = VC;
+    fets.pattern = ASSO;
+    fets.fromall = false;
+    fets.toall = false;
+    fets.use_root = -1;
+    fets.set_graph((double)n, (double)(G->hg.m / 2), G->attr);
+    c.en.conf.conf_fuse_inspect = false;
+    c.en.conf.force.has_fusion = true;
+    c.en.conf.force.fusion = Standalone;
+    c.en.conf.conf_first_round = false;
+    c.as.reset();
+    c.as.init(ALL_INACTIVE);
+    g.level = deepest;
+    for (int l = 0; l < o.max_levels && g.level >= 0; ++l) {
+      c.en.inspector.inspect(c.as, g, h, c.en.stats, fets, c.en.conf);
+      if (g.level == 0) break;  // the root is final; it has no parents
+      if (c.as.finish(g, h, c.en.conf)) break;
+      c.en.selector.select(c.en.stats, fets, c.en.conf);
+      c.en.executor.filter(c.as, g, h, c.en.stats, fets, c.en.conf);
+      g.update_level(-1);
+      c.en.executor.expand(c.as, g, h, c.en.stats, fets, c.en.conf);
+      if (c.as.finish(g, h, c.en.conf)) break;
+      ++levels;
+    }
+    H_ERR(hipEventRecord(G->ev1, st));
+    const double ms = elapsed(G);
+    collect_records(G, c.en.executor, c.as);
+    G->records.insert(G->records.begin(), fwd_records.begin(), fwd_records.end());
+    if (levels_out || nsp_out || dep_out) {
+      double *d_nsp = (double *)G->d_bc_out;
+      double *d_dep = d_nsp + n;
+      int32_t *d_lvl = (int32_t *)(d_dep + n);
+      hipLaunchKernelGGL(k_bc_output, dim3(gsw::grid_for(n)), dim3(gsw::BLOCK),
+                         0, st, c.f.data.dg_wa, h.data.dg_wa, n, d_lvl, d_nsp,
+                         d_dep);
+      H_ERR(hipGetLastError());
+      if (levels_out)
+        H_ERR(hipMemcpy(levels_out, d_lvl, sizeof(int32_t) * n,
+                        hipMemcpyDeviceToHost));
+      if (nsp_out)
+        H_ERR(hipMemcpy(nsp_out, d_nsp, sizeof(double) * n, hipMemcpyDeviceToHost));
+      if (dep_out)
+        H_ERR(hipMemcpy(dep_out, d_dep, sizeof(double) * n, hipMemcpyDeviceToHost));
+    }
+    if (stats) {
+      stats->device_ms = ms;
+      stats->levels = levels;
+      stats->n_records = (int32_t)G->records.size();
+      stats->reached = 0;
+    }
   });
 }
 

@@ -126,7 +126,10 @@ struct executor_t {
         conf.conf_lb = CM;
     }
     if constexpr (std::is_same<G, device_graph_t<CSR, typename G::edge_t>>::value) {
+      // fused levels skip the inspector, so the filter's per-vertex update
+      // is applied to the queued vertices here (kernel_fusion.cuh:84-96)
       if (conf.conf_compensation && conf.conf_centric == VC &&
+          conf.conf_fuse_inspect && !conf.conf_first_round &&
           as.fmt == Queue) {
         hipLaunchKernelGGL((gsw::k_compensation<G, F>),
                            dim3(gsw::grid_for(as.get_size_host())),

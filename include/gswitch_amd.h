@@ -150,6 +150,14 @@ gsw_status gsw_cc(gsw_graph *g, const gsw_run_opts *opts, int32_t *labels_out,
 gsw_status gsw_pagerank(gsw_graph *g, float damping, int32_t iters, float eps,
                         const gsw_run_opts *opts, float *pr_out,
                         gsw_run_stats *stats);
+/* Betweenness dependencies from one root (Brandes): levels_out = BFS level
+ * (-1 unreached), nsp_out = number of shortest root->v paths, dep_out =
+ * dependency of the root on v. Any output may be null. Replaces run_bc
+ * (application/bc.cu:97-160) with functors bc.cu:16-94; nsp and dep are
+ * FP64 where the reference keeps int and float. Undirected graphs only. */
+gsw_status gsw_bc(gsw_graph *g, int32_t root, const gsw_run_opts *opts,
+                  int32_t *levels_out, double *nsp_out, double *dep_out,
+                  gsw_run_stats *stats);
 /* 64-source bitmask BFS: bit s of mask_out[v] is set iff v is reachable from
  * sources[s] (1 <= nsrc <= 64). level_sums_out[s] (64 entries, optional) =
  * sum over reached v of the BFS level of v from sources[s]. No counterpart
