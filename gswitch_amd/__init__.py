@@ -38,7 +38,7 @@ class RunOpts(C.Structure):
                 ("verbose", C.c_int32), ("twc_small", C.c_int32),
                 ("twc_large", C.c_int32), ("profile", C.c_int32),
                 ("pull_kernel", C.c_int32), ("capture", C.c_int32),
-                ("grid", C.c_int32)]
+                ("grid", C.c_int32), ("selector", C.c_int32)]
 
 
 class RunStats(C.Structure):
@@ -103,6 +103,16 @@ _SIGS = {
                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_double)]),
+    "gsw_selector_load": (C.c_int32, [_P, C.c_int32]),
+    "gsw_mlp_features": (C.c_int32, [_P, _P]),
+    "gsw_mlp_forward": (C.c_int32, [_P, _P, C.c_int64, _P]),
+    "gsw_mlp_select": (C.c_int32, [_P, _P, C.c_int64, C.c_uint32, _P]),
+    "gsw_mlp_grad": (C.c_int32, [_P, _P, _P, _P, C.c_int64, C.POINTER(C.c_float),
+                                 _P]),
+    "gsw_mlp_train": (C.c_int32, [_P, _P, _P, C.c_int64, C.c_int32, C.c_int32,
+                                  C.c_float, C.c_uint64, _P, _P]),
+    "gsw_explore_bfs": (C.c_int32, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                    _P, _P, _P, _P, C.POINTER(C.c_int32)]),
     "gsw_tree_eval": (C.c_int32, [C.c_char_p, _P, C.c_int32,
                                   C.POINTER(C.c_int32)]),
     "gsw_scan_exclusive_i64": (C.c_int32, [_P, C.c_int64, _P]),
@@ -225,9 +235,82 @@ def tree_eval(name, fv):
     return cls.value
 
 
+# --------------------------------------------------------------- learned selector
+MLP_NPARAM, MLP_NIN, MLP_NOUT = 6942, 12, 30
+MLP_GROUPS = ["Push-Queue-%s-Standalone", "Push-SortedQueue-%s-Standalone",
+              "Push-Bitmap-%s-Standalone", "Push-Queue-%s-Fused",
+              "Pull-Queue-%s-Standalone", "Pull-Bitmap-%s-Standalone"]
+MLP_LBS = ["TM", "WM", "CM", "STRICT", "TWC"]
+# class id -> the forced-variant string of that class
+MLP_CLASSES = [g % lb for g in MLP_GROUPS for lb in MLP_LBS]
+
+
+def _f32(a, n=None):
+    a = np.ascontiguousarray(a, np.float32)
+    if n is not None and a.size != n:
+        raise ValueError("expected %d values" % n)
+    return a
+
+
+def selector_load(w):
+    """Install the learned selector's weights (gsw_run_opts.selector = 1)."""
+    w = _f32(w, MLP_NPARAM)
+    _check(_lib.gsw_selector_load(_ptr(w), MLP_NPARAM))
+
+
+def mlp_features(counts):
+    """12 MLP inputs from (n, m, avg, std, gini, level, active, inactive,
+    push workload, pull workload, max active degree)."""
+    c = np.ascontiguousarray(counts, np.float64)
+    if c.size != 11:
+        raise ValueError("expected 11 counts")
+    x = np.empty(MLP_NIN, np.float32)
+    _check(_lib.gsw_mlp_features(_ptr(c), _ptr(x)))
+    return x
+
+
+def mlp_forward(w, x):
+    w, x = _f32(w, MLP_NPARAM), _f32(x).reshape(-1, MLP_NIN)
+    y = np.empty((len(x), MLP_NOUT), np.float32)
+    _check(_lib.gsw_mlp_forward(_ptr(w), _ptr(x), len(x), _ptr(y)))
+    return y
+
+
+def mlp_select(w, x, mask=(1 << 30) - 1):
+    w, x = _f32(w, MLP_NPARAM), _f32(x).reshape(-1, MLP_NIN)
+    cls = np.empty(len(x), np.int32)
+    _check(_lib.gsw_mlp_select(_ptr(w), _ptr(x), len(x), mask, _ptr(cls)))
+    return cls
+
+
+def mlp_grad(w, x, t, mask):
+    """(loss, grad[6942]) of sum(mask*(y-t)^2)/sum(mask), on the device."""
+    w, x = _f32(w, MLP_NPARAM), _f32(x).reshape(-1, MLP_NIN)
+    t = _f32(t).reshape(-1, MLP_NOUT)
+    mask = _f32(mask).reshape(-1, MLP_NOUT)
+    g = np.empty(MLP_NPARAM, np.float32)
+    loss = C.c_float()
+    _check(_lib.gsw_mlp_grad(_ptr(w), _ptr(x), _ptr(t), _ptr(mask), len(x),
+                             C.byref(loss), _ptr(g)))
+    return loss.value, g
+
+
+def mlp_train(w0, x, t, mask, epochs=200, batch=256, lr=1e-3, seed=7):
+    """Adam training on the device; returns (weights, loss per epoch)."""
+    w = _f32(w0, MLP_NPARAM).copy()
+    x = _f32(x).reshape(-1, MLP_NIN)
+    t = _f32(t).reshape(-1, MLP_NOUT)
+    mask = _f32(mask).reshape(-1, MLP_NOUT)
+    curve = np.zeros(max(epochs, 1), np.float32)
+    _check(_lib.gsw_mlp_train(_ptr(x), _ptr(t), _ptr(mask), len(x), epochs, batch,
+                              lr, seed, _ptr(w), _ptr(curve)))
+    return w, curve[:epochs]
+
+
 # --------------------------------------------------------------- device graphs
 def run_opts(configs=None, max_levels=0, verbose=False, twc_small=0,
-             twc_large=0, profile=True, pull_kernel=0, capture=0, grid=0):
+             twc_large=0, profile=True, pull_kernel=0, capture=0, grid=0,
+             selector=0):
     o = RunOpts()
     _lib.gsw_run_opts_init(C.byref(o))
     o.configs = configs.encode() if configs else None
@@ -242,6 +325,7 @@ def run_opts(configs=None, max_levels=0, verbose=False, twc_small=0,
     o.pull_kernel = int(pull_kernel)
     o.capture = int(capture)
     o.grid = int(grid)
+    o.selector = int(selector)
     return o
 
 
@@ -345,6 +429,20 @@ class Graph:
                                   C.byref(st)))
         self.last_stats = st
         return lv
+
+    def explore_bfs(self, root, max_levels=64, stride=1, reps=2):
+        """Exhaustive per-level timing of the 30 variant classes for BFS from
+        `root`: dict(features[L,12], ms[L,30], rule[L], level[L])."""
+        feat = np.zeros((max_levels, MLP_NIN), np.float32)
+        ms = np.zeros((max_levels, MLP_NOUT), np.float32)
+        rule = np.zeros(max_levels, np.int32)
+        lvl = np.zeros(max_levels, np.int32)
+        k = C.c_int32()
+        _check(_lib.gsw_explore_bfs(self._h, int(root), max_levels, stride, reps,
+                                    _ptr(feat), _ptr(ms), _ptr(rule), _ptr(lvl),
+                                    C.byref(k)))
+        k = k.value
+        return dict(features=feat[:k], ms=ms[:k], rule=rule[:k], level=lvl[:k])
 
     def autotune_bfs(self, roots, capture=8):
         """Fastest captured-loop shape for these roots:

@@ -189,10 +189,15 @@ struct engine_t {
   feature_t fets;
   config_t conf;
   stat_t stats;
+  // learned-selector device block: [0] running inactive count, [1] running
+  // pull workload, [2..3] next-frontier degree sum / max, [4] ticket
+  long long *d_mlp_block = nullptr;
   engine_t() { executor.bind(inspector); }
   void release() {
     stats.release();
     executor.release();
+    if (d_mlp_block) hipFree(d_mlp_block);
+    d_mlp_block = nullptr;
   }
 };
 
@@ -203,7 +208,36 @@ struct run_opts_t {
   bool sorted = false;
   int twc_small = 32, twc_large = 2048;
   bool mp_pull = true;
+  int selector = 0;  // 0 rule trees, 1 learned MLP
 };
+
+// Switch the engine to the learned selector (weights on the device and a
+// host copy) for a graph with these degree statistics. Called after
+// init_conf: the first level's choice (fused or not) is the network's.
+inline void enable_learned(engine_t &en, const float *w_dev, const float *w_host,
+                           const graph_attr_t &attr, double n, double m,
+                           bool fromall, bool toall) {
+  if (!en.d_mlp_block) {
+    H_ERR(hipMalloc((void **)&en.d_mlp_block, sizeof(long long) * 8));
+    H_ERR(hipMemset(en.d_mlp_block, 0, sizeof(long long) * 8));
+  }
+  gsw_mlp::sel_dev_t &d = en.inspector.mlp;
+  d.w = w_dev;
+  d.n = (float)n;
+  d.m = (float)m;
+  d.avg_deg = (float)attr.avg_deg;
+  d.std_deg = (float)attr.std_deg;
+  d.gini = (float)attr.GI;
+  d.mask = gsw_mlp::valid_mask(fromall, toall);
+  d.running = en.d_mlp_block;
+  d.ticket = (unsigned int *)(en.d_mlp_block + 4);
+  d.out = nullptr;
+  en.inspector.mlp_host_w = w_host;
+  en.conf.policy_mlp = true;
+  en.conf.mlp_class = -1;
+  en.conf.conf_fuse_inspect = false;
+  en.conf.conf_fusion = false;
+}
 
 inline void apply_opts(config_t &conf, const run_opts_t &o) {
   conf.force = o.force;

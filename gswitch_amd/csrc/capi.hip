@@ -265,6 +265,16 @@ run_opts_t to_opts(const gsw_run_opts *o) {
   if (o->twc_small > 0) r.twc_small = o->twc_small;
   if (o->twc_large > 0) r.twc_large = o->twc_large;
   r.mp_pull = o->pull_kernel == 0;
+  r.selector = o->selector;
+  return r;
+}
+
+// options of traversals whose variant choice stays with the rule trees (the
+// learned selector is trained on BFS levels)
+run_opts_t to_opts_rule(const gsw_run_opts *o) {
+  run_opts_t r = to_opts(o);
+  GSW_REQUIRE(r.selector == 0, GSW_E_UNSUPPORTED,
+              "the learned selector is available for BFS only");
   return r;
 }
 
@@ -324,6 +334,21 @@ double elapsed(gsw_graph *G) {
   H_ERR(hipEventSynchronize(G->ev1));
   H_ERR(hipEventElapsedTime(&ms, G->ev0, G->ev1));
   return ms;
+}
+
+// ---- learned selector state (process-wide, one device) ---------------------
+float *g_mlp_dev = nullptr;
+std::vector<float> g_mlp_host;
+
+template <typename F>
+void maybe_learned(app_ctx_t<F> &c, const run_opts_t &o, const gsw_graph *G,
+                   bool fromall, bool toall) {
+  if (o.selector == 0) return;
+  GSW_REQUIRE(o.selector == 1, GSW_E_INVALID, "unknown selector");
+  GSW_REQUIRE(g_mlp_dev, GSW_E_INVALID,
+              "learned selector requested but no weights are loaded");
+  enable_learned(c.en, g_mlp_dev, g_mlp_host.data(), G->attr, (double)G->hg.n,
+                 (double)G->hg.m, fromall, toall);
 }
 
 app_ctx_t<BFS> &bfs_ctx(gsw_graph *G) {
@@ -393,6 +418,7 @@ int bfs_engine_run(gsw_graph *G, const gsw_run_opts *opts, int32_t root,
   c.en.conf.conf_pruning = true;
   view_t<GU> gv{&G->hg, G->attr, 0.0};
   init_conf(c.en.stats, fets, c.en.conf, gv, c.f);
+  maybe_learned(c, o, G, false, false);
 
   H_ERR(hipEventRecord(G->ev0, gsw::stream()));
   c.as.reset();
@@ -499,6 +525,63 @@ int bfs_captured_run(gsw_graph *G, int64_t k, LB lb, int small, int large,
 
 int default_capture_grid(const gsw_graph *G) {
   return gsw::grid_for(G->hg.n, gsw::BLOCK, gsw::MAX_GRID);
+}
+
+ins_t ins_of_class(int cls) {
+  const gsw_mlp::variant_t v = gsw_mlp::variant_of_class(cls);
+  ins_t f;
+  f.has_direction = f.has_asfmt = f.has_lb = f.has_fusion = f.has_sorted = true;
+  f.d = v.dir;
+  f.asf = v.fmt;
+  f.lb = v.lb;
+  f.fusion = v.fused ? Fused : Standalone;
+  f.sorted = v.sorted;
+  return f;
+}
+
+// Fresh per-level engine state for a BFS level probe (forced class `cls`,
+// or the rule selector when cls < 0): the level is inspected as a
+// standalone level whatever came before it.
+void bfs_level_setup(gsw_graph *G, app_ctx_t<BFS> &c, int root, int cls) {
+  run_opts_t o;
+  if (cls >= 0) o.force = ins_of_class(cls);
+  prepare_engine(c, o, false);
+  feature_t &fets = c.en.fets;
+  fets.centric = VC;
+  fets.pattern = Idem;
+  fets.fromall = false;
+  fets.toall = false;
+  fets.use_root = root;
+  fets.set_graph((double)G->hg.n, (double)(G->hg.m / 2), G->attr);
+  c.en.conf.conf_pruning = true;
+  c.en.conf.conf_first_round = false;
+  c.en.conf.conf_fuse_inspect = false;
+  c.as.reset();
+}
+
+// Run BFS level `lvl` from the current levels array with class `cls`;
+// returns the wall time in ms of what that class costs per level.
+double bfs_probe_level(gsw_graph *G, app_ctx_t<BFS> &c, int root, int cls,
+                       int lvl) {
+  bfs_level_setup(G, c, root, cls);
+  engine_t &en = c.en;
+  GU g = G->dg;
+  g.level = lvl;
+  const bool fused = gsw_mlp::variant_of_class(cls).fused;
+  gsw::sync();
+  double t0 = gsw_now_ms();
+  en.inspector.inspect(c.as, g, c.f, en.stats, en.fets, en.conf);
+  if (c.as.finish(g, c.f, en.conf)) return 0.0;
+  en.selector.select(en.stats, en.fets, en.conf);
+  en.executor.filter(c.as, g, c.f, en.stats, en.fets, en.conf);
+  if (fused) {
+    gsw::sync();
+    t0 = gsw_now_ms();
+  }
+  g.update_level();
+  en.executor.expand(c.as, g, c.f, en.stats, en.fets, en.conf);
+  gsw::sync();
+  return gsw_now_ms() - t0;
 }
 
 }  // namespace
@@ -848,7 +931,7 @@ gsw_status gsw_bc(gsw_graph *G, int32_t root, const gsw_run_opts *opts,
     GSW_REQUIRE(root >= 0 && root < G->hg.n, GSW_E_INVALID, "root out of range");
     GSW_REQUIRE(!G->hg.directed, GSW_E_UNSUPPORTED,
                 "BC needs an undirected graph");
-    const run_opts_t o = to_opts(opts);
+    const run_opts_t o = to_opts_rule(opts);
     const int64_t n = G->hg.n;
     if (!G->bc) {
       G->bc = std::make_unique<app_ctx_t<BCF>>();
@@ -958,7 +1041,7 @@ gsw_status gsw_sssp(gsw_graph *G, int32_t root, const gsw_run_opts *opts,
     GSW_REQUIRE(G, GSW_E_INVALID, "null graph");
     GSW_REQUIRE(G->has_w, GSW_E_INVALID, "SSSP needs a weighted graph");
     GSW_REQUIRE(root >= 0 && root < G->hg.n, GSW_E_INVALID, "root out of range");
-    const run_opts_t o = to_opts(opts);
+    const run_opts_t o = to_opts_rule(opts);
     if (!G->sssp) {
       G->sssp = std::make_unique<app_ctx_t<SSSP>>();
       G->sssp->as.build(G->hg.n, Normal);
@@ -1009,7 +1092,7 @@ gsw_status gsw_cc(gsw_graph *G, const gsw_run_opts *opts, int32_t *labels_out,
                   gsw_run_stats *stats) {
   return guarded([&] {
     GSW_REQUIRE(G, GSW_E_INVALID, "null graph");
-    const run_opts_t o = to_opts(opts);
+    const run_opts_t o = to_opts_rule(opts);
     if (!G->has_coo) {
       G->coo.build(G->hg);
       G->has_coo = true;
@@ -1072,7 +1155,7 @@ gsw_status gsw_pagerank(gsw_graph *G, float damping, int32_t iters, float eps,
     GSW_REQUIRE(G, GSW_E_INVALID, "null graph");
     GSW_REQUIRE(iters >= 0 && damping >= 0 && damping <= 1 && eps >= 0,
                 GSW_E_INVALID, "invalid PageRank parameters");
-    const run_opts_t o = to_opts(opts);
+    const run_opts_t o = to_opts_rule(opts);
     if (!G->pr) {
       G->pr = std::make_unique<app_ctx_t<PR>>();
       G->pr->as.build(G->hg.n, Normal);
@@ -1209,6 +1292,299 @@ gsw_status gsw_msbfs64(gsw_graph *G, const int32_t *sources, int32_t nsrc,
       stats->n_records = (int32_t)G->records.size();
       stats->reached = 0;
     }
+  });
+}
+
+// ---- learned selector -------------------------------------------------------
+gsw_status gsw_selector_load(const float *w, int32_t nparam) {
+  return guarded([&] {
+    GSW_REQUIRE(w && nparam == gsw_mlp::NPARAM, GSW_E_INVALID,
+                "expected 6942 weights");
+    require_gpu();
+    for (int32_t i = 0; i < nparam; ++i)
+      GSW_REQUIRE(std::isfinite(w[i]), GSW_E_INVALID, "non-finite weight");
+    if (!g_mlp_dev)
+      H_ERR(hipMalloc((void **)&g_mlp_dev, sizeof(float) * gsw_mlp::NPARAM));
+    H_ERR(hipMemcpy(g_mlp_dev, w, sizeof(float) * gsw_mlp::NPARAM,
+                    hipMemcpyHostToDevice));
+    g_mlp_host.assign(w, w + nparam);
+  });
+}
+
+gsw_status gsw_mlp_features(const double *counts, float *x) {
+  return guarded([&] {
+    GSW_REQUIRE(counts && x, GSW_E_INVALID, "null argument");
+    gsw_mlp::counts_t c;
+    c.n = (float)counts[0];
+    c.m = (float)counts[1];
+    c.avg_deg = (float)counts[2];
+    c.std_deg = (float)counts[3];
+    c.gini = (float)counts[4];
+    c.level = (float)counts[5];
+    c.active = (float)counts[6];
+    c.inactive = (float)counts[7];
+    c.push_wl = (float)counts[8];
+    c.pull_wl = (float)counts[9];
+    c.max_deg_active = (float)counts[10];
+    gsw_mlp::features(c, x);
+  });
+}
+
+}  // extern "C"
+
+namespace {
+// device buffers of one offline MLP call, freed on every exit path
+struct dev_bufs_t {
+  std::vector<void *> p;
+  template <typename T> T *get(size_t n) {
+    void *q = nullptr;
+    H_ERR(hipMalloc(&q, sizeof(T) * (n ? n : 1)));
+    p.push_back(q);
+    return (T *)q;
+  }
+  template <typename T> T *put(const T *h, size_t n) {
+    T *d = get<T>(n);
+    if (n) H_ERR(hipMemcpy(d, h, sizeof(T) * n, hipMemcpyHostToDevice));
+    return d;
+  }
+  ~dev_bufs_t() {
+    for (void *q : p) hipFree(q);
+  }
+};
+
+int batch_grid(int64_t B) {
+  return (int)MAX((int64_t)1, MIN((int64_t)1024, (B + 3) / 4));
+}
+
+// One gradient evaluation over `B` samples (idx == nullptr: off .. off+B)
+// into d_grad; returns nothing (loss written to d_loss).
+void mlp_grad_device(const float *w, const float *x, const float *t,
+                     const float *mask, const int32_t *idx, int64_t off,
+                     int64_t B, float scale, float *part, float *part_loss,
+                     int nwave_cap, float *d_grad, float *d_loss) {
+  constexpr int PER = 8;
+  const int64_t nw = MAX((int64_t)1, (B + PER - 1) / PER);
+  const int grid = (int)((nw + 3) / 4);
+  GSW_REQUIRE(grid * 4 <= nwave_cap, GSW_E_INTERNAL, "gradient scratch too small");
+  hipStream_t st = gsw::stream();
+  hipLaunchKernelGGL(gsw::k_mlp_grad_partial, dim3(grid), dim3(gsw::BLOCK),
+                     gsw::MLP_GRAD_LDS, st, w, x, t, mask, idx, off, B, PER,
+                     scale, part, part_loss);
+  hipLaunchKernelGGL(gsw::k_mlp_reduce,
+                     dim3((gsw_mlp::NPARAM + gsw::BLOCK - 1) / gsw::BLOCK),
+                     dim3(gsw::BLOCK), 0, st, part, part_loss, grid * 4, d_grad,
+                     d_loss);
+  H_ERR(hipGetLastError());
+}
+}  // namespace
+
+extern "C" {
+
+gsw_status gsw_mlp_forward(const float *w, const float *x, int64_t B, float *y) {
+  return guarded([&] {
+    GSW_REQUIRE(w && x && y && B >= 0, GSW_E_INVALID, "invalid arguments");
+    require_gpu();
+    if (B == 0) return;
+    dev_bufs_t d;
+    const float *dw = d.put(w, gsw_mlp::NPARAM);
+    const float *dx = d.put(x, (size_t)B * gsw_mlp::NIN);
+    float *dy = d.get<float>((size_t)B * gsw_mlp::NOUT);
+    hipLaunchKernelGGL(gsw::k_mlp_forward_batch, dim3(batch_grid(B)),
+                       dim3(gsw::BLOCK), 0, gsw::stream(), dw, dx, B, dy);
+    H_ERR(hipGetLastError());
+    H_ERR(hipMemcpy(y, dy, sizeof(float) * B * gsw_mlp::NOUT,
+                    hipMemcpyDeviceToHost));
+  });
+}
+
+gsw_status gsw_mlp_select(const float *w, const float *x, int64_t B,
+                          uint32_t mask, int32_t *cls) {
+  return guarded([&] {
+    GSW_REQUIRE(w && x && cls && B >= 0, GSW_E_INVALID, "invalid arguments");
+    require_gpu();
+    if (B == 0) return;
+    dev_bufs_t d;
+    const float *dw = d.put(w, gsw_mlp::NPARAM);
+    const float *dx = d.put(x, (size_t)B * gsw_mlp::NIN);
+    int32_t *dc = d.get<int32_t>((size_t)B);
+    hipLaunchKernelGGL(gsw::k_mlp_select_batch, dim3(batch_grid(B)),
+                       dim3(gsw::BLOCK), 0, gsw::stream(), dw, dx, B, mask, dc);
+    H_ERR(hipGetLastError());
+    H_ERR(hipMemcpy(cls, dc, sizeof(int32_t) * B, hipMemcpyDeviceToHost));
+  });
+}
+
+gsw_status gsw_mlp_grad(const float *w, const float *x, const float *t,
+                        const float *mask, int64_t B, float *loss,
+                        float *grad) {
+  return guarded([&] {
+    GSW_REQUIRE(w && x && t && mask && grad && B >= 1, GSW_E_INVALID,
+                "invalid arguments");
+    require_gpu();
+    double msum = 0;
+    for (int64_t i = 0; i < B * gsw_mlp::NOUT; ++i) msum += mask[i];
+    const float scale = msum > 0 ? (float)(1.0 / msum) : 0.f;
+    dev_bufs_t d;
+    const float *dw = d.put(w, gsw_mlp::NPARAM);
+    const float *dx = d.put(x, (size_t)B * gsw_mlp::NIN);
+    const float *dt = d.put(t, (size_t)B * gsw_mlp::NOUT);
+    const float *dm = d.put(mask, (size_t)B * gsw_mlp::NOUT);
+    const int cap = (int)(((B + 7) / 8 + 3) / 4 * 4);
+    float *part = d.get<float>((size_t)cap * gsw_mlp::NPARAM);
+    float *pl = d.get<float>((size_t)cap);
+    float *dg = d.get<float>(gsw_mlp::NPARAM);
+    float *dl = d.get<float>(1);
+    mlp_grad_device(dw, dx, dt, dm, nullptr, 0, B, scale, part, pl, cap, dg, dl);
+    H_ERR(hipMemcpy(grad, dg, sizeof(float) * gsw_mlp::NPARAM,
+                    hipMemcpyDeviceToHost));
+    float l = 0;
+    H_ERR(hipMemcpy(&l, dl, sizeof(float), hipMemcpyDeviceToHost));
+    if (loss) *loss = l * scale;
+  });
+}
+
+gsw_status gsw_mlp_train(const float *x, const float *t, const float *mask,
+                         int64_t B, int32_t epochs, int32_t batch, float lr,
+                         uint64_t seed, float *w_inout, float *loss_curve) {
+  return guarded([&] {
+    GSW_REQUIRE(x && t && mask && w_inout && B >= 1 && epochs >= 0 &&
+                    batch >= 1 && lr > 0,
+                GSW_E_INVALID, "invalid arguments");
+    require_gpu();
+    using namespace gsw_mlp;
+    dev_bufs_t d;
+    float *dw = d.put(w_inout, NPARAM);
+    const float *dx = d.put(x, (size_t)B * NIN);
+    const float *dt = d.put(t, (size_t)B * NOUT);
+    const float *dm = d.put(mask, (size_t)B * NOUT);
+    float *m1 = d.get<float>(NPARAM);
+    float *m2 = d.get<float>(NPARAM);
+    gsw::fill_async<float>(m1, 0.f, NPARAM);
+    gsw::fill_async<float>(m2, 0.f, NPARAM);
+    const int64_t bsz_max = MIN((int64_t)batch, B);
+    const int cap = (int)(((bsz_max + 7) / 8 + 3) / 4 * 4);
+    float *part = d.get<float>((size_t)cap * NPARAM);
+    float *pl = d.get<float>((size_t)cap);
+    float *dg = d.get<float>(NPARAM);
+    const int64_t steps = (B + batch - 1) / batch;
+    float *dl = d.get<float>((size_t)steps);
+    int32_t *didx = d.get<int32_t>((size_t)B);
+    std::vector<int32_t> perm((size_t)B);
+    std::vector<float> hl((size_t)steps);
+    // per-sample mask sums for the minibatch scale
+    std::vector<double> ms((size_t)B, 0.0);
+    double mtot = 0;
+    for (int64_t i = 0; i < B; ++i) {
+      for (int c = 0; c < NOUT; ++c) ms[i] += mask[i * NOUT + c];
+      mtot += ms[i];
+    }
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    int64_t step = 0;
+    const uint64_t key = gsw_host::stream_key(seed, 11);
+    for (int e = 0; e < epochs; ++e) {
+      for (int64_t i = 0; i < B; ++i) perm[i] = (int32_t)i;
+      for (int64_t i = B - 1; i > 0; --i) {  // Fisher-Yates, counter-based draws
+        const uint64_t r = gsw_host::draw(key, (uint64_t)e * (uint64_t)B + (uint64_t)i);
+        std::swap(perm[i], perm[gsw_host::below(r, (uint64_t)i + 1)]);
+      }
+      H_ERR(hipMemcpy(didx, perm.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice));
+      for (int64_t s = 0; s < steps; ++s) {
+        const int64_t off = s * batch, bsz = MIN((int64_t)batch, B - off);
+        double msum = 0;
+        for (int64_t i = 0; i < bsz; ++i) msum += ms[perm[off + i]];
+        const float scale = msum > 0 ? (float)(1.0 / msum) : 0.f;
+        mlp_grad_device(dw, dx, dt, dm, didx, off, bsz, scale, part, pl, cap, dg,
+                        dl + s);
+        ++step;
+        const double bc1 = 1.0 - std::pow((double)b1, (double)step);
+        const double bc2 = 1.0 - std::pow((double)b2, (double)step);
+        const float lr_t = (float)(lr * std::sqrt(bc2) / bc1);
+        hipLaunchKernelGGL(gsw::k_adam, dim3((NPARAM + gsw::BLOCK - 1) / gsw::BLOCK),
+                           dim3(gsw::BLOCK), 0, gsw::stream(), dw, dg, m1, m2,
+                           NPARAM, lr_t, b1, b2, eps);
+        H_ERR(hipGetLastError());
+      }
+      if (loss_curve) {
+        H_ERR(hipMemcpy(hl.data(), dl, sizeof(float) * steps, hipMemcpyDeviceToHost));
+        double s = 0;
+        for (float v : hl) s += v;
+        loss_curve[e] = mtot > 0 ? (float)(s / mtot) : 0.f;
+      }
+    }
+    H_ERR(hipMemcpy(w_inout, dw, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
+  });
+}
+
+// ---- exhaustive per-level timing (selector training labels) ----------------
+gsw_status gsw_explore_bfs(gsw_graph *G, int32_t root, int32_t max_levels,
+                           int32_t stride, int32_t reps, float *feat_out,
+                           float *ms_out, int32_t *rule_out, int32_t *level_out,
+                           int32_t *n_out) {
+  return guarded([&] {
+    using namespace gsw_mlp;
+    GSW_REQUIRE(G && n_out && max_levels >= 1 && stride >= 1 && reps >= 1,
+                GSW_E_INVALID, "invalid arguments");
+    GSW_REQUIRE(root >= 0 && root < G->hg.n, GSW_E_INVALID, "root out of range");
+    auto &c = bfs_ctx(G);
+    arch_once(c);
+    const int64_t n = G->hg.n;
+    dev_bufs_t d;
+    int32_t *snap = d.get<int32_t>((size_t)n);
+    hipStream_t st = gsw::stream();
+    init_bfs_levels(G, c.f.data.dg_wa, root, nullptr, 1);
+    int written = 0;
+    for (int lvl = 0; lvl < (1 << 20) && written < max_levels; ++lvl) {
+      H_ERR(hipMemcpyAsync(snap, c.f.data.dg_wa, sizeof(int32_t) * n,
+                           hipMemcpyDeviceToDevice, st));
+      // the level as the rule selector sees it
+      bfs_level_setup(G, c, root, -1);
+      GU g = G->dg;
+      g.level = lvl;
+      c.en.inspector.inspect(c.as, g, c.f, c.en.stats, c.en.fets, c.en.conf);
+      const long long *info = c.en.stats.h_runtime_info;
+      // the engine stops here too: no frontier, or nothing left to reach
+      if (info[0] == 0 || info[1] == 0) break;
+      const bool probe = (lvl % stride) == 0;
+      if (!probe) {
+        c.en.selector.select(c.en.stats, c.en.fets, c.en.conf);
+        c.en.executor.filter(c.as, g, c.f, c.en.stats, c.en.fets, c.en.conf);
+        g.update_level();
+        c.en.executor.expand(c.as, g, c.f, c.en.stats, c.en.fets, c.en.conf);
+        gsw::sync();
+        continue;
+      }
+      counts_t k;
+      k.n = (float)n;
+      k.m = (float)G->hg.m;
+      k.avg_deg = (float)G->attr.avg_deg;
+      k.std_deg = (float)G->attr.std_deg;
+      k.gini = (float)G->attr.GI;
+      k.level = (float)lvl;
+      k.active = (float)info[0];
+      k.inactive = (float)info[1];
+      k.push_wl = (float)info[2];
+      k.pull_wl = (float)info[3];
+      k.max_deg_active = (float)info[4];
+      if (feat_out) features(k, feat_out + (size_t)written * NIN);
+      c.en.selector.select(c.en.stats, c.en.fets, c.en.conf);
+      const config_t &rc = c.en.conf;
+      if (rule_out)
+        rule_out[written] = class_of(rc.conf_dir, rc.conf_asfmt, rc.conf_sorted,
+                                     rc.conf_fuse_inspect, rc.conf_lb);
+      if (level_out) level_out[written] = lvl;
+      for (int cls = 0; cls < NOUT; ++cls) {
+        double best = 1e300;
+        for (int r = 0; r < reps; ++r) {
+          H_ERR(hipMemcpyAsync(c.f.data.dg_wa, snap, sizeof(int32_t) * n,
+                               hipMemcpyDeviceToDevice, st));
+          best = MIN(best, bfs_probe_level(G, c, root, cls, lvl));
+        }
+        if (ms_out) ms_out[(size_t)written * NOUT + cls] = (float)best;
+      }
+      // every class leaves the same levels array: the state of level lvl+1
+      ++written;
+    }
+    *n_out = written;
   });
 }
 

@@ -62,6 +62,10 @@ struct config_t {
   // from-all bitmap pulls use the merge-path kernel (k_pull_mp*) instead of
   // the segmented-reduction STRICT/TWOD kernels
   bool mp_pull = true;
+  // learned selector: every level runs the variant class the MLP picks
+  // (mlp_class, written by the inspector / the fused-level selection)
+  bool policy_mlp = false;
+  int mlp_class = -1;
   ins_t force;
 
   void reset() {

@@ -122,7 +122,8 @@ struct executor_t {
         conf.conf_idle += 1;
       else
         conf.conf_idle = 0;
-      if (conf.ctanum < 5 && conf.conf_lb == STRICT && !conf.force.has_lb)
+      if (conf.ctanum < 5 && conf.conf_lb == STRICT && !conf.force.has_lb &&
+          !conf.policy_mlp)
         conf.conf_lb = CM;
     }
     if constexpr (std::is_same<G, device_graph_t<CSR, typename G::edge_t>>::value) {
@@ -213,10 +214,33 @@ struct executor_t {
       // the next queue's length: grid size for the next level, convergence
       const int out_slot = (as.queue.mode == Normal) ? (as.queue.cursor ^ 1)
                                                      : (as.queue.cursor + 1);
+      constexpr bool csr = std::is_same<G, device_graph_t<CSR, typename G::edge_t>>::value;
+      const bool learned = csr && conf.policy_mlp && insp && insp->mlp.w;
+      if constexpr (csr) if (learned) {
+        // next level's class from the new frontier, read with its length
+        gsw_mlp::sel_dev_t d = insp->mlp;
+        d.level = (float)g.level;
+        const int32_t *out = (as.queue.mode == Normal)
+                                 ? as.queue.buf[as.queue.cursor ^ 1]
+                                 : as.queue.buf[0] + as.queue.offset_out;
+        hipLaunchKernelGGL(gsw::k_mlp_fused_select, dim3(gsw::MLP_SELECT_GRID),
+                           dim3(gsw::BLOCK), 0, gsw::stream(), d, out,
+                           as.queue.sizes + out_slot, g.dg_row_ptr,
+                           insp->mlp.running + 2, as.dg_ctrl + CTRL_MLP);
+        H_ERR(hipGetLastError());
+      }
       int nq = as.read_qsize_device(out_slot);
       if (insp) insp->known_active = nq;
       as.check_overflow_cached();
       if (nq == 0) as.halt_host();
+      if (learned) {
+        conf.mlp_class = as.h_ctrl[CTRL_MLP];
+        if (nq > 0 && conf.mlp_class >= 0 &&
+            !gsw_mlp::variant_of_class(conf.mlp_class).fused) {
+          conf.conf_fuse_inspect = false;  // the next level inspects
+          as.swap_queues();
+        }
+      }
     }
     const double e = gsw_now_ms();
     const int L = g.level > 0 ? g.level : 1;
@@ -227,7 +251,7 @@ struct executor_t {
     fets.last_expand_time = stat.last_expand_time;
     if (fets.pattern == Idem && fets.avg_expand_time > 0 &&
         fets.last_expand_time > 5 * fets.avg_expand_time &&
-        conf.conf_fuse_inspect && !conf.force.has_fusion) {
+        conf.conf_fuse_inspect && !conf.force.has_fusion && !conf.policy_mlp) {
       conf.conf_fuse_inspect = false;
       as.swap_queues();
     }

@@ -22,6 +22,34 @@ inline double gsw_now_ms() {
 struct inspector_t {
   // raw counts of the current frontier, used to size queues without a sync
   long long known_active = 0, known_inactive = 0;
+  // learned selector (mlp.w == nullptr: rule trees only); the host keeps a
+  // copy of the weights for levels decided without an inspector pass
+  gsw_mlp::sel_dev_t mlp;
+  const float *mlp_host_w = nullptr;
+
+  // Selector inference for a level whose counters are known on the host
+  // (the first round): same features and argmin as the device path.
+  void mlp_host_select(const feature_t &fets, config_t &conf, int level) {
+    gsw_mlp::counts_t c;
+    c.n = mlp.n;
+    c.m = mlp.m;
+    c.avg_deg = mlp.avg_deg;
+    c.std_deg = mlp.std_deg;
+    c.gini = mlp.gini;
+    c.level = (float)level;
+    c.active = (float)fets.active_vertex;
+    c.inactive = (float)fets.unactive_vertex;
+    c.push_wl = (float)fets.push_workload;
+    c.pull_wl = (float)fets.pull_workload;
+    c.max_deg_active = (float)fets.cur_max_deg_active;
+    float x[gsw_mlp::NIN], y[gsw_mlp::NOUT];
+    gsw_mlp::features(c, x);
+    gsw_mlp::forward_host(mlp_host_w, x, y);
+    conf.mlp_class = gsw_mlp::argmin_valid(y, mlp.mask);
+    // running inactive count / pull workload for fused-level features
+    gsw::set_one_async<long long>(mlp.running, (long long)fets.unactive_vertex);
+    gsw::set_one_async<long long>(mlp.running + 1, (long long)fets.pull_workload);
+  }
 
   template <typename E, typename F>
   void inspect(active_set_t &as, device_graph_t<CSR, E> &g, F &f,
@@ -39,16 +67,23 @@ struct inspector_t {
     }
     if (need && conf.conf_first_round) {
       set_first_fets(as, fets, conf, (double)g.nvertexs, (double)g.nedges);
+      if (conf.policy_mlp && mlp.w) mlp_host_select(fets, conf, g.level);
       need = false;
     }
     if (need) {
       stat.clean();
+      gsw_mlp::sel_dev_t d;
+      if (conf.policy_mlp && mlp.w) {
+        d = mlp;
+        d.level = (float)g.level;
+      }
       hipLaunchKernelGGL((gsw::k_inspect_vc<device_graph_t<CSR, E>, F>),
                          dim3(gsw::grid_for((g.nvertexs + 63) / 64 * 64)),
                          dim3(gsw::BLOCK), 0, gsw::stream(), as, g, f,
-                         stat.dg_runtime_info);
+                         stat.dg_runtime_info, d);
       H_ERR(hipGetLastError());
       stat.collect();
+      if (d.w) conf.mlp_class = (int)stat.h_runtime_info[6];
       set_fets(as, stat, fets, conf, (double)g.nvertexs, (double)g.nedges);
     }
     fets.flatten();

@@ -9,6 +9,7 @@
 // >= 2 -> STRICT (selector.cuh:126-131). The frontier target status follows
 // from (direction, from-all, to-all) as at selector.cuh:292-302.
 #pragma once
+#include "../model/mlp.h"
 #include "../model/trees_table.h"
 #include "config.h"
 #include "features.h"
@@ -163,6 +164,31 @@ struct selector_t {
       conf.conf_lb = STRICT;
   }
 
+  // The learned selector's class for this level. Entering fused mode
+  // builds the queue from this level's bitmaps; leaving it happens after a
+  // fused expand (the next level then runs the inspector), so a fused level
+  // given a standalone class keeps the fused form with the class's LB.
+  static void select_learned(config_t &conf) {
+    const gsw_mlp::variant_t v = gsw_mlp::variant_of_class(conf.mlp_class);
+    if (v.fused || conf.conf_fuse_inspect) {
+      if (!conf.conf_fuse_inspect) conf.conf_switch_to_fusion = true;
+      conf.conf_fuse_inspect = true;
+      conf.conf_dir = Push;
+      conf.conf_asfmt = Queue;
+      conf.conf_sorted = false;
+    } else {
+      conf.conf_dir = v.dir;
+      conf.conf_asfmt = v.fmt;
+      conf.conf_sorted = v.sorted;
+    }
+    conf.conf_lb = v.lb;
+    if (conf.force.has_direction) conf.conf_dir = conf.force.d;
+    if (conf.force.has_asfmt) conf.conf_asfmt = conf.force.asf;
+    if (conf.force.has_lb) conf.conf_lb = conf.force.lb;
+    if (conf.force.has_sorted) conf.conf_sorted = conf.force.sorted;
+    set_target(conf);
+  }
+
   static void set_target(config_t &conf) {
     if (conf.conf_dir == Push)
       conf.conf_target = conf.conf_fromall ? All : Active;
@@ -188,6 +214,10 @@ struct selector_t {
     conf.conf_fromall = fets.fromall;
     conf.conf_toall = fets.toall;
 
+    if (conf.policy_mlp && conf.mlp_class >= 0) {
+      select_learned(conf);
+      return;
+    }
     if (fets.pattern == Idem)
       trigger_fusion(fets, conf);
     else

@@ -10,6 +10,7 @@
 #include "../data_structures/active_set.h"
 #include "../data_structures/functor.h"
 #include "../utils/wave.h"
+#include "mlp.h"
 
 namespace gsw {
 
@@ -26,7 +27,8 @@ __device__ __forceinline__ void flush_max(T x, T *sh, long long *dst) {
 
 template <typename G, typename F>
 __global__ void __launch_bounds__(BLOCK)
-    k_inspect_vc(active_set_t as, G g, F f, long long *info) {
+    k_inspect_vc(active_set_t as, G g, F f, long long *info,
+                 gsw_mlp::sel_dev_t mlp) {
   __shared__ long long sh[BLOCK / WAVE];
   const int lane = lane_id();
   const int64_t n = g.nvertexs;
@@ -66,6 +68,9 @@ __global__ void __launch_bounds__(BLOCK)
   flush_sum(pullw, sh, info + 3);
   flush_max(maxa, sh, info + 4);
   flush_max(maxi, sh, info + 5);
+  // learned selector: the last workgroup turns the counters into features
+  // and picks the next variant (info[6]) in the same launch
+  if (mlp.w) mlp_inspector_tail(mlp, info);
 }
 
 // Edge-centric inspector: an edge stays active while filter(v0, v1) says so.

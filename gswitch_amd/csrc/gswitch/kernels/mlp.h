@@ -1,0 +1,354 @@
+// HIP kernels of the learned selector (model/mlp.h): wave-per-sample
+// forward, argmin selection (used inside the inspector and after fused
+// expands), and training: per-wave partial gradients of the masked squared
+// error, a fixed-order reduction (so training is bit-reproducible) and Adam.
+//
+// Inside a wavefront lane j owns hidden unit j (and output j < 30): every
+// weight read W[k][j] is one coalesced row across the lanes, activations are
+// broadcast from LDS. No MFMA: the network is 6942 parameters and one level
+// needs 13.6 kFLOP.
+#pragma once
+#include "../model/mlp.h"
+#include "../utils/wave.h"
+
+namespace gsw {
+
+using namespace gsw_mlp;
+
+// per-wave LDS scratch of one sample
+struct mlp_wave_sh_t {
+  float x[16];
+  float h1[NH];
+  float h2[NH];
+  float dy[32];
+  float dh2[NH];
+};
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Forward pass of the sample in s.x; returns output j (lanes >= 30: 0).
+__device__ __forceinline__ float mlp_wave_forward(const float *w,
+                                                  mlp_wave_sh_t &s) {
+  const int j = lane_id();
+  float a = w[OB1 + j];
+#pragma unroll
+  for (int k = 0; k < NIN; ++k) a += s.x[k] * w[OW1 + k * NH + j];
+  s.h1[j] = a > 0.f ? a : 0.f;
+  wave_lds_sync();
+  a = w[OB2 + j];
+#pragma unroll 8
+  for (int k = 0; k < NH; ++k) a += s.h1[k] * w[OW2 + k * NH + j];
+  s.h2[j] = a > 0.f ? a : 0.f;
+  wave_lds_sync();
+  float y = 0.f;
+  if (j < NOUT) {
+    y = w[OB3 + j];
+#pragma unroll 8
+    for (int k = 0; k < NH; ++k) y += s.h2[k] * w[OW3 + k * NOUT + j];
+  }
+  return y;
+}
+
+// argmin over the classes set in `mask`, ties to the lower id (all lanes)
+__device__ __forceinline__ int wave_argmin(float y, uint32_t mask) {
+  const int j = lane_id();
+  float v = (j < NOUT && ((mask >> j) & 1u)) ? y : __builtin_inff();
+  int id = (v < __builtin_inff()) ? j : 1 << 30;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const float ov = __shfl_xor(v, o, 64);
+    const int oi = __shfl_xor(id, o, 64);
+    if (ov < v || (ov == v && oi < id)) {
+      v = ov;
+      id = oi;
+    }
+  }
+  return id >= NOUT ? -1 : id;
+}
+
+__device__ __forceinline__ void mlp_counts_features(const sel_dev_t &d, float act,
+                                                    float inact, float pushw,
+                                                    float pullw, float maxa,
+                                                    mlp_wave_sh_t &s) {
+  counts_t c;
+  c.n = d.n;
+  c.m = d.m;
+  c.avg_deg = d.avg_deg;
+  c.std_deg = d.std_deg;
+  c.gini = d.gini;
+  c.level = d.level;
+  c.active = act;
+  c.inactive = inact;
+  c.push_wl = pushw;
+  c.pull_wl = pullw;
+  c.max_deg_active = maxa;
+  float x[NIN];
+  features(c, x);
+  const int j = lane_id();
+#pragma unroll
+  for (int k = 0; k < NIN; ++k)
+    if (j == k) s.x[k] = x[k];
+  wave_lds_sync();
+}
+
+__device__ __forceinline__ long long load_agent(const long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Called by every workgroup of the inspector after its counters are flushed:
+// the last workgroup to finish reads the six counters and writes the chosen
+// class to info[6] (and the running inactive / pull workload).
+__device__ __forceinline__ void mlp_inspector_tail(const sel_dev_t &d,
+                                                   long long *info) {
+  __shared__ int s_last;
+  __shared__ mlp_wave_sh_t s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned t = atomicAdd(d.ticket, 1u);
+    s_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last || threadIdx.x >= WAVE) return;
+  __threadfence();
+  const float act = (float)load_agent(info + 0);
+  const float inact = (float)load_agent(info + 1);
+  const float pushw = (float)load_agent(info + 2);
+  const float pullw = (float)load_agent(info + 3);
+  const float maxa = (float)load_agent(info + 4);
+  mlp_counts_features(d, act, inact, pushw, pullw, maxa, s);
+  const float y = mlp_wave_forward(d.w, s);
+  const int cls = wave_argmin(y, d.mask);
+  if (lane_id() == 0) {
+    info[6] = cls;
+    d.running[0] = load_agent(info + 1);
+    d.running[1] = load_agent(info + 3);
+    *d.ticket = 0u;
+  }
+}
+
+constexpr int MLP_SELECT_GRID = 256;
+
+// After a fused expand: degree sum / maximum of the next frontier (the
+// queue), then the last workgroup derives the next level's features (the
+// frontier leaves the inactive set) and writes the class to *out.
+__global__ void __launch_bounds__(BLOCK)
+    k_mlp_fused_select(sel_dev_t d, const int32_t *queue, const int32_t *qsize,
+                       const int64_t *row_ptr, long long *acc, int32_t *out) {
+  __shared__ long long sh[BLOCK / WAVE];
+  __shared__ int s_last;
+  __shared__ mlp_wave_sh_t s;
+  const int64_t nq = *qsize;
+  long long sum = 0, mx = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int u = queue[i];
+    const long long dg = row_ptr[u + 1] - row_ptr[u];
+    sum += dg;
+    mx = dg > mx ? dg : mx;
+  }
+  sum = block_sum(sum, sh);
+  mx = block_max(mx, sh);
+  if (threadIdx.x == 0) {
+    if (sum) atomicAdd((unsigned long long *)acc, (unsigned long long)sum);
+    if (mx) atomicMax(acc + 1, mx);
+    __threadfence();
+    const unsigned t = atomicAdd(d.ticket, 1u);
+    s_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last || threadIdx.x >= WAVE) return;
+  __threadfence();
+  const long long deg_sum = load_agent(acc);
+  const long long deg_max = load_agent(acc + 1);
+  long long inact = load_agent(d.running) - nq;
+  long long pullw = load_agent(d.running + 1) - deg_sum;
+  inact = inact < 0 ? 0 : inact;
+  pullw = pullw < 0 ? 0 : pullw;
+  mlp_counts_features(d, (float)nq, (float)inact, (float)deg_sum, (float)pullw,
+                      (float)deg_max, s);
+  const float y = mlp_wave_forward(d.w, s);
+  const int cls = wave_argmin(y, d.mask);
+  if (lane_id() == 0) {
+    *out = cls;
+    d.running[0] = inact;
+    d.running[1] = pullw;
+    acc[0] = 0;
+    acc[1] = 0;
+    *d.ticket = 0u;
+  }
+}
+
+// ---- batch paths (training, tests) -----------------------------------------
+__device__ __forceinline__ void load_sample_x(const float *x, int64_t i,
+                                              mlp_wave_sh_t &s) {
+  const int j = lane_id();
+  if (j < NIN) s.x[j] = x[i * NIN + j];
+  wave_lds_sync();
+}
+
+// y[i][0..30) for B samples, one wavefront per sample at a time
+__global__ void __launch_bounds__(BLOCK)
+    k_mlp_forward_batch(const float *w, const float *x, int64_t B, float *y) {
+  __shared__ mlp_wave_sh_t sh[BLOCK / WAVE];
+  mlp_wave_sh_t &s = sh[threadIdx.x >> LOG_WAVE];
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> LOG_WAVE;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> LOG_WAVE;
+  for (int64_t i = wave; i < B; i += nwaves) {
+    load_sample_x(x, i, s);
+    const float v = mlp_wave_forward(w, s);
+    if (lane_id() < NOUT) y[i * NOUT + lane_id()] = v;
+    wave_lds_sync();
+  }
+}
+
+// class[i] = argmin over `mask` of the forward pass (the inference path)
+__global__ void __launch_bounds__(BLOCK)
+    k_mlp_select_batch(const float *w, const float *x, int64_t B, uint32_t mask,
+                       int32_t *cls) {
+  __shared__ mlp_wave_sh_t sh[BLOCK / WAVE];
+  mlp_wave_sh_t &s = sh[threadIdx.x >> LOG_WAVE];
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> LOG_WAVE;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> LOG_WAVE;
+  for (int64_t i = wave; i < B; i += nwaves) {
+    load_sample_x(x, i, s);
+    const int c = wave_argmin(mlp_wave_forward(w, s), mask);
+    if (lane_id() == 0) cls[i] = c;
+    wave_lds_sync();
+  }
+}
+
+// Weights staged in LDS with transposed copies of W2 and W3 for the
+// backward pass (lane j reads row j as a coalesced column).
+struct mlp_lds_w_t {
+  float w[NPARAM];
+  float w2t[NH * NH];    // w2t[c][j] = W2[j][c]
+  float w3t[NOUT * NH];  // w3t[c][j] = W3[j][c]
+};
+
+// Partial gradients of L = scale * sum_i sum_c mask (y - t)^2 over the
+// samples idx[off + i] (idx == nullptr: off + i), i < B. Wavefront g
+// handles samples g*per .. g*per+per-1 and writes its partial gradient to
+// part[g][0..NPARAM) and its partial loss to part_loss[g].
+__global__ void __launch_bounds__(BLOCK)
+    k_mlp_grad_partial(const float *w, const float *x, const float *t,
+                       const float *mask, const int32_t *idx, int64_t off,
+                       int64_t B, int per, float scale, float *part,
+                       float *part_loss) {
+  extern __shared__ float dyn[];
+  mlp_lds_w_t &L = *reinterpret_cast<mlp_lds_w_t *>(dyn);
+  mlp_wave_sh_t *shw = reinterpret_cast<mlp_wave_sh_t *>(dyn + sizeof(mlp_lds_w_t) / 4);
+  for (int p = threadIdx.x; p < NPARAM; p += blockDim.x) L.w[p] = w[p];
+  for (int p = threadIdx.x; p < NH * NH; p += blockDim.x) {
+    const int c = p / NH, j = p % NH;
+    L.w2t[p] = w[OW2 + j * NH + c];
+  }
+  for (int p = threadIdx.x; p < NOUT * NH; p += blockDim.x) {
+    const int c = p / NH, j = p % NH;
+    L.w3t[p] = w[OW3 + j * NOUT + c];
+  }
+  __syncthreads();
+  mlp_wave_sh_t &s = shw[threadIdx.x >> LOG_WAVE];
+  const int j = lane_id();
+  const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> LOG_WAVE;
+  float g1[NIN], g2[NH], g3[NH];
+  float gb1 = 0.f, gb2 = 0.f, gb3 = 0.f, lsum = 0.f;
+#pragma unroll
+  for (int k = 0; k < NIN; ++k) g1[k] = 0.f;
+#pragma unroll
+  for (int k = 0; k < NH; ++k) {
+    g2[k] = 0.f;
+    g3[k] = 0.f;
+  }
+  const int64_t i0 = g * per;
+  const int64_t i1 = (i0 + per < B) ? i0 + per : B;
+  for (int64_t i = i0; i < i1; ++i) {
+    const int64_t r = idx ? (int64_t)idx[off + i] : off + i;
+    load_sample_x(x, r, s);
+    const float y = mlp_wave_forward(L.w, s);
+    float dy = 0.f;
+    if (j < NOUT) {
+      const float mk = mask[r * NOUT + j];
+      const float diff = y - t[r * NOUT + j];
+      lsum += mk * diff * diff;
+      dy = 2.f * scale * mk * diff;
+      s.dy[j] = dy;
+    }
+    gb3 += dy;
+#pragma unroll
+    for (int k = 0; k < NH; ++k) g3[k] += s.h2[k] * dy;
+    wave_lds_sync();
+    float dh2 = 0.f;
+    if (s.h2[j] > 0.f) {
+#pragma unroll 6
+      for (int c = 0; c < NOUT; ++c) dh2 += L.w3t[c * NH + j] * s.dy[c];
+    }
+    s.dh2[j] = dh2;
+    gb2 += dh2;
+#pragma unroll
+    for (int k = 0; k < NH; ++k) g2[k] += s.h1[k] * dh2;
+    wave_lds_sync();
+    float dh1 = 0.f;
+    if (s.h1[j] > 0.f) {
+#pragma unroll 8
+      for (int c = 0; c < NH; ++c) dh1 += L.w2t[c * NH + j] * s.dh2[c];
+    }
+    gb1 += dh1;
+#pragma unroll
+    for (int k = 0; k < NIN; ++k) g1[k] += s.x[k] * dh1;
+    wave_lds_sync();
+  }
+  float *o = part + g * NPARAM;
+#pragma unroll
+  for (int k = 0; k < NIN; ++k) o[OW1 + k * NH + j] = g1[k];
+  o[OB1 + j] = gb1;
+#pragma unroll
+  for (int k = 0; k < NH; ++k) o[OW2 + k * NH + j] = g2[k];
+  o[OB2 + j] = gb2;
+  if (j < NOUT) {
+#pragma unroll
+    for (int k = 0; k < NH; ++k) o[OW3 + k * NOUT + j] = g3[k];
+    o[OB3 + j] = gb3;
+  }
+  lsum = wave_sum_any(lsum);
+  if (j == 0) part_loss[g] = lsum;
+}
+
+constexpr size_t MLP_GRAD_LDS = sizeof(mlp_lds_w_t) + sizeof(mlp_wave_sh_t) * (BLOCK / WAVE);
+
+// out[p] = sum over partials in ascending order (bit-reproducible)
+__global__ void __launch_bounds__(BLOCK)
+    k_mlp_reduce(const float *part, const float *part_loss, int nparts,
+                 float *out, float *loss) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < NPARAM) {
+    float s = 0.f;
+    for (int q = 0; q < nparts; ++q) s += part[(int64_t)q * NPARAM + p];
+    out[p] = s;
+  }
+  if (p == 0 && loss) {
+    float s = 0.f;
+    for (int q = 0; q < nparts; ++q) s += part_loss[q];
+    *loss = s;
+  }
+}
+
+// Adam with bias correction folded into lr_t (computed on the host)
+__global__ void __launch_bounds__(BLOCK)
+    k_adam(float *w, const float *grad, float *m1, float *m2, int n, float lr_t,
+           float b1, float b2, float eps) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const float gr = grad[p];
+  const float a = b1 * m1[p] + (1.f - b1) * gr;
+  const float b = b2 * m2[p] + (1.f - b2) * gr * gr;
+  m1[p] = a;
+  m2[p] = b;
+  w[p] -= lr_t * a / (sqrtf(b) + eps);
+}
+
+}  // namespace gsw

@@ -1,0 +1,254 @@
+"""Training pipeline of the learned variant selector (BFS).
+
+1. `collect`: the engine times all 30 variant classes at every probed BFS
+   level of seeded synthetic graphs (gsw_explore_bfs: snapshot, run each
+   class, restore) and records the level's 12 features and the class the
+   rule trees would pick.
+2. `train`: targets are log2(time / fastest time) per class, clipped at
+   CLIP; the 12 -> 64 -> 64 -> 30 network is fitted by the library's own
+   HIP forward / backward / Adam kernels (gsw_mlp_train).
+3. `evaluate`: on held-out graphs, per-level regret of the network's and
+   the rule trees' choices against the exhaustive optimum, and end-to-end
+   BFS time under the rule trees, the network, and fixed strategies.
+
+Run as a module on a GPU machine:
+    python -m gswitch_amd.learn collect --set train --out DIR
+    python -m gswitch_amd.learn train --data DIR/train.npz --out DIR
+    python -m gswitch_amd.learn collect --set eval --out DIR
+    python -m gswitch_amd.learn evaluate --data DIR/eval.npz --model M --out DIR
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+import gswitch_amd as gsw
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MODEL_PATH = os.path.join(HERE, "model", "selector_bfs.npz")
+CLIP = 6.0
+TRAIN_SEED, EVAL_SEED = 20240501, 20240777
+
+
+def _u(rng, lo, hi):
+    return float(rng.uniform(lo, hi))
+
+
+def training_specs(count=300, seed=TRAIN_SEED):
+    """R-MAT scales 14-20 (a in [0.45, 0.65], b = c = (1 - a - 0.05) / 2),
+    Barabasi-Albert (m in [2, 16]), Erdos-Renyi and perturbed 2-D grids."""
+    rng = np.random.default_rng(seed)
+    kinds = ["rmat"] * 120 + ["ba"] * 60 + ["er"] * 60 + ["grid"] * 60
+    kinds = [kinds[i] for i in rng.permutation(len(kinds))][:count]
+    out = []
+    for i, k in enumerate(kinds):
+        s = dict(kind=k, seed=int(rng.integers(1, 1 << 30)),
+                 root="max" if i % 2 == 0 else "random")
+        if k == "rmat":
+            a = _u(rng, 0.45, 0.65)
+            s.update(scale=int(rng.integers(14, 21)), ef=int(rng.choice([8, 16])),
+                     a=a, b=(1 - a - 0.05) / 2)
+        elif k == "ba":
+            s.update(n=int(2 ** rng.integers(14, 19)), m=int(rng.integers(2, 17)))
+        elif k == "er":
+            s.update(n=int(2 ** rng.integers(14, 20)), d=_u(rng, 2, 32))
+        else:
+            s.update(rows=int(rng.integers(64, 1025)), cols=int(rng.integers(64, 1025)),
+                     p=_u(rng, 0.0, 0.3), lr=_u(rng, 0.0, 0.02))
+        out.append(s)
+    return out
+
+
+def eval_specs(seed=EVAL_SEED):
+    """40 held-out graphs at scales 20-24."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for scale in [20] * 4 + [21] * 4 + [22] * 4 + [23] * 2 + [24] * 2:
+        a = _u(rng, 0.5, 0.62)
+        out.append(dict(kind="rmat", scale=scale, ef=16, a=a, b=(1 - a - 0.05) / 2,
+                        seed=int(rng.integers(1, 1 << 30)), root="max"))
+    for _ in range(8):
+        out.append(dict(kind="er", n=int(2 ** rng.integers(20, 23)), d=_u(rng, 4, 32),
+                        seed=int(rng.integers(1, 1 << 30)), root="random"))
+    for _ in range(8):
+        out.append(dict(kind="ba", n=int(2 ** rng.integers(20, 22)),
+                        m=int(rng.integers(4, 17)),
+                        seed=int(rng.integers(1, 1 << 30)), root="max"))
+    for _ in range(8):
+        side = int(rng.integers(1024, 2049))
+        out.append(dict(kind="grid", rows=side, cols=side, p=_u(rng, 0.0, 0.2),
+                        lr=_u(rng, 0.0, 0.01), seed=int(rng.integers(1, 1 << 30)),
+                        root="random"))
+    return out
+
+
+def build(spec):
+    k = spec["kind"]
+    if k == "rmat":
+        return gsw.rmat(spec["scale"], spec["ef"], spec["a"], spec["b"], spec["b"],
+                        seed=spec["seed"])
+    if k == "ba":
+        return gsw.barabasi_albert(spec["n"], spec["m"], seed=spec["seed"])
+    if k == "er":
+        return gsw.erdos_renyi(spec["n"], spec["d"], seed=spec["seed"])
+    return gsw.grid2d(spec["rows"], spec["cols"], spec["p"], spec["lr"],
+                      seed=spec["seed"])
+
+
+def pick_root(spec, h):
+    if spec["root"] == "max":
+        return h.max_degree_vertex()
+    deg = h.degrees
+    nz = np.flatnonzero(deg > 0)
+    rng = np.random.default_rng(spec["seed"] ^ 0x5EED)
+    return int(nz[rng.integers(len(nz))]) if len(nz) else 0
+
+
+def level_stride(depth, max_levels):
+    """Probe every level of shallow traversals, and about max_levels evenly
+    spaced levels of deep ones (high-diameter grids)."""
+    return max(1, (depth + max_levels - 1) // max_levels)
+
+
+def collect(specs, reps=2, max_levels=32, log=print):
+    rows = dict(features=[], ms=[], rule=[], level=[], graph=[])
+    t0 = time.time()
+    for gi, spec in enumerate(specs):
+        h = build(spec)
+        g = gsw.Graph(h)
+        root = pick_root(spec, h)
+        depth = int(g.bfs(root).max()) + 1
+        ex = g.explore_bfs(root, max_levels=max_levels,
+                           stride=level_stride(depth, max_levels), reps=reps)
+        g.close()
+        for key in ["features", "ms", "rule", "level"]:
+            rows[key].append(ex[key])
+        rows["graph"].append(np.full(len(ex["level"]), gi, np.int32))
+        log("graph %d/%d %s n=%d m=%d levels=%d (%.0fs)" % (
+            gi + 1, len(specs), spec["kind"], h.n, h.m, len(ex["level"]),
+            time.time() - t0))
+    return {k: np.concatenate(v) for k, v in rows.items()}
+
+
+def targets(ms):
+    best = ms.min(axis=1, keepdims=True)
+    y = np.log2(np.maximum(ms, 1e-6) / np.maximum(best, 1e-6))
+    return np.minimum(y, CLIP).astype(np.float32), np.ones_like(y, np.float32)
+
+
+def init_weights(seed=1):
+    """He-normal hidden layers, small output layer, zero biases."""
+    rng = np.random.default_rng(seed)
+    w = np.zeros(gsw.MLP_NPARAM, np.float32)
+    w[0:768] = rng.normal(0, np.sqrt(2 / 12), 768)
+    w[832:4928] = rng.normal(0, np.sqrt(2 / 64), 4096)
+    w[4992:6912] = rng.normal(0, 0.1 * np.sqrt(1 / 64), 1920)
+    w[6912:] = 1.0
+    return w
+
+
+def train(data, epochs=400, batch=128, lr=2e-3, seed=7):
+    y, m = targets(data["ms"])
+    w, curve = gsw.mlp_train(init_weights(), data["features"], y, m, epochs=epochs,
+                             batch=batch, lr=lr, seed=seed)
+    return w, curve
+
+
+def per_level_regret(ms, choice):
+    """sum over levels of the chosen class's time / the optimum's."""
+    idx = np.arange(len(ms))
+    return float(ms[idx, choice].sum() / ms.min(axis=1).sum())
+
+
+def save_model(path, w, meta):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.savez(path, w=np.asarray(w, np.float32), meta=json.dumps(meta))
+
+
+def load_model(path=MODEL_PATH):
+    with np.load(path, allow_pickle=False) as z:
+        w = z["w"].astype(np.float32)
+    if w.size != gsw.MLP_NPARAM or not np.isfinite(w).all():
+        raise ValueError("bad selector model")
+    return w
+
+
+FIXED = ["Push-Queue-WM-Standalone", "Push-Queue-WM-Fused", None]
+
+
+def end_to_end(specs, w, reps=3, log=print):
+    """Median BFS device time per graph under the rule trees, the network
+    and fixed strategies (results checked identical)."""
+    gsw.selector_load(w)
+    out = []
+    for spec in specs:
+        h = build(spec)
+        g = gsw.Graph(h)
+        root = pick_root(spec, h)
+        ref = g.bfs(root)
+        rec = dict(kind=spec["kind"], n=h.n, m=h.m)
+        for name, kw in [("rule", {}), ("mlp", dict(selector=1)),
+                         ("fixed_wm", dict(configs=FIXED[0])),
+                         ("fixed_fused", dict(configs=FIXED[1]))]:
+            ts = []
+            for _ in range(reps):
+                lv = g.bfs(root, profile=False, **kw)
+                assert np.array_equal(lv, ref), (name, spec)
+                ts.append(g.last_stats.device_ms)
+            rec[name] = float(np.median(ts))
+        g.close()
+        log("%s n=%d rule %.3f mlp %.3f fixed %.3f / %.3f ms" % (
+            spec["kind"], h.n, rec["rule"], rec["mlp"], rec["fixed_wm"],
+            rec["fixed_fused"]))
+        out.append(rec)
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("cmd", choices=["collect", "train", "evaluate"])
+    ap.add_argument("--set", default="train", choices=["train", "eval"])
+    ap.add_argument("--count", type=int, default=300)
+    ap.add_argument("--data")
+    ap.add_argument("--model", default=MODEL_PATH)
+    ap.add_argument("--out", default="gpurun_out/selector")
+    ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--epochs", type=int, default=400)
+    a = ap.parse_args(argv)
+    os.makedirs(a.out, exist_ok=True)
+    log = lambda s: print(s, flush=True)
+    if a.cmd == "collect":
+        specs = training_specs(a.count) if a.set == "train" else eval_specs()
+        d = collect(specs, reps=a.reps, log=log)
+        np.savez(os.path.join(a.out, a.set + ".npz"), specs=json.dumps(specs), **d)
+    elif a.cmd == "train":
+        with np.load(a.data, allow_pickle=False) as z:
+            d = {k: z[k] for k in z.files}
+        w, curve = train(d, epochs=a.epochs)
+        log("loss %.4f -> %.4f" % (curve[0], curve[-1]))
+        cls = gsw.mlp_select(w, d["features"])
+        log("train regret: mlp %.3f rule %.3f" % (per_level_regret(d["ms"], cls),
+                                                  per_level_regret(d["ms"], d["rule"])))
+        save_model(os.path.join(a.out, "selector_bfs.npz"), w,
+                   dict(samples=int(len(d["ms"])), epochs=a.epochs,
+                        loss_first=float(curve[0]), loss_last=float(curve[-1])))
+    else:
+        with np.load(a.data, allow_pickle=False) as z:
+            d = {k: z[k] for k in z.files}
+        w = load_model(a.model)
+        cls = gsw.mlp_select(w, d["features"])
+        res = dict(levels=int(len(d["ms"])),
+                   regret_mlp=per_level_regret(d["ms"], cls),
+                   regret_rule=per_level_regret(d["ms"], d["rule"]))
+        log("eval per-level regret: mlp %.3f rule %.3f" % (res["regret_mlp"],
+                                                           res["regret_rule"]))
+        res["end_to_end"] = end_to_end(json.loads(str(d["specs"])), w, log=log)
+        with open(os.path.join(a.out, "eval.json"), "w") as f:
+            json.dump(res, f, indent=1)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

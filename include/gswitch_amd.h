@@ -100,6 +100,8 @@ typedef struct {
                           2*capture levels per replay; 0 = per-level host loop */
   int32_t grid;        /* gsw_bfs_multi captured loop: workgroups per launch
                           (0 = sized from the graph) */
+  int32_t selector;    /* 0 rule-based trees (src/model/select_*.h);
+                          1 learned MLP (weights from gsw_selector_load) */
 } gsw_run_opts;
 void gsw_run_opts_init(gsw_run_opts *o);
 
@@ -202,6 +204,49 @@ gsw_status gsw_last_records(const gsw_graph *g, gsw_level_record *out,
  * ("select_dir_idem", "select_lb_idem", ...) on a feature vector. */
 gsw_status gsw_tree_eval(const char *name, const double *fv, int32_t nfv,
                          int32_t *cls);
+
+/* ---- learned selector (no counterpart in the reference, which trains its
+ *      trees offline from feature dumps: src/abstraction/features.cuh:86-104)
+ * MLP 12 -> 64 -> 64 -> 30, FP32, 6942 parameters laid out as
+ * W1[12][64] b1[64] W2[64][64] b2[64] W3[64][30] b3[30]. Output c predicts
+ * log2(cost of variant class c / cost of the fastest class) for a level;
+ * class = group*5 + lb, lb in {TM, WM, CM, STRICT, TWC}, groups
+ * Push-Queue, Push-SortedQueue, Push-Bitmap, Push-Queue-Fused, Pull-Queue,
+ * Pull-Bitmap. Inputs: see gsw_mlp_features. */
+#define GSW_MLP_NPARAM 6942
+#define GSW_MLP_NIN 12
+#define GSW_MLP_NOUT 30
+/* Install the weights used when gsw_run_opts.selector == 1. */
+gsw_status gsw_selector_load(const float *w, int32_t nparam);
+/* The 12 inputs from counts[11] = {n, m, avg deg, std deg, Gini, level,
+ * active, inactive, push workload, pull workload, max active degree}. */
+gsw_status gsw_mlp_features(const double *counts, float *x);
+/* Device forward pass: y[B][30]. */
+gsw_status gsw_mlp_forward(const float *w, const float *x, int64_t B, float *y);
+/* Device selection (the inference path of the inspector): cls[B] = argmin of
+ * the outputs over the classes set in `mask`. */
+gsw_status gsw_mlp_select(const float *w, const float *x, int64_t B,
+                          uint32_t mask, int32_t *cls);
+/* Device loss and gradient of L = sum(mask*(y-t)^2) / sum(mask). */
+gsw_status gsw_mlp_grad(const float *w, const float *x, const float *t,
+                        const float *mask, int64_t B, float *loss,
+                        float *grad);
+/* Adam training on the device (minibatches in a seeded shuffled order,
+ * deterministic reductions): w_inout holds the initial weights and receives
+ * the trained ones; loss_curve (optional) gets the mean loss per epoch. */
+gsw_status gsw_mlp_train(const float *x, const float *t, const float *mask,
+                         int64_t B, int32_t epochs, int32_t batch, float lr,
+                         uint64_t seed, float *w_inout, float *loss_curve);
+/* Exhaustive per-level timing for BFS from `root`: for each probed level
+ * (every `stride`-th, at most max_levels) the 12 features, the wall time in
+ * ms of each of the 30 classes run from a snapshot of the level state
+ * (best of `reps`; standalone classes include the inspector and filter,
+ * fused ones the expand and its queue-length read), and the class the rule
+ * selector would pick. *n_out = levels written. */
+gsw_status gsw_explore_bfs(gsw_graph *g, int32_t root, int32_t max_levels,
+                           int32_t stride, int32_t reps, float *feat_out,
+                           float *ms_out, int32_t *rule_out, int32_t *level_out,
+                           int32_t *n_out);
 
 /* ---- primitives (replace moderngpu's mgpu::scan / sorted_search used at
  *      src/kernel_libs/expand_VC_STRICT.cuh:540-552) --------------------- */

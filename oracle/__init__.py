@@ -118,3 +118,28 @@ def traversed_edges(row_ptr, levels):
     rp = np.ascontiguousarray(row_ptr, np.int64)
     lv = np.ascontiguousarray(levels, np.int32)
     return _lib.orc_traversed_edges(len(rp) - 1, _p(rp), _p(lv))
+
+
+_lib.orc_mlp_forward.argtypes = [_P, _P, C.c_int64, _P]
+_lib.orc_mlp_forward.restype = None
+_lib.orc_mlp_grad.argtypes = [_P, _P, _P, _P, C.c_int64, _P]
+_lib.orc_mlp_grad.restype = C.c_double
+
+
+def mlp_forward(w, x):
+    w = np.ascontiguousarray(w, np.float64)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 12)
+    y = np.empty((len(x), 30), np.float64)
+    _lib.orc_mlp_forward(_p(w), _p(x), len(x), _p(y))
+    return y
+
+
+def mlp_grad(w, x, t, mask):
+    """(loss, grad) of sum(mask*(y-t)^2)/sum(mask) in FP64."""
+    w = np.ascontiguousarray(w, np.float64)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 12)
+    t = np.ascontiguousarray(t, np.float64).reshape(-1, 30)
+    mask = np.ascontiguousarray(mask, np.float64).reshape(-1, 30)
+    g = np.empty(6942, np.float64)
+    loss = _lib.orc_mlp_grad(_p(w), _p(x), _p(t), _p(mask), len(x), _p(g))
+    return loss, g

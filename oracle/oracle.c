@@ -288,3 +288,81 @@ int64_t orc_traversed_edges(int64_t n, const int64_t *row_ptr,
     if (reached_level[v] >= 0) s += row_ptr[v + 1] - row_ptr[v];
   return s / 2;
 }
+
+/* ---- learned-selector network (FP64) ------------------------------------ */
+enum { MLP_IN = 12, MLP_H = 64, MLP_OUT = 30 };
+enum {
+  MLP_W1 = 0,
+  MLP_B1 = MLP_W1 + MLP_IN * MLP_H,
+  MLP_W2 = MLP_B1 + MLP_H,
+  MLP_B2 = MLP_W2 + MLP_H * MLP_H,
+  MLP_W3 = MLP_B2 + MLP_H,
+  MLP_B3 = MLP_W3 + MLP_H * MLP_OUT,
+  MLP_NP = MLP_B3 + MLP_OUT
+};
+
+static void mlp_one(const double *w, const double *x, double *h1, double *h2,
+                    double *y) {
+  for (int j = 0; j < MLP_H; ++j) {
+    double a = w[MLP_B1 + j];
+    for (int k = 0; k < MLP_IN; ++k) a += x[k] * w[MLP_W1 + k * MLP_H + j];
+    h1[j] = a > 0 ? a : 0;
+  }
+  for (int j = 0; j < MLP_H; ++j) {
+    double a = w[MLP_B2 + j];
+    for (int k = 0; k < MLP_H; ++k) a += h1[k] * w[MLP_W2 + k * MLP_H + j];
+    h2[j] = a > 0 ? a : 0;
+  }
+  for (int j = 0; j < MLP_OUT; ++j) {
+    double a = w[MLP_B3 + j];
+    for (int k = 0; k < MLP_H; ++k) a += h2[k] * w[MLP_W3 + k * MLP_OUT + j];
+    y[j] = a;
+  }
+}
+
+void orc_mlp_forward(const double *w, const double *x, int64_t B, double *y) {
+  double h1[MLP_H], h2[MLP_H];
+  for (int64_t i = 0; i < B; ++i)
+    mlp_one(w, x + i * MLP_IN, h1, h2, y + i * MLP_OUT);
+}
+
+double orc_mlp_grad(const double *w, const double *x, const double *t,
+                    const double *mask, int64_t B, double *grad) {
+  double h1[MLP_H], h2[MLP_H], y[MLP_OUT], dy[MLP_OUT], dh2[MLP_H], dh1[MLP_H];
+  double msum = 0, loss = 0;
+  for (int64_t i = 0; i < B * MLP_OUT; ++i) msum += mask[i];
+  const double scale = msum > 0 ? 1.0 / msum : 0.0;
+  memset(grad, 0, sizeof(double) * MLP_NP);
+  for (int64_t i = 0; i < B; ++i) {
+    const double *xi = x + i * MLP_IN;
+    mlp_one(w, xi, h1, h2, y);
+    for (int c = 0; c < MLP_OUT; ++c) {
+      const double mk = mask[i * MLP_OUT + c];
+      const double d = y[c] - t[i * MLP_OUT + c];
+      loss += mk * d * d;
+      dy[c] = 2.0 * scale * mk * d;
+      grad[MLP_B3 + c] += dy[c];
+    }
+    for (int k = 0; k < MLP_H; ++k) {
+      double s = 0;
+      for (int c = 0; c < MLP_OUT; ++c) {
+        grad[MLP_W3 + k * MLP_OUT + c] += h2[k] * dy[c];
+        s += w[MLP_W3 + k * MLP_OUT + c] * dy[c];
+      }
+      dh2[k] = h2[k] > 0 ? s : 0;
+    }
+    for (int j = 0; j < MLP_H; ++j) grad[MLP_B2 + j] += dh2[j];
+    for (int k = 0; k < MLP_H; ++k) {
+      double s = 0;
+      for (int j = 0; j < MLP_H; ++j) {
+        grad[MLP_W2 + k * MLP_H + j] += h1[k] * dh2[j];
+        s += w[MLP_W2 + k * MLP_H + j] * dh2[j];
+      }
+      dh1[k] = h1[k] > 0 ? s : 0;
+    }
+    for (int j = 0; j < MLP_H; ++j) grad[MLP_B1 + j] += dh1[j];
+    for (int k = 0; k < MLP_IN; ++k)
+      for (int j = 0; j < MLP_H; ++j) grad[MLP_W1 + k * MLP_H + j] += xi[k] * dh1[j];
+  }
+  return loss * scale;
+}

@@ -76,6 +76,16 @@ void orc_msbfs64(int64_t n, const int64_t *row_ptr, const int32_t *col,
 int64_t orc_traversed_edges(int64_t n, const int64_t *row_ptr,
                             const int32_t *reached_level);
 
+/* Learned-selector network, FP64 (no reference counterpart: the reference
+ * selects with trees only). 12 -> 64 -> 64 -> 30 with ReLU on both hidden
+ * layers; w = W1[12][64] b1[64] W2[64][64] b2[64] W3[64][30] b3[30]
+ * (6942 doubles, input index first). y[B][30]. */
+void orc_mlp_forward(const double *w, const double *x, int64_t B, double *y);
+/* L = sum(mask * (y - t)^2) / sum(mask) (0 if the mask is empty); grad
+ * receives dL/dw (6942 doubles). Returns L. */
+double orc_mlp_grad(const double *w, const double *x, const double *t,
+                    const double *mask, int64_t B, double *grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,159 @@
+"""GPU checks of the learned selector: the HIP forward / selection /
+gradient kernels against the FP64 oracle network, reproducible training,
+the engine's exhaustive per-level timing (its features against counts
+recomputed from oracle BFS levels), and BFS under the learned policy staying
+bit-exact whatever classes the network picks."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NP = 6942
+
+
+def rand_problem(seed, B=50, scale=0.25):
+    rng = np.random.default_rng(seed)
+    w = rng.normal(0, scale, NP).astype(np.float32)
+    x = rng.random((B, 12)).astype(np.float32)
+    t = (rng.random((B, 30)) * 3).astype(np.float32)
+    m = (rng.random((B, 30)) > 0.25).astype(np.float32)
+    return w, x, t, m
+
+
+def test_forward_matches_oracle(gsw, orc):
+    w, x, _, _ = rand_problem(0, B=300)
+    y = gsw.mlp_forward(w, x)
+    ref = orc.mlp_forward(w, x)
+    np.testing.assert_allclose(y, ref, rtol=1e-4, atol=1e-5 * np.abs(ref).max())
+
+
+def test_select_is_masked_argmin(gsw, orc):
+    w, x, _, _ = rand_problem(1, B=400)
+    ref = orc.mlp_forward(w, x)
+    for mask in [(1 << 30) - 1, 0b10101010101010101010101010101, 1 << 17]:
+        cls = gsw.mlp_select(w, x, mask)
+        allowed = np.array([(mask >> c) & 1 for c in range(30)], bool)
+        r = np.where(allowed, ref, np.inf)
+        assert allowed[cls].all()
+        # the chosen class is a minimum up to FP32 rounding
+        best = r.min(axis=1)
+        assert np.all(r[np.arange(len(x)), cls] <= best + 1e-5 * (1 + np.abs(best)))
+
+
+@pytest.mark.parametrize("seed,B", [(2, 1), (3, 37), (4, 513)])
+def test_gradient_matches_oracle(gsw, orc, seed, B):
+    w, x, t, m = rand_problem(seed, B=B)
+    loss, g = gsw.mlp_grad(w, x, t, m)
+    rl, rg = orc.mlp_grad(w, x, t, m)
+    assert loss == pytest.approx(rl, rel=1e-4)
+    np.testing.assert_allclose(g, rg, rtol=1e-3, atol=1e-5 * np.abs(rg).max())
+
+
+def test_training_reproducible_and_learns(gsw, orc):
+    rng = np.random.default_rng(5)
+    x = rng.random((700, 12)).astype(np.float32)
+    teacher = rng.normal(0, 0.3, NP)
+    t = orc.mlp_forward(teacher, x).astype(np.float32)
+    m = np.ones_like(t)
+    w0 = (rng.normal(0, 1, NP) * 0.1).astype(np.float32)
+    w1, c1 = gsw.mlp_train(w0, x, t, m, epochs=40, batch=64, lr=3e-3, seed=9)
+    w2, c2 = gsw.mlp_train(w0, x, t, m, epochs=40, batch=64, lr=3e-3, seed=9)
+    assert np.array_equal(w1, w2) and np.array_equal(c1, c2)
+    assert c1[-1] < 0.3 * c1[0]
+    # the reported epoch loss is the loss of the data under the weights
+    l_end, _ = orc.mlp_grad(w1, x, t, m)
+    assert l_end < c1[0]
+
+
+def bfs_counts(h, lv, l):
+    deg = h.degrees.astype(np.float64)
+    act = lv == l
+    inact = (lv < 0) | (lv > l)
+    return (act.sum(), inact.sum(), deg[act].sum(), deg[inact].sum(),
+            deg[act].max() if act.any() else 0)
+
+
+def test_explore_bfs_features_and_times(gsw, orc):
+    h = gsw.rmat(12, 16, seed=1)
+    g = gsw.Graph(h)
+    root = h.max_degree_vertex()
+    lv = orc.bfs(h.row_ptr, h.col, root)
+    ex = g.explore_bfs(root, max_levels=64, stride=1, reps=1)
+    # levels the engine runs: a frontier exists and something is unvisited
+    live = [l for l in range(lv.max() + 1) if bfs_counts(h, lv, l)[1] > 0]
+    assert list(ex["level"]) == live and len(live) >= 3
+    assert np.all(ex["ms"] > 0) and np.isfinite(ex["ms"]).all()
+    assert np.all((ex["rule"] >= 0) & (ex["rule"] < 30))
+    for k, l in enumerate(ex["level"]):
+        a, i, pw, lw, md = bfs_counts(h, lv, l)
+        c = (h.n, h.m, h.degrees.mean(), h.degrees.std(ddof=1),
+             gsw.degree_stats(h.row_ptr)["GI"], l, a, i, pw, lw, md)
+        np.testing.assert_allclose(ex["features"][k], gsw.mlp_features(c),
+                                   rtol=1e-5, atol=1e-6)
+    # the probes leave a finished BFS behind: levels still bit-exact after
+    assert np.array_equal(g.bfs(root), lv)
+
+
+def test_explore_stride(gsw, orc):
+    h = gsw.grid2d(30, 30, 0.0, 0.0, 3)
+    g = gsw.Graph(h)
+    ex = g.explore_bfs(0, max_levels=8, stride=5, reps=1)
+    assert list(ex["level"]) == [0, 5, 10, 15, 20, 25, 30, 35]
+
+
+GRAPHS = [("rmat", lambda gsw: gsw.rmat(13, 16, seed=2)),
+          ("grid", lambda gsw: gsw.grid2d(60, 60, 0.1, 0.01, 3)),
+          ("ba", lambda gsw: gsw.barabasi_albert(4000, 4, 6)),
+          ("er", lambda gsw: gsw.erdos_renyi(5000, 3.0, 5))]
+
+
+@pytest.mark.parametrize("name,make", GRAPHS)
+def test_learned_policy_bit_exact(gsw, orc, name, make):
+    h = make(gsw)
+    g = gsw.Graph(h)
+    root = h.max_degree_vertex()
+    exp = orc.bfs(h.row_ptr, h.col, root)
+    seen = set()
+    for seed in range(6):  # random networks visit many classes and switches
+        w = np.random.default_rng(seed).normal(0, 0.5, NP).astype(np.float32)
+        gsw.selector_load(w)
+        assert np.array_equal(g.bfs(root, selector=1), exp), seed
+        for r in g.records():
+            seen.add((r["dir"], r["fmt"], r["lb"], r["fused"]))
+    assert len(seen) >= 3
+    # multi-root BFS under the learned policy
+    roots = np.array([root, 0, h.n - 1], np.int32)
+    ref = None
+    for r in np.unique(roots):
+        lv = orc.bfs(h.row_ptr, h.col, int(r))
+        lv = np.where(lv < 0, 1 << 30, lv)
+        ref = lv if ref is None else np.minimum(ref, lv)
+    ref = np.where(ref == 1 << 30, -1, ref)
+    assert np.array_equal(g.bfs_multi(roots, selector=1), ref)
+
+
+def test_learned_policy_errors(gsw):
+    h = gsw.rmat(10, 8, seed=3, weighted=True)
+    g = gsw.Graph(h)
+    with pytest.raises(gsw.GswError):
+        g.sssp(0, selector=1)
+    with pytest.raises(ValueError):
+        gsw.selector_load(np.zeros(10, np.float32))
+    bad = np.zeros(NP, np.float32)
+    assert gsw._lib.gsw_selector_load(bad.ctypes.data, 10) == 1  # GSW_E_INVALID
+    bad[7] = np.nan
+    assert gsw._lib.gsw_selector_load(bad.ctypes.data, NP) == 1
+
+
+def test_trained_model_file(gsw, orc):
+    from gswitch_amd import learn
+    if not os.path.exists(learn.MODEL_PATH):
+        pytest.skip("no trained selector committed")
+    w = learn.load_model()
+    gsw.selector_load(w)
+    h = gsw.rmat(14, 16, seed=11)
+    g = gsw.Graph(h)
+    root = h.max_degree_vertex()
+    assert np.array_equal(g.bfs(root, selector=1), orc.bfs(h.row_ptr, h.col, root))
