@@ -190,7 +190,7 @@ struct engine_t {
   config_t conf;
   stat_t stats;
   // learned-selector device block: [0] running inactive count, [1] running
-  // pull workload, [2..3] next-frontier degree sum / max, [4] ticket
+  // pull workload, [4] last-workgroup ticket of the inspector's inference
   long long *d_mlp_block = nullptr;
   engine_t() { executor.bind(inspector); }
   void release() {

@@ -579,9 +579,20 @@ double bfs_probe_level(gsw_graph *G, app_ctx_t<BFS> &c, int root, int cls,
     t0 = gsw_now_ms();
   }
   g.update_level();
+  en.executor.profile = true;
   en.executor.expand(c.as, g, c.f, en.stats, en.fets, en.conf);
   gsw::sync();
-  return gsw_now_ms() - t0;
+  const double wall = gsw_now_ms() - t0;
+  static const bool verbose = std::getenv("GSW_EXPLORE_VERBOSE") != nullptr;
+  if (verbose) {
+    en.executor.finalize_log(c.as);
+    const auto &r = en.executor.log.back();
+    fprintf(stderr, "probe level %d class %d: dir %d fmt %d lb %d fused %d "
+                    "frontier %lld edges %lld expand %.3f ms, level %.3f ms\n",
+            lvl, cls, (int)r.dir, (int)r.fmt, (int)r.lb, (int)r.fused,
+            (long long)r.frontier, (long long)r.edges, r.ms, wall);
+  }
+  return wall;
 }
 
 }  // namespace
@@ -903,7 +914,7 @@ gsw_status gsw_autotune_bfs(gsw_graph *G, const int32_t *roots, int32_t nroots,
         for (int rep = 0; rep < 3; ++rep) {
           bfs_captured_run(G, k, cd.lb, cd.small, cd.large, grids[gi],
                            2 * capture, 1 << 20);
-          t = MIN(t, elapsed(G));
+          t = std::min(t, elapsed(G));
         }
         if (t < best) {
           best = t;
@@ -1577,7 +1588,8 @@ gsw_status gsw_explore_bfs(gsw_graph *G, int32_t root, int32_t max_levels,
         for (int r = 0; r < reps; ++r) {
           H_ERR(hipMemcpyAsync(c.f.data.dg_wa, snap, sizeof(int32_t) * n,
                                hipMemcpyDeviceToDevice, st));
-          best = MIN(best, bfs_probe_level(G, c, root, cls, lvl));
+          // one probe per restore (a macro MIN would run the probe twice)
+          best = std::min(best, bfs_probe_level(G, c, root, cls, lvl));
         }
         if (ms_out) ms_out[(size_t)written * NOUT + cls] = (float)best;
       }

@@ -220,13 +220,9 @@ struct executor_t {
         // next level's class from the new frontier, read with its length
         gsw_mlp::sel_dev_t d = insp->mlp;
         d.level = (float)g.level;
-        const int32_t *out = (as.queue.mode == Normal)
-                                 ? as.queue.buf[as.queue.cursor ^ 1]
-                                 : as.queue.buf[0] + as.queue.offset_out;
-        hipLaunchKernelGGL(gsw::k_mlp_fused_select, dim3(gsw::MLP_SELECT_GRID),
-                           dim3(gsw::BLOCK), 0, gsw::stream(), d, out,
-                           as.queue.sizes + out_slot, g.dg_row_ptr,
-                           insp->mlp.running + 2, as.dg_ctrl + CTRL_MLP);
+        hipLaunchKernelGGL(gsw::k_mlp_fused_select, dim3(1), dim3(gsw::BLOCK), 0,
+                           gsw::stream(), d, as.queue.sizes + out_slot,
+                           as.dg_work, as.dg_ctrl + CTRL_MLP);
         H_ERR(hipGetLastError());
       }
       int nq = as.read_qsize_device(out_slot);

@@ -74,7 +74,9 @@ __global__ void __launch_bounds__(256)
 
 // CTRL_MLP: the learned selector's class for the next level (fused levels)
 enum : int { CTRL_HALT = 0, CTRL_OVERFLOW = 1, CTRL_FLAG = 2, CTRL_MLP = 3, CTRL_N = 8 };
-enum : int { WORK_EDGES = 0, WORK_VERTS = 1, WORK_N = 4 };
+// per-expand counters: edges / vertices expanded, and (fused levels under
+// the learned selector) out-degree sum / maximum of the next frontier
+enum : int { WORK_EDGES = 0, WORK_VERTS = 1, WORK_NEXT_DEG = 2, WORK_NEXT_MAX = 3, WORK_N = 4 };
 constexpr int WORK_ROWS = 16384;  // per-expand counter rows kept per traversal
 
 struct active_set_t {
@@ -223,6 +225,8 @@ struct active_set_t {
   // route the next expand's work counters to record row `i`
   void select_work_row(int i) {
     dg_work = dg_work_base + (int64_t)(i < WORK_ROWS ? i : WORK_ROWS - 1) * WORK_N;
+    // past the last row every expand reuses it: start it from zero
+    if (i >= WORK_ROWS - 1) gsw::fill_async<unsigned long long>(dg_work, 0ull, WORK_N);
   }
 
   void ensure_block_count(int nb) {

@@ -81,13 +81,49 @@ template <typename G, typename F, bool FUSED> struct push_body {
   }
 
   __device__ __forceinline__ void emit_q(bool q, int u) {
-    if constexpr (FUSED) as.queue.push(q, u);
+    if constexpr (FUSED) {
+      as.queue.push(q, u);
+      if (q && conf.policy_mlp) {
+        const unsigned long long d = (unsigned long long)(rows()[u + 1] - rows()[u]);
+        atomicAdd(&next_stats()[0], d);
+        atomicMax(&next_stats()[1], d);
+      }
+    }
+  }
+
+  // Out-degree sum / maximum of the vertices a fused expand appends (the
+  // next frontier), for the learned selector's next decision: accumulated
+  // per workgroup in LDS, one global atomic pair per workgroup.
+  __device__ __forceinline__ unsigned long long *next_stats() const {
+    __shared__ unsigned long long s_next[2];
+    return s_next;
+  }
+  __device__ __forceinline__ bool track_next() const {
+    if constexpr (FUSED) return conf.policy_mlp;
+    else return false;
+  }
+  __device__ __forceinline__ void next_begin() const {
+    if (track_next()) {
+      if (threadIdx.x < 2) next_stats()[threadIdx.x] = 0;
+      __syncthreads();
+    }
+  }
+  __device__ __forceinline__ void next_end() const {
+    if (track_next()) {
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        const unsigned long long *s = next_stats();
+        if (s[0]) atomicAdd(as.dg_work + WORK_NEXT_DEG, s[0]);
+        if (s[1]) atomicMax(as.dg_work + WORK_NEXT_MAX, s[1]);
+      }
+    }
   }
 };
 
 // ---------------------------------------------------------------- TM
 template <class B>
 __global__ void __launch_bounds__(BLOCK) k_push_tm(B body) {
+  body.next_begin();
   const int64_t N = body.as.get_size();
   const Status want = body.conf.want();
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -104,12 +140,14 @@ __global__ void __launch_bounds__(BLOCK) k_push_tm(B body) {
       body.emit_q(body(v, u, j), u);
     }
   }
+  body.next_end();
   add_work(body.as.dg_work, ne, nv);
 }
 
 // ---------------------------------------------------------------- WM
 template <class B>
 __global__ void __launch_bounds__(BLOCK) k_push_wm(B body) {
+  body.next_begin();
   __shared__ int s_v[BLOCK];
   __shared__ int s_scan[BLOCK];
   __shared__ int64_t s_start[BLOCK];
@@ -151,12 +189,14 @@ __global__ void __launch_bounds__(BLOCK) k_push_wm(B body) {
     }
     wave_sync();
   }
+  body.next_end();
   add_work(body.as.dg_work, ne, nv);
 }
 
 // ---------------------------------------------------------------- CM
 template <class B>
 __global__ void __launch_bounds__(BLOCK) k_push_cm(B body) {
+  body.next_begin();
   __shared__ int s_v[BLOCK];
   __shared__ int s_scan[BLOCK];
   __shared__ int64_t s_start[BLOCK];
@@ -196,6 +236,7 @@ __global__ void __launch_bounds__(BLOCK) k_push_cm(B body) {
     }
     __syncthreads();
   }
+  body.next_end();
   add_work(body.as.dg_work, ne, nv);
 }
 
@@ -275,6 +316,7 @@ __device__ __forceinline__ int64_t strict_next_owner(const int64_t *s_scan,
 template <class B>
 __global__ void __launch_bounds__(BLOCK)
     k_push_strict(B body, const int64_t *__restrict__ scan) {
+  body.next_begin();
   __shared__ int64_t s_scan[SCHUNK + 1];
   __shared__ int64_t s_start[SCHUNK + 1];
   __shared__ int s_v[SCHUNK + 1];
@@ -322,6 +364,7 @@ __global__ void __launch_bounds__(BLOCK)
     __syncthreads();
     k0 = nk;
   }
+  body.next_end();
   add_work(body.as.dg_work, ne, 0);
 }
 
@@ -352,6 +395,7 @@ __global__ void __launch_bounds__(BLOCK)
 
 template <class B>
 __global__ void __launch_bounds__(BLOCK) k_push_twc(B body) {
+  body.next_begin();
   const int lane = lane_id();
   const int32_t *sizes = body.as.bins.sizes;
   unsigned long long ne = 0, nv = 0;
@@ -411,6 +455,7 @@ __global__ void __launch_bounds__(BLOCK) k_push_twc(B body) {
       body.emit_q(body(v, u, j), u);
     }
   }
+  body.next_end();
   add_work(body.as.dg_work, ne, nv);
 }
 

@@ -70,29 +70,87 @@ __device__ __forceinline__ int wave_argmin(float y, uint32_t mask) {
   return id >= NOUT ? -1 : id;
 }
 
-__device__ __forceinline__ void mlp_counts_features(const sel_dev_t &d, float act,
-                                                    float inact, float pushw,
-                                                    float pullw, float maxa,
-                                                    mlp_wave_sh_t &s) {
-  counts_t c;
-  c.n = d.n;
-  c.m = d.m;
-  c.avg_deg = d.avg_deg;
-  c.std_deg = d.std_deg;
-  c.gini = d.gini;
-  c.level = d.level;
-  c.active = act;
-  c.inactive = inact;
-  c.push_wl = pushw;
-  c.pull_wl = pullw;
-  c.max_deg_active = maxa;
-  float x[NIN];
-  features(c, x);
-  const int j = lane_id();
+// One-sample inference by a whole 256-thread workgroup (the selection path
+// inside the inspector and after fused expands). Every thread issues all of
+// its weight loads before any arithmetic, so the three layers cost about one
+// memory round trip; hidden layers are split over the four wavefronts
+// (16 inputs each) and summed through LDS. Returns the argmin class over
+// `mask` in every thread of wavefront 0 (others: undefined).
+struct mlp_block_sh_t {
+  float x[16];
+  float h1[NH];
+  float h2[NH];
+  float part[BLOCK / WAVE][NH];
+};
+
+__device__ __forceinline__ int mlp_block_select(const float *w, uint32_t mask,
+                                                mlp_block_sh_t &s) {
+  const int t = threadIdx.x, j = t & (WAVE - 1), q = t >> LOG_WAVE;
+  constexpr int KQ = NH / (BLOCK / WAVE);  // 16 inputs per wavefront
+  float w1[NIN], w2[KQ], w3[KQ];
+  float b1 = 0.f, b2 = 0.f, b3 = 0.f;
+  if (q == 0) {
 #pragma unroll
-  for (int k = 0; k < NIN; ++k)
-    if (j == k) s.x[k] = x[k];
-  wave_lds_sync();
+    for (int k = 0; k < NIN; ++k) w1[k] = w[OW1 + k * NH + j];
+    b1 = w[OB1 + j];
+    b2 = w[OB2 + j];
+    if (j < NOUT) b3 = w[OB3 + j];
+  }
+#pragma unroll
+  for (int k = 0; k < KQ; ++k) w2[k] = w[OW2 + (q * KQ + k) * NH + j];
+#pragma unroll
+  for (int k = 0; k < KQ; ++k)
+    w3[k] = (j < NOUT) ? w[OW3 + (q * KQ + k) * NOUT + j] : 0.f;
+  __syncthreads();  // s.x written by the caller
+  if (q == 0) {
+    float a = b1;
+#pragma unroll
+    for (int k = 0; k < NIN; ++k) a += s.x[k] * w1[k];
+    s.h1[j] = a > 0.f ? a : 0.f;
+  }
+  __syncthreads();
+  float a = 0.f;
+#pragma unroll
+  for (int k = 0; k < KQ; ++k) a += s.h1[q * KQ + k] * w2[k];
+  s.part[q][j] = a;
+  __syncthreads();
+  if (q == 0) {
+    float h = b2;
+#pragma unroll
+    for (int r = 0; r < BLOCK / WAVE; ++r) h += s.part[r][j];
+    s.h2[j] = h > 0.f ? h : 0.f;
+  }
+  __syncthreads();
+  a = 0.f;
+#pragma unroll
+  for (int k = 0; k < KQ; ++k) a += s.h2[q * KQ + k] * w3[k];
+  s.part[q][j] = a;
+  __syncthreads();
+  float y = b3;
+#pragma unroll
+  for (int r = 0; r < BLOCK / WAVE; ++r) y += s.part[r][j];
+  return wave_argmin(y, mask);
+}
+
+__device__ __forceinline__ void block_features(const sel_dev_t &d, float act,
+                                               float inact, float pushw,
+                                               float pullw, float maxa,
+                                               mlp_block_sh_t &s) {
+  if (threadIdx.x == 0) {
+    counts_t c;
+    c.n = d.n;
+    c.m = d.m;
+    c.avg_deg = d.avg_deg;
+    c.std_deg = d.std_deg;
+    c.gini = d.gini;
+    c.level = d.level;
+    c.active = act;
+    c.inactive = inact;
+    c.push_wl = pushw;
+    c.pull_wl = pullw;
+    c.max_deg_active = maxa;
+    features(c, s.x);
+  }
 }
 
 __device__ __forceinline__ long long load_agent(const long long *p) {
@@ -105,7 +163,7 @@ __device__ __forceinline__ long long load_agent(const long long *p) {
 __device__ __forceinline__ void mlp_inspector_tail(const sel_dev_t &d,
                                                    long long *info) {
   __shared__ int s_last;
-  __shared__ mlp_wave_sh_t s;
+  __shared__ mlp_block_sh_t s;
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
@@ -113,17 +171,15 @@ __device__ __forceinline__ void mlp_inspector_tail(const sel_dev_t &d,
     s_last = (t == gridDim.x - 1);
   }
   __syncthreads();
-  if (!s_last || threadIdx.x >= WAVE) return;
+  if (!s_last) return;  // uniform per workgroup
   __threadfence();
-  const float act = (float)load_agent(info + 0);
-  const float inact = (float)load_agent(info + 1);
-  const float pushw = (float)load_agent(info + 2);
-  const float pullw = (float)load_agent(info + 3);
-  const float maxa = (float)load_agent(info + 4);
-  mlp_counts_features(d, act, inact, pushw, pullw, maxa, s);
-  const float y = mlp_wave_forward(d.w, s);
-  const int cls = wave_argmin(y, d.mask);
-  if (lane_id() == 0) {
+  if (threadIdx.x == 0) {
+    block_features(d, (float)load_agent(info + 0), (float)load_agent(info + 1),
+                   (float)load_agent(info + 2), (float)load_agent(info + 3),
+                   (float)load_agent(info + 4), s);
+  }
+  const int cls = mlp_block_select(d.w, d.mask, s);
+  if (threadIdx.x == 0) {
     info[6] = cls;
     d.running[0] = load_agent(info + 1);
     d.running[1] = load_agent(info + 3);
@@ -131,55 +187,29 @@ __device__ __forceinline__ void mlp_inspector_tail(const sel_dev_t &d,
   }
 }
 
-constexpr int MLP_SELECT_GRID = 256;
-
-// After a fused expand: degree sum / maximum of the next frontier (the
-// queue), then the last workgroup derives the next level's features (the
-// frontier leaves the inactive set) and writes the class to *out.
+// After a fused expand (one workgroup): the expand counted the next
+// frontier's out-degree sum and maximum (work[2], work[3]); the next level's
+// features follow (the frontier leaves the inactive set and its edges the
+// pull workload) and the chosen class goes to *out.
 __global__ void __launch_bounds__(BLOCK)
-    k_mlp_fused_select(sel_dev_t d, const int32_t *queue, const int32_t *qsize,
-                       const int64_t *row_ptr, long long *acc, int32_t *out) {
-  __shared__ long long sh[BLOCK / WAVE];
-  __shared__ int s_last;
-  __shared__ mlp_wave_sh_t s;
-  const int64_t nq = *qsize;
-  long long sum = 0, mx = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int u = queue[i];
-    const long long dg = row_ptr[u + 1] - row_ptr[u];
-    sum += dg;
-    mx = dg > mx ? dg : mx;
-  }
-  sum = block_sum(sum, sh);
-  mx = block_max(mx, sh);
-  if (threadIdx.x == 0) {
-    if (sum) atomicAdd((unsigned long long *)acc, (unsigned long long)sum);
-    if (mx) atomicMax(acc + 1, mx);
-    __threadfence();
-    const unsigned t = atomicAdd(d.ticket, 1u);
-    s_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!s_last || threadIdx.x >= WAVE) return;
-  __threadfence();
-  const long long deg_sum = load_agent(acc);
-  const long long deg_max = load_agent(acc + 1);
-  long long inact = load_agent(d.running) - nq;
-  long long pullw = load_agent(d.running + 1) - deg_sum;
+    k_mlp_fused_select(sel_dev_t d, const int32_t *qsize,
+                       const unsigned long long *work, int32_t *out) {
+  __shared__ mlp_block_sh_t s;
+  const long long nq = *qsize;
+  const long long deg_sum = (long long)work[2];
+  const long long deg_max = (long long)work[3];
+  long long inact = d.running[0] - nq;
+  long long pullw = d.running[1] - deg_sum;
   inact = inact < 0 ? 0 : inact;
   pullw = pullw < 0 ? 0 : pullw;
-  mlp_counts_features(d, (float)nq, (float)inact, (float)deg_sum, (float)pullw,
-                      (float)deg_max, s);
-  const float y = mlp_wave_forward(d.w, s);
-  const int cls = wave_argmin(y, d.mask);
-  if (lane_id() == 0) {
+  block_features(d, (float)nq, (float)inact, (float)deg_sum, (float)pullw,
+                 (float)deg_max, s);
+  const int cls = mlp_block_select(d.w, d.mask, s);
+  __syncthreads();  // every thread has read d.running
+  if (threadIdx.x == 0) {
     *out = cls;
     d.running[0] = inact;
     d.running[1] = pullw;
-    acc[0] = 0;
-    acc[1] = 0;
-    *d.ticket = 0u;
   }
 }
 
@@ -210,15 +240,13 @@ __global__ void __launch_bounds__(BLOCK)
 __global__ void __launch_bounds__(BLOCK)
     k_mlp_select_batch(const float *w, const float *x, int64_t B, uint32_t mask,
                        int32_t *cls) {
-  __shared__ mlp_wave_sh_t sh[BLOCK / WAVE];
-  mlp_wave_sh_t &s = sh[threadIdx.x >> LOG_WAVE];
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> LOG_WAVE;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> LOG_WAVE;
-  for (int64_t i = wave; i < B; i += nwaves) {
-    load_sample_x(x, i, s);
-    const int c = wave_argmin(mlp_wave_forward(w, s), mask);
-    if (lane_id() == 0) cls[i] = c;
-    wave_lds_sync();
+  // the workgroup-per-sample path of the inspector / fused selection
+  __shared__ mlp_block_sh_t s;
+  for (int64_t i = blockIdx.x; i < B; i += gridDim.x) {
+    __syncthreads();  // the previous sample's reads of s are done
+    if (threadIdx.x < NIN) s.x[threadIdx.x] = x[i * NIN + threadIdx.x];
+    const int c = mlp_block_select(w, mask, s);
+    if (threadIdx.x == 0) cls[i] = c;
   }
 }
 

@@ -15,12 +15,16 @@
 //         2 Push-Bitmap-Standalone      5 Pull-Bitmap-Standalone
 //
 // Features (all FP32, identical on host and device):
-//   0 log2(1+n)/24        4 Gini of the degrees    8 push workload / m
-//   1 log2(1+m)/30        5 log2(1+level)/8        9 pull workload / m
+//   0 log2(1+n)/24        4 Gini of the degrees    8 log2(1+push workload)/32
+//   1 log2(1+m)/32        5 log2(1+level)/8        9 log2(1+pull workload)/32
 //   2 log2(1+avg deg)/8   6 active / n            10 log2(1+push wl/active)/16
 //   3 log2(1+std/avg)/4   7 inactive / n          11 log2(1+max active deg)/24
-// where m counts directed CSR entries, "active" the frontier and "inactive"
-// the vertices a pull would visit.
+// where m counts directed CSR entries, "active" the frontier, "inactive"
+// the vertices a pull would visit, and the workloads are the edge counts a
+// push (out-edges of the frontier) or a pull (in-edges of the inactive
+// vertices) would scan. The workloads enter as absolute log sizes so that
+// the cost of a variant on a larger graph than any seen in training is an
+// extrapolation along a known axis.
 #pragma once
 #include <cmath>
 #include <cstdint>
@@ -97,15 +101,15 @@ __host__ __device__ inline float safe_div(float a, float b) {
 
 __host__ __device__ inline void features(const counts_t &c, float *x) {
   x[0] = log2f(1.f + c.n) / 24.f;
-  x[1] = log2f(1.f + c.m) / 30.f;
+  x[1] = log2f(1.f + c.m) / 32.f;
   x[2] = log2f(1.f + c.avg_deg) / 8.f;
   x[3] = log2f(1.f + safe_div(c.std_deg, c.avg_deg)) / 4.f;
   x[4] = c.gini;
   x[5] = log2f(1.f + fmaxf(c.level, 0.f)) / 8.f;
   x[6] = safe_div(c.active, c.n);
   x[7] = safe_div(c.inactive, c.n);
-  x[8] = safe_div(c.push_wl, c.m);
-  x[9] = safe_div(c.pull_wl, c.m);
+  x[8] = log2f(1.f + c.push_wl) / 32.f;
+  x[9] = log2f(1.f + c.pull_wl) / 32.f;
   x[10] = log2f(1.f + safe_div(c.push_wl, fmaxf(c.active, 1.f))) / 16.f;
   x[11] = log2f(1.f + c.max_deg_active) / 24.f;
 }

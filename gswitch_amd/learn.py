@@ -38,7 +38,7 @@ def _u(rng, lo, hi):
 
 
 def training_specs(count=300, seed=TRAIN_SEED):
-    """R-MAT scales 14-20 (a in [0.45, 0.65], b = c = (1 - a - 0.05) / 2),
+    """R-MAT scales 14-22 (a in [0.45, 0.65], b = c = (1 - a - 0.05) / 2),
     Barabasi-Albert (m in [2, 16]), Erdos-Renyi and perturbed 2-D grids."""
     rng = np.random.default_rng(seed)
     kinds = ["rmat"] * 120 + ["ba"] * 60 + ["er"] * 60 + ["grid"] * 60
@@ -49,12 +49,12 @@ def training_specs(count=300, seed=TRAIN_SEED):
                  root="max" if i % 2 == 0 else "random")
         if k == "rmat":
             a = _u(rng, 0.45, 0.65)
-            s.update(scale=int(rng.integers(14, 21)), ef=int(rng.choice([8, 16])),
+            s.update(scale=int(rng.integers(14, 23)), ef=int(rng.choice([8, 16])),
                      a=a, b=(1 - a - 0.05) / 2)
         elif k == "ba":
-            s.update(n=int(2 ** rng.integers(14, 19)), m=int(rng.integers(2, 17)))
+            s.update(n=int(2 ** rng.integers(14, 21)), m=int(rng.integers(2, 17)))
         elif k == "er":
-            s.update(n=int(2 ** rng.integers(14, 20)), d=_u(rng, 2, 32))
+            s.update(n=int(2 ** rng.integers(14, 22)), d=_u(rng, 2, 32))
         else:
             s.update(rows=int(rng.integers(64, 1025)), cols=int(rng.integers(64, 1025)),
                      p=_u(rng, 0.0, 0.3), lr=_u(rng, 0.0, 0.02))

@@ -219,7 +219,10 @@ gsw_status gsw_tree_eval(const char *name, const double *fv, int32_t nfv,
 /* Install the weights used when gsw_run_opts.selector == 1. */
 gsw_status gsw_selector_load(const float *w, int32_t nparam);
 /* The 12 inputs from counts[11] = {n, m, avg deg, std deg, Gini, level,
- * active, inactive, push workload, pull workload, max active degree}. */
+ * active, inactive, push workload, pull workload, max active degree}:
+ * log2(1+n)/24, log2(1+m)/32, log2(1+avg)/8, log2(1+std/avg)/4, Gini,
+ * log2(1+level)/8, active/n, inactive/n, log2(1+push)/32, log2(1+pull)/32,
+ * log2(1+push/max(active,1))/16, log2(1+max active degree)/24. */
 gsw_status gsw_mlp_features(const double *counts, float *x);
 /* Device forward pass: y[B][30]. */
 gsw_status gsw_mlp_forward(const float *w, const float *x, int64_t B, float *y);

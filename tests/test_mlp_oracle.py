@@ -71,10 +71,11 @@ def features_np(c):
     f = np.float32
     sd = lambda a, b: f(a) / f(b) if b > 0 else f(0)
     return np.array([
-        np.log2(f(1) + f(n)) / f(24), np.log2(f(1) + f(m)) / f(30),
+        np.log2(f(1) + f(n)) / f(24), np.log2(f(1) + f(m)) / f(32),
         np.log2(f(1) + f(avg)) / f(8), np.log2(f(1) + sd(std, avg)) / f(4),
         f(gini), np.log2(f(1) + max(f(level), f(0))) / f(8),
-        sd(act, n), sd(inact, n), sd(pw, m), sd(lw, m),
+        sd(act, n), sd(inact, n),
+        np.log2(f(1) + f(pw)) / f(32), np.log2(f(1) + f(lw)) / f(32),
         np.log2(f(1) + sd(pw, max(f(act), f(1)))) / f(16),
         np.log2(f(1) + f(md)) / f(24)], np.float32)
 
