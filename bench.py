@@ -106,6 +106,149 @@ def cpu_baseline(h, root, sssp_edges, pr_edges):
     }, dist, pr5
 
 
+def bfs_level_bytes(rec, next_active, n):
+    """Minimal bytes of one BFS level (no weights): push reads the queue
+    and offsets of the frontier, a column and the target state per edge,
+    writes state + queue entry per discovered vertex; pull reads the two
+    frontier bitmaps, offsets + state per target, a column and a frontier
+    word per inspected in-edge, and writes found vertices."""
+    if rec["dir"] == "Push":
+        return 20.0 * rec["frontier"] + 8.0 * rec["edges"] + 8.0 * next_active
+    return n / 4.0 + 20.0 * rec["verts"] + 8.0 * rec["edges"] + 4.0 * next_active
+
+
+def graph500_edges(h, levels):
+    return int(np.diff(h.row_ptr)[levels >= 0].sum()) // 2
+
+
+def bfs_single(gsw, h, root, reps, **kw):
+    """Best-of-reps device time of one BFS, its records and byte model."""
+    g = gsw.Graph(h)
+    ref = g.bfs(root, **kw)
+    best, recs = None, None
+    for _ in range(reps):
+        g.bfs(root, out=False, **kw)
+        if best is None or g.last_stats.device_ms < best:
+            best, recs = g.last_stats.device_ms, g.records()
+    g.close()
+    nbytes = 0.0
+    for i, r in enumerate(recs):
+        nxt = recs[i + 1]["active"] if i + 1 < len(recs) else 0
+        nbytes += bfs_level_bytes(r, nxt, h.n)
+    return ref, best, recs, nbytes
+
+
+def extended(gsw, args):
+    """Single-query BFS at R-MAT 24, 64-source BFS (R-MAT 22 seed 4), a
+    sample of the 20,000-graph ER batch (seed 5), and the learned selector
+    against the rule trees and a fixed strategy. Every comparison checks
+    that the results agree."""
+    out = {}
+    fixed = "Push-Queue-WM-Standalone"
+    # --- single query, R-MAT 24
+    t = time.perf_counter()
+    h = gsw.rmat(args.bfs_scale, 16, 0.57, 0.19, 0.19, seed=1)
+    root = h.max_degree_vertex()
+    log("bfs graph: scale %d n=%d m=%d built in %.1fs" % (args.bfs_scale, h.n, h.m,
+                                                          time.perf_counter() - t))
+    lv, ms, recs, nbytes = bfs_single(gsw, h, root, 5)
+    e = graph500_edges(h, lv)
+    _, ms_fixed, _, _ = bfs_single(gsw, h, root, 3, configs=fixed)
+    single = {"scale": args.bfs_scale, "n": h.n, "m_undirected": h.m // 2,
+              "levels": len(recs), "ms": round(ms, 3),
+              "gteps": round(e / (ms * 1e-3) / 1e9, 2),
+              "algorithmic_GBps": round(nbytes / (ms * 1e-3) / 1e9, 1),
+              "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+              "expand_ms": round(sum(r["ms"] for r in recs), 3),
+              "fixed_strategy_ms": round(ms_fixed, 3),
+              "variants": ["%s-%s-%s-%s" % (r["dir"], r["fmt"], r["lb"],
+                                            "F" if r["fused"] else "S") for r in recs]}
+    from gswitch_amd import learn
+    learned = None
+    if os.path.exists(learn.MODEL_PATH):
+        gsw.selector_load(learn.load_model())
+        lv2, ms_l, recs_l, _ = bfs_single(gsw, h, root, 5, selector=1)
+        single["learned_ms"] = round(ms_l, 3)
+        single["learned_bit_exact"] = bool(np.array_equal(lv2, lv))
+        learned = ms_l
+    out["bfs_single"] = single
+    del h
+    # --- 64-source BFS, R-MAT 22 seed 4
+    h = gsw.rmat(22, 16, 0.57, 0.19, 0.19, seed=4)
+    g = gsw.Graph(h)
+    src = np.argsort(-np.diff(h.row_ptr), kind="stable")[:64].astype(np.int32)
+    masks, _ = g.msbfs64(src)
+    t_ms = []
+    for _ in range(3):
+        g.msbfs64(src, out=False)
+        t_ms.append(g.last_stats.device_ms)
+    ms64 = min(t_ms)
+    t_seq = 0.0
+    edges64 = 0
+    ref = np.zeros(h.n, np.uint64)
+    for k, s in enumerate(src):
+        lv = g.bfs(int(s), configs=fixed)
+        t_seq += g.last_stats.device_ms
+        edges64 += graph500_edges(h, lv)
+        ref[lv >= 0] |= np.uint64(1) << np.uint64(k)
+    out["msbfs64"] = {"ms": round(ms64, 3), "gteps": round(edges64 / (ms64 * 1e-3) / 1e9, 1),
+                      "fixed_64_single_ms": round(t_seq, 3),
+                      "speedup": round(t_seq / ms64, 2),
+                      "bit_exact_vs_64_bfs": bool(np.array_equal(masks, ref))}
+    g.close()
+    del h
+    # --- ER batch sample (instances 0 .. count-1 of the seed-5 batch)
+    t = time.perf_counter()
+    h, off = gsw.erdos_renyi_batch(args.batch, 1000, 200000, 2.0, 32.0, seed=5)
+    log("batch: %d graphs n=%d m=%d in %.1fs" % (args.batch, h.n, h.m,
+                                                 time.perf_counter() - t))
+    g = gsw.Graph(h)
+    roots = off[:-1].astype(np.int32)
+    tune = g.autotune_bfs(roots, capture=8)
+    kw = dict(configs="Push-x-%s-x" % tune["lb"], capture=8, grid=tune["grid"],
+              twc_small=tune["twc_small"], twc_large=tune["twc_large"])
+    lvb = g.bfs_multi(roots, **kw)
+    bms = []
+    for _ in range(3):
+        g.bfs_multi(roots, out=False, profile=False, **kw)
+        t_b = g.last_stats.device_ms
+        g.cc(out=False, profile=False)
+        bms.append(t_b + g.last_stats.device_ms)
+    ms_batch = min(bms)
+    labb = g.cc()
+    e_bfs = graph500_edges(h, lvb)
+    e_cc = h.m // 2
+    # per-graph launch loop with a fixed strategy (upload outside the timing)
+    t_loop = 0.0
+    same = True
+    for i in range(args.batch):
+        lo, hi = int(off[i]), int(off[i + 1])
+        rp = h.row_ptr[lo:hi + 1] - h.row_ptr[lo]
+        col = h.col[h.row_ptr[lo]:h.row_ptr[hi]] - lo
+        gi = gsw.Graph(row_ptr=rp, col=col)
+        lvi = gi.bfs(0, configs=fixed, profile=False)
+        t_loop += gi.last_stats.device_ms
+        labi = gi.cc(profile=False)
+        t_loop += gi.last_stats.device_ms
+        same &= bool(np.array_equal(lvi, lvb[lo:hi])) and \
+            bool(np.array_equal(labi, labb[lo:hi] - lo))
+        gi.close()
+    out["er_batch"] = {"graphs": args.batch, "n": h.n, "m_undirected": h.m // 2,
+                       "autotune": tune, "batched_ms": round(ms_batch, 3),
+                       "batched_gteps": round((e_bfs + e_cc) / (ms_batch * 1e-3) / 1e9, 2),
+                       "per_graph_loop_ms": round(t_loop, 3),
+                       "speedup": round(t_loop / ms_batch, 2),
+                       "bit_exact_vs_per_graph": same}
+    g.close()
+    # --- aggregate: batched + learned vs plain per-query fixed strategy
+    t_new = ms_batch + ms64 + (learned if learned is not None else ms)
+    t_old = t_loop + t_seq + ms_fixed
+    out["batched_learned_vs_fixed"] = {
+        "speedup": round(t_old / t_new, 2),
+        "note": "same work (ER batch BFS+CC, 64 BFS, one R-MAT BFS); device ms"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +259,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--configs-sssp", default=None)
     ap.add_argument("--configs-pr", default=None)
+    ap.add_argument("--quick", action="store_true",
+                    help="headline workload only (no BFS / batch / selector lines)")
+    ap.add_argument("--bfs-scale", type=int, default=24)
+    ap.add_argument("--batch", type=int, default=1000,
+                    help="ER batch sample size (instances 0..batch-1 of 20,000)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU "
@@ -212,6 +360,10 @@ def main():
             "pagerank5_max_rel_err": float(np.max(np.abs(gpr5 - cpr5) / np.abs(cpr5))),
         }
 
+    g.close()
+    if not args.quick:
+        detail.update(extended(gsw, args))
+
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -235,7 +387,6 @@ def main():
         "detail": detail,
     }
     print(json.dumps(out), flush=True)
-    g.close()
 
 
 if __name__ == "__main__":

@@ -20,6 +20,7 @@ using GW = device_graph_t<CSR, uint32_t>;
 using GC = device_graph_t<COO, Empty>;
 
 struct BFS : Functor<VC, int, Empty, Empty> {
+  static constexpr bool level_sync = true;
   __device__ Status filter(int v, GU g) {
     const int lvl = *wa_of(v);
     if (lvl == g.get_level()) return Active;
@@ -125,6 +126,7 @@ struct bc_bwd_t {
 };
 
 struct BCF : Functor<VC, bc_fwd_t, Empty, Empty> {
+  static constexpr bool level_sync = true;
   __device__ Status filter(int v, GU g) {
     const int lvl = wa_of(v)->level;
     if (lvl == g.get_level()) return Active;

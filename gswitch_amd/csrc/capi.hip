@@ -559,8 +559,13 @@ void bfs_level_setup(gsw_graph *G, app_ctx_t<BFS> &c, int root, int cls) {
   c.as.reset();
 }
 
-// Run BFS level `lvl` from the current levels array with class `cls`;
-// returns the wall time in ms of what that class costs per level.
+// Cost of running BFS level `lvl` (from the current levels array) with
+// class `cls`, in wall-clock ms, attributed the way the level loop incurs
+// it: a standalone level pays its filter, its expand and the inspection of
+// the next level (nothing extra when the expand already inspected it, as a
+// level-synchronous TM pull does); a fused level pays its expand and the
+// read of the next queue length. This level's own inspection belongs to the
+// level before it and is not timed.
 double bfs_probe_level(gsw_graph *G, app_ctx_t<BFS> &c, int root, int cls,
                        int lvl) {
   bfs_level_setup(G, c, root, cls);
@@ -568,10 +573,10 @@ double bfs_probe_level(gsw_graph *G, app_ctx_t<BFS> &c, int root, int cls,
   GU g = G->dg;
   g.level = lvl;
   const bool fused = gsw_mlp::variant_of_class(cls).fused;
-  gsw::sync();
-  double t0 = gsw_now_ms();
   en.inspector.inspect(c.as, g, c.f, en.stats, en.fets, en.conf);
   if (c.as.finish(g, c.f, en.conf)) return 0.0;
+  gsw::sync();
+  double t0 = gsw_now_ms();
   en.selector.select(en.stats, en.fets, en.conf);
   en.executor.filter(c.as, g, c.f, en.stats, en.fets, en.conf);
   if (fused) {
@@ -581,6 +586,7 @@ double bfs_probe_level(gsw_graph *G, app_ctx_t<BFS> &c, int root, int cls,
   g.update_level();
   en.executor.profile = true;
   en.executor.expand(c.as, g, c.f, en.stats, en.fets, en.conf);
+  if (!fused) en.inspector.inspect(c.as, g, c.f, en.stats, en.fets, en.conf);
   gsw::sync();
   const double wall = gsw_now_ms() - t0;
   static const bool verbose = std::getenv("GSW_EXPLORE_VERBOSE") != nullptr;

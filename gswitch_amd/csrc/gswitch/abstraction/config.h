@@ -66,6 +66,9 @@ struct config_t {
   // (mlp_class, written by the inspector / the fused-level selection)
   bool policy_mlp = false;
   int mlp_class = -1;
+  // set by an expand that already inspected the next level (TM pull of a
+  // level-synchronous functor): the inspector then only reads the counters
+  bool conf_pre_inspected = false;
   ins_t force;
 
   void reset() {
@@ -73,6 +76,7 @@ struct config_t {
     conf_switch_to_fusion = false;
     conf_switch_to_standalone = false;
     conf_fuse_inspect = false;
+    conf_pre_inspected = false;
     conf_idle = 0;
   }
 

@@ -202,7 +202,7 @@ struct executor_t {
       e0 = ev_next;
       H_ERR(hipEventRecord(next_event(), gsw::stream()));
     }
-    dispatch(as, g, f, conf);
+    dispatch(as, g, f, conf, stat);
     if (profile) {
       e1 = ev_next;
       H_ERR(hipEventRecord(next_event(), gsw::stream()));
@@ -258,7 +258,7 @@ struct executor_t {
   }
 
   template <typename G, typename F>
-  void dispatch(active_set_t &as, G &g, F &f, config_t &conf);
+  void dispatch(active_set_t &as, G &g, F &f, config_t &conf, stat_t &stat);
 };
 
 // grid for a frontier of N items processed `per` items per workgroup
@@ -268,7 +268,8 @@ inline int expand_grid(int64_t N, int per) {
 }
 
 template <typename G, typename F>
-void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf) {
+void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf,
+                          stat_t &stat) {
   using namespace gsw;
   hipStream_t st = stream();
   if constexpr (std::is_same<G, device_graph_t<COO, typename G::edge_t>>::value) {
@@ -350,6 +351,27 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf) {
       Bd body{as, g, f, conf};
       switch (lb) {
         case TM:
+          if constexpr (is_level_sync<F>::value) {
+            if (as.fmt == Bitmap && !conf.conf_fromall && !conf.conf_toall) {
+              // this level's pull also inspects the next one
+              stat.clean();
+              gsw_mlp::sel_dev_t d;
+              if (conf.policy_mlp && insp && insp->mlp.w) {
+                d = insp->mlp;
+                d.level = (float)g.level;
+              }
+              hipLaunchKernelGGL(k_pull_tm_inspect<Bd>,
+                                 dim3(grid_for((g.nvertexs + 63) / 64 * 64)),
+                                 dim3(BLOCK), 0, st, body,
+                                 as.bitmap.visited.dg_bits, stat.dg_runtime_info, d);
+              H_ERR(hipGetLastError());
+              // the next frontier becomes `active`; the old active words
+              // (vertices reached earlier) become the `visited` marks
+              std::swap(as.bitmap.active.dg_bits, as.bitmap.visited.dg_bits);
+              conf.conf_pre_inspected = true;
+              break;
+            }
+          }
           hipLaunchKernelGGL(k_pull_tm<Bd>, dim3(expand_grid(N, BLOCK)),
                              dim3(BLOCK), 0, st, body);
           break;

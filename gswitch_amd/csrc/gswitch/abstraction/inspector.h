@@ -58,6 +58,8 @@ struct inspector_t {
     f.data.window.adjust(fets.active_vertex, fets.cur_avg_deg_active,
                          conf.conf_fusion);
     bool need = !conf.conf_fuse_inspect;
+    const bool pre = conf.conf_pre_inspected;
+    conf.conf_pre_inspected = false;
     init_fets(fets, (double)g.nvertexs, (double)g.nedges);
     if (conf.conf_first_round) {
       // the initial frontier is known exactly (root or everything), also
@@ -68,6 +70,14 @@ struct inspector_t {
     if (need && conf.conf_first_round) {
       set_first_fets(as, fets, conf, (double)g.nvertexs, (double)g.nedges);
       if (conf.policy_mlp && mlp.w) mlp_host_select(fets, conf, g.level);
+      need = false;
+    }
+    if (need && pre) {
+      // the previous expand already classified this level (bitmaps written,
+      // counters and, under the learned selector, the class in stat)
+      stat.collect();
+      if (conf.policy_mlp && mlp.w) conf.mlp_class = (int)stat.h_runtime_info[6];
+      set_fets(as, stat, fets, conf, (double)g.nvertexs, (double)g.nedges);
       need = false;
     }
     if (need) {

@@ -80,6 +80,16 @@ template <typename WA, typename RA> struct vdata_t {
 
 template <Centric C, typename WA, typename RA, typename E> struct Functor {};
 
+// A functor is level-synchronous when a vertex updated during a level is
+// exactly the next level's frontier and every other vertex keeps its status
+// (BFS-like traversals). Such a functor may declare
+//   static constexpr bool level_sync = true;
+// and the engine then folds the next level's inspection into TM pulls.
+template <typename F, typename = void> struct is_level_sync : std::false_type {};
+template <typename F>
+struct is_level_sync<F, std::void_t<decltype(F::level_sync)>>
+    : std::integral_constant<bool, F::level_sync> {};
+
 template <typename WA, typename RA, typename E>
 struct Functor<EC, WA, RA, E> {
   typedef WA wa_t;

@@ -16,6 +16,7 @@
 //          the visited bitmap
 #pragma once
 #include "expand_push.h"
+#include "inspect.h"
 
 namespace gsw {
 
@@ -107,6 +108,82 @@ __global__ void __launch_bounds__(BLOCK) k_pull_tm(B body) {
     if (any) *body.f.wa_of(v) = acc;
   }
   add_work(body.as.dg_work, ne, nv);
+}
+
+// TM pull over a bitmap frontier with the next level's inspection folded
+// in, for level-synchronous functors (a vertex updated in this level is
+// exactly the next level's frontier; the others keep their status): one
+// 64-vertex word per wavefront step, the ballot of updated vertices is the
+// next `active` word (written to next_active, swapped in by the host), they
+// leave the inactive set in place (each word is touched by its own
+// wavefront only), and the inspector's six counters of the next level are
+// reduced here, so that level needs no inspector pass.
+template <class B>
+__global__ void __launch_bounds__(BLOCK)
+    k_pull_tm_inspect(B body, uint64_t *next_active, long long *info,
+                      gsw_mlp::sel_dev_t mlp) {
+  using WA = typename B::WA;
+  __shared__ long long sh[BLOCK / WAVE];
+  const int lane = lane_id();
+  const int64_t n = body.g.nvertexs;
+  const int64_t nw = (n + 63) >> 6;
+  const bool prune = body.conf.conf_pruning;
+  uint64_t *inact_bits = body.as.bitmap.inactive.dg_bits;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> LOG_WAVE;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> LOG_WAVE;
+  unsigned long long ne = 0, nv = 0;
+  long long act = 0, inact = 0, pushw = 0, pullw = 0, maxa = 0, maxi = 0;
+  for (int64_t w = wave; w < nw; w += nwaves) {
+    const int64_t v = (w << 6) + lane;
+    const uint64_t word = inact_bits[w];  // bit set: not inactive
+    bool found = false;
+    if (v < n && !((word >> lane) & 1ull)) {
+      ++nv;
+      const int64_t s = body.rows()[v], e = body.rows()[v + 1];
+      WA acc;
+      int64_t j = s;
+      for (; j < e; ++j) {
+        const int u = body.cols()[j];
+        if (!body.source_ok(u)) continue;
+        const WA m = body.msg(u, j);
+        if (!found) {
+          acc = *body.f.wa_of(v);
+          found = true;
+        }
+        body.f.comp(&acc, m, body.g);
+        if (prune) {
+          ++j;
+          break;
+        }
+      }
+      ne += j - s;
+      if (found) {
+        *body.f.wa_of(v) = acc;
+        const long long od = body.g.get_out_degree((int)v);
+        ++act;
+        pushw += od;
+        maxa = od > maxa ? od : maxa;
+      } else {
+        const long long id = e - s;
+        ++inact;
+        pullw += id;
+        maxi = id > maxi ? id : maxi;
+      }
+    }
+    const uint64_t fm = ballot(found);
+    if (lane == 0) {
+      next_active[w] = fm;
+      if (fm) inact_bits[w] = word | fm;
+    }
+  }
+  flush_sum(act, sh, info + 0);
+  flush_sum(inact, sh, info + 1);
+  flush_sum(pushw, sh, info + 2);
+  flush_sum(pullw, sh, info + 3);
+  flush_max(maxa, sh, info + 4);
+  flush_max(maxi, sh, info + 5);
+  add_work(body.as.dg_work, ne, nv);
+  if (mlp.w) mlp_inspector_tail(mlp, info);
 }
 
 // ---------------------------------------------------------------- WM

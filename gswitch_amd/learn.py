@@ -113,6 +113,28 @@ def level_stride(depth, max_levels):
     return max(1, (depth + max_levels - 1) // max_levels)
 
 
+def class_of_record(r):
+    """Variant class of one expand record of the engine."""
+    fmt = r["fmt"]
+    if r["dir"] == "Push" and fmt == "Queue" and r["sorted"] and not r["fused"]:
+        fmt = "SortedQueue"
+    lb = r["lb"] if r["lb"] in gsw.MLP_LBS else "STRICT"
+    if r["fused"]:
+        name = "Push-Queue-%s-Fused" % lb
+    elif r["dir"] == "Pull":
+        name = "Pull-%s-%s-Standalone" % ("Bitmap" if fmt == "Bitmap" else "Queue", lb)
+    else:
+        name = "Push-%s-%s-Standalone" % (fmt, lb)
+    return gsw.MLP_CLASSES.index(name)
+
+
+def rule_trajectory(g, root):
+    """The rule selector's actual class per BFS level (fused levels
+    included), keyed by the level of the expanded frontier."""
+    g.bfs(root, out=False, profile=True)
+    return {r["level"] - 1: class_of_record(r) for r in g.records()}
+
+
 def collect(specs, reps=2, max_levels=32, log=print):
     rows = dict(features=[], ms=[], rule=[], level=[], graph=[])
     t0 = time.time()
@@ -121,8 +143,12 @@ def collect(specs, reps=2, max_levels=32, log=print):
         g = gsw.Graph(h)
         root = pick_root(spec, h)
         depth = int(g.bfs(root).max()) + 1
+        traj = rule_trajectory(g, root)
         ex = g.explore_bfs(root, max_levels=max_levels,
                            stride=level_stride(depth, max_levels), reps=reps)
+        # the rule's class in its own run (fused state included) where known
+        ex["rule"] = np.array([traj.get(int(l), int(c)) for l, c in
+                               zip(ex["level"], ex["rule"])], np.int32)
         g.close()
         for key in ["features", "ms", "rule", "level"]:
             rows[key].append(ex[key])

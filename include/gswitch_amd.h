@@ -243,9 +243,10 @@ gsw_status gsw_mlp_train(const float *x, const float *t, const float *mask,
 /* Exhaustive per-level timing for BFS from `root`: for each probed level
  * (every `stride`-th, at most max_levels) the 12 features, the wall time in
  * ms of each of the 30 classes run from a snapshot of the level state
- * (best of `reps`; standalone classes include the inspector and filter,
- * fused ones the expand and its queue-length read), and the class the rule
- * selector would pick. *n_out = levels written. */
+ * (best of `reps`; a standalone class pays its filter, expand and the
+ * inspection of the next level, a fused one its expand and the read of the
+ * next queue length), and the class the rule selector would pick.
+ * *n_out = levels written. */
 gsw_status gsw_explore_bfs(gsw_graph *g, int32_t root, int32_t max_levels,
                            int32_t stride, int32_t reps, float *feat_out,
                            float *ms_out, int32_t *rule_out, int32_t *level_out,
