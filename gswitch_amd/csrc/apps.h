@@ -191,15 +191,10 @@ struct engine_t {
   feature_t fets;
   config_t conf;
   stat_t stats;
-  // learned-selector device block: [0] running inactive count, [1] running
-  // pull workload, [4] last-workgroup ticket of the inspector's inference
-  long long *d_mlp_block = nullptr;
   engine_t() { executor.bind(inspector); }
   void release() {
     stats.release();
     executor.release();
-    if (d_mlp_block) hipFree(d_mlp_block);
-    d_mlp_block = nullptr;
   }
 };
 
@@ -219,10 +214,6 @@ struct run_opts_t {
 inline void enable_learned(engine_t &en, const float *w_dev, const float *w_host,
                            const graph_attr_t &attr, double n, double m,
                            bool fromall, bool toall) {
-  if (!en.d_mlp_block) {
-    H_ERR(hipMalloc((void **)&en.d_mlp_block, sizeof(long long) * 8));
-    H_ERR(hipMemset(en.d_mlp_block, 0, sizeof(long long) * 8));
-  }
   gsw_mlp::sel_dev_t &d = en.inspector.mlp;
   d.w = w_dev;
   d.n = (float)n;
@@ -231,9 +222,6 @@ inline void enable_learned(engine_t &en, const float *w_dev, const float *w_host
   d.std_deg = (float)attr.std_deg;
   d.gini = (float)attr.GI;
   d.mask = gsw_mlp::valid_mask(fromall, toall);
-  d.running = en.d_mlp_block;
-  d.ticket = (unsigned int *)(en.d_mlp_block + 4);
-  d.out = nullptr;
   en.inspector.mlp_host_w = w_host;
   en.conf.policy_mlp = true;
   en.conf.mlp_class = -1;

@@ -66,6 +66,10 @@ struct config_t {
   // (mlp_class, written by the inspector / the fused-level selection)
   bool policy_mlp = false;
   int mlp_class = -1;
+  // fused levels under the learned selector: the frontier size when the
+  // network last chose a fused class (re-inspection waits for growth)
+  bool mlp_entered_fused = false;
+  long long mlp_fused_ref = 0;
   // set by an expand that already inspected the next level (TM pull of a
   // level-synchronous functor): the inspector then only reads the counters
   bool conf_pre_inspected = false;

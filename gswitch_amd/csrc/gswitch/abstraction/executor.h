@@ -214,26 +214,23 @@ struct executor_t {
       // the next queue's length: grid size for the next level, convergence
       const int out_slot = (as.queue.mode == Normal) ? (as.queue.cursor ^ 1)
                                                      : (as.queue.cursor + 1);
-      constexpr bool csr = std::is_same<G, device_graph_t<CSR, typename G::edge_t>>::value;
-      const bool learned = csr && conf.policy_mlp && insp && insp->mlp.w;
-      if constexpr (csr) if (learned) {
-        // next level's class from the new frontier, read with its length
-        gsw_mlp::sel_dev_t d = insp->mlp;
-        d.level = (float)g.level;
-        hipLaunchKernelGGL(gsw::k_mlp_fused_select, dim3(1), dim3(gsw::BLOCK), 0,
-                           gsw::stream(), d, as.queue.sizes + out_slot,
-                           as.dg_work, as.dg_ctrl + CTRL_MLP);
-        H_ERR(hipGetLastError());
-      }
+      const bool learned = conf.policy_mlp && insp && insp->mlp.w;
       int nq = as.read_qsize_device(out_slot);
       if (insp) insp->known_active = nq;
       as.check_overflow_cached();
       if (nq == 0) as.halt_host();
+      // Learned selector on fused levels: a small next frontier stays fused
+      // with the current class; a large one (where a pull or another format
+      // may win) is handed to the next level's inspector, whose fused
+      // inference picks the variant.
       if (learned) {
-        conf.mlp_class = as.h_ctrl[CTRL_MLP];
-        if (nq > 0 && conf.mlp_class >= 0 &&
-            !gsw_mlp::variant_of_class(conf.mlp_class).fused) {
-          conf.conf_fuse_inspect = false;  // the next level inspects
+        if (conf.mlp_entered_fused) {  // the inspector chose fused for this level
+          conf.mlp_fused_ref = rec.frontier;
+          conf.mlp_entered_fused = false;
+        }
+        if (nq > 0 && (double)nq * gsw_mlp::INSPECT_FRACTION >= (double)as.size &&
+            (double)nq >= gsw_mlp::INSPECT_GROWTH * (double)conf.mlp_fused_ref) {
+          conf.conf_fuse_inspect = false;
           as.swap_queues();
         }
       }
@@ -363,7 +360,10 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf,
               hipLaunchKernelGGL(k_pull_tm_inspect<Bd>,
                                  dim3(grid_for((g.nvertexs + 63) / 64 * 64)),
                                  dim3(BLOCK), 0, st, body,
-                                 as.bitmap.visited.dg_bits, stat.dg_runtime_info, d);
+                                 as.bitmap.visited.dg_bits, stat.dg_runtime_info);
+              if (d.w)
+                hipLaunchKernelGGL(k_mlp_select_counts, dim3(1), dim3(BLOCK), 0, st,
+                                   d, stat.dg_runtime_info);
               H_ERR(hipGetLastError());
               // the next frontier becomes `active`; the old active words
               // (vertices reached earlier) become the `visited` marks

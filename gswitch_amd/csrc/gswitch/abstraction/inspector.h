@@ -46,9 +46,6 @@ struct inspector_t {
     gsw_mlp::features(c, x);
     gsw_mlp::forward_host(mlp_host_w, x, y);
     conf.mlp_class = gsw_mlp::argmin_valid(y, mlp.mask);
-    // running inactive count / pull workload for fused-level features
-    gsw::set_one_async<long long>(mlp.running, (long long)fets.unactive_vertex);
-    gsw::set_one_async<long long>(mlp.running + 1, (long long)fets.pull_workload);
   }
 
   template <typename E, typename F>
@@ -90,7 +87,10 @@ struct inspector_t {
       hipLaunchKernelGGL((gsw::k_inspect_vc<device_graph_t<CSR, E>, F>),
                          dim3(gsw::grid_for((g.nvertexs + 63) / 64 * 64)),
                          dim3(gsw::BLOCK), 0, gsw::stream(), as, g, f,
-                         stat.dg_runtime_info, d);
+                         stat.dg_runtime_info);
+      if (d.w)
+        hipLaunchKernelGGL(gsw::k_mlp_select_counts, dim3(1), dim3(gsw::BLOCK), 0,
+                           gsw::stream(), d, stat.dg_runtime_info);
       H_ERR(hipGetLastError());
       stat.collect();
       if (d.w) conf.mlp_class = (int)stat.h_runtime_info[6];

@@ -171,7 +171,10 @@ struct selector_t {
   static void select_learned(config_t &conf) {
     const gsw_mlp::variant_t v = gsw_mlp::variant_of_class(conf.mlp_class);
     if (v.fused || conf.conf_fuse_inspect) {
-      if (!conf.conf_fuse_inspect) conf.conf_switch_to_fusion = true;
+      if (!conf.conf_fuse_inspect) {
+        conf.conf_switch_to_fusion = true;
+        conf.mlp_entered_fused = true;
+      }
       conf.conf_fuse_inspect = true;
       conf.conf_dir = Push;
       conf.conf_asfmt = Queue;

@@ -72,11 +72,8 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// CTRL_MLP: the learned selector's class for the next level (fused levels)
-enum : int { CTRL_HALT = 0, CTRL_OVERFLOW = 1, CTRL_FLAG = 2, CTRL_MLP = 3, CTRL_N = 8 };
-// per-expand counters: edges / vertices expanded, and (fused levels under
-// the learned selector) out-degree sum / maximum of the next frontier
-enum : int { WORK_EDGES = 0, WORK_VERTS = 1, WORK_NEXT_DEG = 2, WORK_NEXT_MAX = 3, WORK_N = 4 };
+enum : int { CTRL_HALT = 0, CTRL_OVERFLOW = 1, CTRL_FLAG = 2, CTRL_N = 8 };
+enum : int { WORK_EDGES = 0, WORK_VERTS = 1, WORK_N = 4 };
 constexpr int WORK_ROWS = 16384;  // per-expand counter rows kept per traversal
 
 struct active_set_t {
@@ -199,7 +196,7 @@ struct active_set_t {
   // (one stream sync); raises on a queue overflow.
   int read_qsize_device(int slot) {
     int32_t *h = h_ctrl;
-    H_ERR(hipMemcpyAsync(h, dg_ctrl, sizeof(int32_t) * (CTRL_MLP + 1),
+    H_ERR(hipMemcpyAsync(h, dg_ctrl, sizeof(int32_t) * CTRL_FLAG,
                          hipMemcpyDeviceToHost, gsw::stream()));
     H_ERR(hipMemcpyAsync(h + CTRL_FLAG, queue.sizes + slot, sizeof(int32_t),
                          hipMemcpyDeviceToHost, gsw::stream()));

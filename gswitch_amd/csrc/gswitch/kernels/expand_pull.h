@@ -120,8 +120,7 @@ __global__ void __launch_bounds__(BLOCK) k_pull_tm(B body) {
 // reduced here, so that level needs no inspector pass.
 template <class B>
 __global__ void __launch_bounds__(BLOCK)
-    k_pull_tm_inspect(B body, uint64_t *next_active, long long *info,
-                      gsw_mlp::sel_dev_t mlp) {
+    k_pull_tm_inspect(B body, uint64_t *next_active, long long *info) {
   using WA = typename B::WA;
   __shared__ long long sh[BLOCK / WAVE];
   const int lane = lane_id();
@@ -183,7 +182,6 @@ __global__ void __launch_bounds__(BLOCK)
   flush_max(maxa, sh, info + 4);
   flush_max(maxi, sh, info + 5);
   add_work(body.as.dg_work, ne, nv);
-  if (mlp.w) mlp_inspector_tail(mlp, info);
 }
 
 // ---------------------------------------------------------------- WM

@@ -48,11 +48,6 @@ __device__ __forceinline__ void add_work(unsigned long long *work,
   }
 }
 
-// per-lane statistics of the vertices a fused expand appends
-struct next_acc_t {
-  unsigned long long sum = 0, max = 0;
-};
-
 template <typename G, typename F, bool FUSED> struct push_body {
   static constexpr bool fused = FUSED;
   active_set_t as;
@@ -85,39 +80,14 @@ template <typename G, typename F, bool FUSED> struct push_body {
     }
   }
 
-  __device__ __forceinline__ void emit_q(bool q, int u, next_acc_t &nx) {
-    if constexpr (FUSED) {
-      as.queue.push(q, u);
-      if (q && conf.policy_mlp) {
-        const unsigned long long d = (unsigned long long)(rows()[u + 1] - rows()[u]);
-        nx.sum += d;
-        nx.max = d > nx.max ? d : nx.max;
-      }
-    }
-  }
-
-  // Out-degree sum / maximum of the vertices a fused expand appends (the
-  // next frontier), for the learned selector's next decision: kept per lane
-  // and flushed with one atomic pair per wavefront when the kernel ends
-  // (every wavefront reaches that point).
-  __device__ __forceinline__ void next_flush(const next_acc_t &nx) const {
-    if constexpr (FUSED) {
-      if (conf.policy_mlp) {
-        const unsigned long long s = wave_sum_any(nx.sum);
-        const unsigned long long m = wave_max_any(nx.max);
-        if (lane_id() == 0 && s) {
-          atomicAdd(as.dg_work + WORK_NEXT_DEG, s);
-          atomicMax(as.dg_work + WORK_NEXT_MAX, m);
-        }
-      }
-    }
+  __device__ __forceinline__ void emit_q(bool q, int u) {
+    if constexpr (FUSED) as.queue.push(q, u);
   }
 };
 
 // ---------------------------------------------------------------- TM
 template <class B>
 __global__ void __launch_bounds__(BLOCK) k_push_tm(B body) {
-  next_acc_t nx;
   const int64_t N = body.as.get_size();
   const Status want = body.conf.want();
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -131,17 +101,15 @@ __global__ void __launch_bounds__(BLOCK) k_push_tm(B body) {
     ++nv;
     for (int64_t j = s; j < e; ++j) {
       const int u = body.cols()[j];
-      body.emit_q(body(v, u, j), u, nx);
+      body.emit_q(body(v, u, j), u);
     }
   }
-  body.next_flush(nx);
   add_work(body.as.dg_work, ne, nv);
 }
 
 // ---------------------------------------------------------------- WM
 template <class B>
 __global__ void __launch_bounds__(BLOCK) k_push_wm(B body) {
-  next_acc_t nx;
   __shared__ int s_v[BLOCK];
   __shared__ int s_scan[BLOCK];
   __shared__ int64_t s_start[BLOCK];
@@ -179,18 +147,16 @@ __global__ void __launch_bounds__(BLOCK) k_push_wm(B body) {
         u = body.cols()[e];
         q = body(s_v[off + o], u, e);
       }
-      body.emit_q(q, u, nx);
+      body.emit_q(q, u);
     }
     wave_sync();
   }
-  body.next_flush(nx);
   add_work(body.as.dg_work, ne, nv);
 }
 
 // ---------------------------------------------------------------- CM
 template <class B>
 __global__ void __launch_bounds__(BLOCK) k_push_cm(B body) {
-  next_acc_t nx;
   __shared__ int s_v[BLOCK];
   __shared__ int s_scan[BLOCK];
   __shared__ int64_t s_start[BLOCK];
@@ -226,11 +192,10 @@ __global__ void __launch_bounds__(BLOCK) k_push_cm(B body) {
         u = body.cols()[e];
         q = body(s_v[o], u, e);
       }
-      body.emit_q(q, u, nx);
+      body.emit_q(q, u);
     }
     __syncthreads();
   }
-  body.next_flush(nx);
   add_work(body.as.dg_work, ne, nv);
 }
 
@@ -310,7 +275,6 @@ __device__ __forceinline__ int64_t strict_next_owner(const int64_t *s_scan,
 template <class B>
 __global__ void __launch_bounds__(BLOCK)
     k_push_strict(B body, const int64_t *__restrict__ scan) {
-  next_acc_t nx;
   __shared__ int64_t s_scan[SCHUNK + 1];
   __shared__ int64_t s_start[SCHUNK + 1];
   __shared__ int s_v[SCHUNK + 1];
@@ -352,13 +316,12 @@ __global__ void __launch_bounds__(BLOCK)
         q = body(v[j], u[j], e[j]);
         ++ne;
       }
-      body.emit_q(q, u[j], nx);
+      body.emit_q(q, u[j]);
     }
     const int64_t nk = strict_next_owner(s_scan, k0, c + SCHUNK);
     __syncthreads();
     k0 = nk;
   }
-  body.next_flush(nx);
   add_work(body.as.dg_work, ne, 0);
 }
 
@@ -389,7 +352,6 @@ __global__ void __launch_bounds__(BLOCK)
 
 template <class B>
 __global__ void __launch_bounds__(BLOCK) k_push_twc(B body) {
-  next_acc_t nx;
   const int lane = lane_id();
   const int32_t *sizes = body.as.bins.sizes;
   unsigned long long ne = 0, nv = 0;
@@ -410,7 +372,7 @@ __global__ void __launch_bounds__(BLOCK) k_push_twc(B body) {
         u = body.cols()[j];
         q = body(v, u, j);
       }
-      body.emit_q(q, u, nx);
+      body.emit_q(q, u);
     }
   }
   // medium: one wavefront per vertex
@@ -432,7 +394,7 @@ __global__ void __launch_bounds__(BLOCK) k_push_twc(B body) {
         u = body.cols()[j];
         q = body(v, u, j);
       }
-      body.emit_q(q, u, nx);
+      body.emit_q(q, u);
     }
   }
   // small: one thread per vertex
@@ -446,10 +408,9 @@ __global__ void __launch_bounds__(BLOCK) k_push_twc(B body) {
     ++nv;
     for (int64_t j = s; j < e; ++j) {
       const int u = body.cols()[j];
-      body.emit_q(body(v, u, j), u, nx);
+      body.emit_q(body(v, u, j), u);
     }
   }
-  body.next_flush(nx);
   add_work(body.as.dg_work, ne, nv);
 }
 

@@ -27,8 +27,7 @@ __device__ __forceinline__ void flush_max(T x, T *sh, long long *dst) {
 
 template <typename G, typename F>
 __global__ void __launch_bounds__(BLOCK)
-    k_inspect_vc(active_set_t as, G g, F f, long long *info,
-                 gsw_mlp::sel_dev_t mlp) {
+    k_inspect_vc(active_set_t as, G g, F f, long long *info) {
   __shared__ long long sh[BLOCK / WAVE];
   const int lane = lane_id();
   const int64_t n = g.nvertexs;
@@ -68,9 +67,6 @@ __global__ void __launch_bounds__(BLOCK)
   flush_sum(pullw, sh, info + 3);
   flush_max(maxa, sh, info + 4);
   flush_max(maxi, sh, info + 5);
-  // learned selector: the last workgroup turns the counters into features
-  // and picks the next variant (info[6]) in the same launch
-  if (mlp.w) mlp_inspector_tail(mlp, info);
 }
 
 // Edge-centric inspector: an edge stays active while filter(v0, v1) says so.

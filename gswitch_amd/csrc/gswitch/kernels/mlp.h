@@ -1,6 +1,6 @@
 // HIP kernels of the learned selector (model/mlp.h): wave-per-sample
-// forward, argmin selection (used inside the inspector and after fused
-// expands), and training: per-wave partial gradients of the masked squared
+// forward, workgroup-per-sample argmin selection (the path inside the
+// inspector), and training: per-wave partial gradients of the masked squared
 // error, a fixed-order reduction (so training is bit-reproducible) and Adam.
 //
 // Inside a wavefront lane j owns hidden unit j (and output j < 30): every
@@ -153,64 +153,19 @@ __device__ __forceinline__ void block_features(const sel_dev_t &d, float act,
   }
 }
 
-__device__ __forceinline__ long long load_agent(const long long *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Called by every workgroup of the inspector after its counters are flushed:
-// the last workgroup to finish reads the six counters and writes the chosen
-// class to info[6] (and the running inactive / pull workload).
-__device__ __forceinline__ void mlp_inspector_tail(const sel_dev_t &d,
-                                                   long long *info) {
-  __shared__ int s_last;
-  __shared__ mlp_block_sh_t s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned t = atomicAdd(d.ticket, 1u);
-    s_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!s_last) return;  // uniform per workgroup
-  __threadfence();
-  if (threadIdx.x == 0) {
-    block_features(d, (float)load_agent(info + 0), (float)load_agent(info + 1),
-                   (float)load_agent(info + 2), (float)load_agent(info + 3),
-                   (float)load_agent(info + 4), s);
-  }
-  const int cls = mlp_block_select(d.w, d.mask, s);
-  if (threadIdx.x == 0) {
-    info[6] = cls;
-    d.running[0] = load_agent(info + 1);
-    d.running[1] = load_agent(info + 3);
-    *d.ticket = 0u;
-  }
-}
-
-// After a fused expand (one workgroup): the expand counted the next
-// frontier's out-degree sum and maximum (work[2], work[3]); the next level's
-// features follow (the frontier leaves the inactive set and its edges the
-// pull workload) and the chosen class goes to *out.
+// One workgroup, launched on the engine's stream right after the pass that
+// filled the six inspector counters (the inspector, or a TM pull that folded
+// the inspection in): turns them into the 12 features and writes the class
+// the network picks to info[6], which the host reads with the counters.
+// A kernel of its own rather than a tail of the pass, so the pass keeps its
+// register budget (occupancy) when the learned selector is on.
 __global__ void __launch_bounds__(BLOCK)
-    k_mlp_fused_select(sel_dev_t d, const int32_t *qsize,
-                       const unsigned long long *work, int32_t *out) {
+    k_mlp_select_counts(sel_dev_t d, long long *info) {
   __shared__ mlp_block_sh_t s;
-  const long long nq = *qsize;
-  const long long deg_sum = (long long)work[2];
-  const long long deg_max = (long long)work[3];
-  long long inact = d.running[0] - nq;
-  long long pullw = d.running[1] - deg_sum;
-  inact = inact < 0 ? 0 : inact;
-  pullw = pullw < 0 ? 0 : pullw;
-  block_features(d, (float)nq, (float)inact, (float)deg_sum, (float)pullw,
-                 (float)deg_max, s);
+  block_features(d, (float)info[0], (float)info[1], (float)info[2],
+                 (float)info[3], (float)info[4], s);
   const int cls = mlp_block_select(d.w, d.mask, s);
-  __syncthreads();  // every thread has read d.running
-  if (threadIdx.x == 0) {
-    *out = cls;
-    d.running[0] = inact;
-    d.running[1] = pullw;
-  }
+  if (threadIdx.x == 0) info[6] = cls;
 }
 
 // ---- batch paths (training, tests) -----------------------------------------

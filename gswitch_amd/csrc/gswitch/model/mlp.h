@@ -144,16 +144,21 @@ inline int argmin_valid(const float *y, uint32_t mask) {
   return best;
 }
 
-// Device-side selector state handed to the inspector and the fused-level
-// selection kernel (by value). `w` = 0 disables inference.
+// Device-side selector state handed to the selection kernel (by value).
+// `w` = 0 disables inference.
 struct sel_dev_t {
   const float *w = nullptr;
   float n = 0, m = 0, avg_deg = 0, std_deg = 0, gini = 0;
   float level = 0;
   uint32_t mask = ALL_CLASSES;
-  int32_t *out = nullptr;           // chosen class
-  unsigned int *ticket = nullptr;   // last-workgroup detection (reset by it)
-  long long *running = nullptr;     // [0] inactive, [1] pull workload
 };
+
+// On a fused level the next frontier's length is known without a pass. A
+// frontier of at least n / INSPECT_FRACTION vertices that has also grown
+// INSPECT_GROWTH-fold since the network last chose a fused class is handed
+// to the next level's inspector (and its inference); otherwise the level
+// stays fused with the current class.
+constexpr double INSPECT_FRACTION = 64.0;
+constexpr double INSPECT_GROWTH = 2.0;
 
 }  // namespace gsw_mlp
