@@ -80,13 +80,19 @@ struct executor_t {
       log[i].ms = ms;
     }
     if (!log.empty() && as.dg_work_base) {
-      std::vector<unsigned long long> w(log.size() * WORK_N);
+      const int64_t rows = std::min<int64_t>((int64_t)log.size(), WORK_ROWS);
+      hipLaunchKernelGGL(k_work_fold, dim3((rows * WORK_N + 255) / 256), dim3(256),
+                         0, gsw::stream(), as.dg_work_base, rows);
+      H_ERR(hipGetLastError());
+      gsw::sync();
+      std::vector<unsigned long long> w(rows * WORK_N);
       H_ERR(hipMemcpy(w.data(), as.dg_work_base,
                       sizeof(unsigned long long) * w.size(),
                       hipMemcpyDeviceToHost));
       for (size_t i = 0; i < log.size(); ++i) {
-        log[i].edges = (long long)w[i * WORK_N + WORK_EDGES];
-        log[i].verts = (long long)w[i * WORK_N + WORK_VERTS];
+        const size_t r = std::min<size_t>(i, (size_t)rows - 1);
+        log[i].edges = (long long)w[r * WORK_N + WORK_EDGES];
+        log[i].verts = (long long)w[r * WORK_N + WORK_VERTS];
       }
     }
   }
@@ -351,7 +357,6 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf,
           if constexpr (is_level_sync<F>::value) {
             if (as.fmt == Bitmap && !conf.conf_fromall && !conf.conf_toall) {
               // this level's pull also inspects the next one
-              stat.clean();
               gsw_mlp::sel_dev_t d;
               if (conf.policy_mlp && insp && insp->mlp.w) {
                 d = insp->mlp;
@@ -360,10 +365,9 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf,
               hipLaunchKernelGGL(k_pull_tm_inspect<Bd>,
                                  dim3(grid_for((g.nvertexs + 63) / 64 * 64)),
                                  dim3(BLOCK), 0, st, body,
-                                 as.bitmap.visited.dg_bits, stat.dg_runtime_info);
-              if (d.w)
-                hipLaunchKernelGGL(k_mlp_select_counts, dim3(1), dim3(BLOCK), 0, st,
-                                   d, stat.dg_runtime_info);
+                                 as.bitmap.visited.dg_bits, stat.slots());
+              hipLaunchKernelGGL(k_stat_finish, dim3(1), dim3(BLOCK), 0, st, d,
+                                 stat.slots(), stat.dg_runtime_info);
               H_ERR(hipGetLastError());
               // the next frontier becomes `active`; the old active words
               // (vertices reached earlier) become the `visited` marks

@@ -78,7 +78,7 @@ struct inspector_t {
       need = false;
     }
     if (need) {
-      stat.clean();
+      // the partial slots are zero between passes: k_stat_finish clears them
       gsw_mlp::sel_dev_t d;
       if (conf.policy_mlp && mlp.w) {
         d = mlp;
@@ -87,10 +87,9 @@ struct inspector_t {
       hipLaunchKernelGGL((gsw::k_inspect_vc<device_graph_t<CSR, E>, F>),
                          dim3(gsw::grid_for((g.nvertexs + 63) / 64 * 64)),
                          dim3(gsw::BLOCK), 0, gsw::stream(), as, g, f,
-                         stat.dg_runtime_info);
-      if (d.w)
-        hipLaunchKernelGGL(gsw::k_mlp_select_counts, dim3(1), dim3(gsw::BLOCK), 0,
-                           gsw::stream(), d, stat.dg_runtime_info);
+                         stat.slots());
+      hipLaunchKernelGGL(gsw::k_stat_finish, dim3(1), dim3(gsw::BLOCK), 0,
+                         gsw::stream(), d, stat.slots(), stat.dg_runtime_info);
       H_ERR(hipGetLastError());
       stat.collect();
       if (d.w) conf.mlp_class = (int)stat.h_runtime_info[6];
@@ -112,10 +111,12 @@ struct inspector_t {
       known_inactive = 0;
       return;
     }
-    stat.clean();
     hipLaunchKernelGGL((gsw::k_inspect_ec<device_graph_t<COO, E>, F>),
                        dim3(gsw::grid_for((g.nedges + 63) / 64 * 64)),
                        dim3(gsw::BLOCK), 0, gsw::stream(), as, g, f,
+                       stat.slots());
+    hipLaunchKernelGGL(gsw::k_stat_finish, dim3(1), dim3(gsw::BLOCK), 0,
+                       gsw::stream(), gsw_mlp::sel_dev_t{}, stat.slots(),
                        stat.dg_runtime_info);
     H_ERR(hipGetLastError());
     stat.collect();

@@ -5,11 +5,23 @@
 // vertex. Counters are int64 (the reference's int32 overflows past 2^31
 // scanned edges) and the maxima are true maxima (the reference block-sums
 // them, inspector.cuh:101-109).
+//
+// Device layout: `dg_runtime_info[0..8)` holds the reduced counters the host
+// copies back; after it come STAT_SLOTS partial slots, one 128-byte line
+// each, that the counting passes add into (workgroup b uses slot
+// b % STAT_SLOTS). Atomics from every workgroup onto one line serialise at
+// about 12 ns each on MI355X (2048 workgroups x 6 counters = 150 us), while
+// the same atomics spread over 64 lines cost nothing measurable. The
+// one-workgroup finish kernel (k_stat_finish) sums the slots into the
+// reduced counters and clears them, so the slots are zero between passes.
 #pragma once
 #include "../utils/runtime.h"
 
+constexpr int STAT_SLOTS = 64;   // partial-counter slots
+constexpr int STAT_STRIDE = 16;  // long longs per slot (one 128-byte line)
+
 struct stat_t {
-  enum { N = 8 };
+  enum { N = 8, SLOT_BASE = 16, TOTAL = SLOT_BASE + STAT_SLOTS * STAT_STRIDE };
   long long *dg_runtime_info = nullptr;
   long long *h_runtime_info = nullptr;
   int size = 6;
@@ -23,7 +35,7 @@ struct stat_t {
   void build(int _size = 6) {
     size = _size;
     if (!dg_runtime_info) {
-      H_ERR(hipMalloc((void **)&dg_runtime_info, sizeof(long long) * N));
+      H_ERR(hipMalloc((void **)&dg_runtime_info, sizeof(long long) * TOTAL));
       H_ERR(hipHostMalloc((void **)&h_runtime_info, sizeof(long long) * N));
     }
     for (int i = 0; i < N; ++i) h_runtime_info[i] = 0;
@@ -41,6 +53,8 @@ struct stat_t {
     avg_filter_time = avg_expand_time = 0;
     level = 0;
   }
+  // partial slots the counting passes add into
+  long long *slots() const { return dg_runtime_info + SLOT_BASE; }
   // enqueue the device->host copy; the caller synchronises
   void collect_async(int num = N) {
     H_ERR(hipMemcpyAsync(h_runtime_info, dg_runtime_info,
@@ -51,7 +65,8 @@ struct stat_t {
     collect_async(num);
     gsw::sync();
   }
-  void clean() { gsw::fill_async<long long>(dg_runtime_info, 0, N); }
+  // zero the reduced counters and every slot
+  void clean() { gsw::fill_async<long long>(dg_runtime_info, 0, TOTAL); }
 
   long long active_num() const { return h_runtime_info[0]; }
   long long unactive_num() const { return h_runtime_info[1]; }

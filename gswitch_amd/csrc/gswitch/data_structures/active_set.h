@@ -73,8 +73,37 @@ __global__ void __launch_bounds__(256)
 }
 
 enum : int { CTRL_HALT = 0, CTRL_OVERFLOW = 1, CTRL_FLAG = 2, CTRL_N = 8 };
+// Per-expand work counters (edges scanned, vertices expanded): one record
+// row of WORK_N counters per expand, WORK_ROWS rows per traversal, kept as
+// WORK_SLOTS partial copies laid out slot-major ([slot][row][counter]) so
+// that the slots of one row sit on different cache lines. k_work_fold sums
+// the slots into slot 0 when the records are read.
 enum : int { WORK_EDGES = 0, WORK_VERTS = 1, WORK_N = 4 };
 constexpr int WORK_ROWS = 16384;  // per-expand counter rows kept per traversal
+constexpr int WORK_SLOTS = 16;
+constexpr int64_t WORK_SLOT_STRIDE = (int64_t)WORK_ROWS * WORK_N;
+constexpr int64_t WORK_TOTAL = WORK_SLOT_STRIDE * WORK_SLOTS;
+
+// zero record row `row` in every slot
+__global__ void __launch_bounds__(64)
+    k_work_clear_row(unsigned long long *row) {
+  const int s = threadIdx.x / WORK_N, k = threadIdx.x % WORK_N;
+  if (s < WORK_SLOTS) row[s * WORK_SLOT_STRIDE + k] = 0;
+}
+
+// slot 0 of rows [0, nrows) := the sum over the slots; the other slots are
+// cleared, so folding again changes nothing
+__global__ void __launch_bounds__(256)
+    k_work_fold(unsigned long long *base, int64_t nrows) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows * WORK_N) return;
+  unsigned long long t = base[i];
+  for (int s = 1; s < WORK_SLOTS; ++s) {
+    t += base[s * WORK_SLOT_STRIDE + i];
+    base[s * WORK_SLOT_STRIDE + i] = 0;
+  }
+  base[i] = t;
+}
 
 struct active_set_t {
   int64_t size = 0;  // number of vertices (VC) or edges (EC)
@@ -86,7 +115,7 @@ struct active_set_t {
   bins_t bins;
   int32_t *dg_ctrl = nullptr;                 // device control block
   unsigned long long *dg_work = nullptr;      // counters of the current expand
-  unsigned long long *dg_work_base = nullptr; // [WORK_ROWS][WORK_N]
+  unsigned long long *dg_work_base = nullptr; // [WORK_SLOTS][WORK_ROWS][WORK_N]
   int32_t *dg_block_count = nullptr;          // sorted-filter scratch [1025]
   int32_t *h_ctrl = nullptr;                  // pinned mirror of dg_ctrl
   bool converged = false;                     // host view of the halt flag
@@ -95,8 +124,7 @@ struct active_set_t {
     size = n;
     int64_t bytes = 0;
     H_ERR(hipMalloc((void **)&dg_ctrl, sizeof(int32_t) * CTRL_N));
-    H_ERR(hipMalloc((void **)&dg_work_base,
-                    sizeof(unsigned long long) * WORK_N * WORK_ROWS));
+    H_ERR(hipMalloc((void **)&dg_work_base, sizeof(unsigned long long) * WORK_TOTAL));
     dg_work = dg_work_base;
     H_ERR(hipHostMalloc((void **)&h_ctrl, sizeof(int32_t) * CTRL_N));
     bytes += queue.build(n, qmode, dg_ctrl + CTRL_OVERFLOW);
@@ -140,8 +168,7 @@ struct active_set_t {
     gsw::fill_async<uint64_t>(bitmap.active.dg_bits, 0, bitmap.active.n_words + 1);
     gsw::fill_async<uint64_t>(bitmap.inactive.dg_bits, 0, bitmap.inactive.n_words + 1);
     gsw::fill_async(dg_ctrl, 0, CTRL_N);
-    gsw::fill_async<unsigned long long>(dg_work_base, 0ull,
-                                        (int64_t)WORK_N * WORK_ROWS);
+    gsw::fill_async<unsigned long long>(dg_work_base, 0ull, WORK_TOTAL);
     dg_work = dg_work_base;
     converged = false;
   }
@@ -223,7 +250,11 @@ struct active_set_t {
   void select_work_row(int i) {
     dg_work = dg_work_base + (int64_t)(i < WORK_ROWS ? i : WORK_ROWS - 1) * WORK_N;
     // past the last row every expand reuses it: start it from zero
-    if (i >= WORK_ROWS - 1) gsw::fill_async<unsigned long long>(dg_work, 0ull, WORK_N);
+    if (i >= WORK_ROWS - 1) {
+      hipLaunchKernelGGL(k_work_clear_row, dim3(1), dim3(64), 0, gsw::stream(),
+                         dg_work);
+      H_ERR(hipGetLastError());
+    }
   }
 
   void ensure_block_count(int nb) {

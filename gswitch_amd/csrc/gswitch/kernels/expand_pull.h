@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(BLOCK) k_pull_tm(B body) {
 // reduced here, so that level needs no inspector pass.
 template <class B>
 __global__ void __launch_bounds__(BLOCK)
-    k_pull_tm_inspect(B body, uint64_t *next_active, long long *info) {
+    k_pull_tm_inspect(B body, uint64_t *next_active, long long *slots) {
   using WA = typename B::WA;
   __shared__ long long sh[BLOCK / WAVE];
   const int lane = lane_id();
@@ -175,12 +175,13 @@ __global__ void __launch_bounds__(BLOCK)
       if (fm) inact_bits[w] = word | fm;
     }
   }
-  flush_sum(act, sh, info + 0);
-  flush_sum(inact, sh, info + 1);
-  flush_sum(pushw, sh, info + 2);
-  flush_sum(pullw, sh, info + 3);
-  flush_max(maxa, sh, info + 4);
-  flush_max(maxi, sh, info + 5);
+  long long *slot = stat_slot(slots);
+  flush_sum(act, sh, slot + 0);
+  flush_sum(inact, sh, slot + 1);
+  flush_sum(pushw, sh, slot + 2);
+  flush_sum(pullw, sh, slot + 3);
+  flush_max(maxa, sh, slot + 4);
+  flush_max(maxi, sh, slot + 5);
   add_work(body.as.dg_work, ne, nv);
 }
 

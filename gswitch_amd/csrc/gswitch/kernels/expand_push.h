@@ -36,15 +36,34 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// per-wave edge / vertex work counters (one atomic per wavefront)
+// Edge / vertex work counters of an expand: a workgroup reduction, then one
+// atomic per counter into the workgroup's slot of the current record row
+// (slot blockIdx.x % WORK_SLOTS; the slots of a row lie WORK_ROWS rows
+// apart, so each has a cache line of its own: atomics from every workgroup
+// onto one line serialise on MI355X). Every thread of the workgroup calls
+// it; a workgroup that leaves early leaves as a whole and counts nothing.
 __device__ __forceinline__ void add_work(unsigned long long *work,
                                          unsigned long long edges,
                                          unsigned long long verts) {
+  __shared__ unsigned long long sh[2][BLOCK / WAVE];
   edges = wave_sum_any(edges);
   verts = wave_sum_any(verts);
   if (lane_id() == 0) {
-    if (edges) atomicAdd(work + WORK_EDGES, edges);
-    if (verts) atomicAdd(work + WORK_VERTS, verts);
+    sh[0][threadIdx.x >> LOG_WAVE] = edges;
+    sh[1][threadIdx.x >> LOG_WAVE] = verts;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long e = 0, v = 0;
+    const int nw = (blockDim.x + WAVE - 1) >> LOG_WAVE;
+    for (int i = 0; i < nw; ++i) {
+      e += sh[0][i];
+      v += sh[1][i];
+    }
+    unsigned long long *slot =
+        work + (int64_t)(blockIdx.x & (WORK_SLOTS - 1)) * WORK_SLOT_STRIDE;
+    if (e) atomicAdd(slot + WORK_EDGES, e);
+    if (v) atomicAdd(slot + WORK_VERTS, v);
   }
 }
 

@@ -14,20 +14,26 @@
 
 namespace gsw {
 
-template <typename T>
-__device__ __forceinline__ void flush_sum(T x, T *sh, long long *dst) {
-  T r = block_sum(x, sh);
-  if (threadIdx.x == 0 && r) atomicAdd((unsigned long long *)dst, (unsigned long long)r);
+// Workgroup reduction, then one atomic into this workgroup's partial slot
+// (slot blockIdx.x % STAT_SLOTS, a 128-byte line of its own); k_stat_finish
+// reduces the slots.
+__device__ __forceinline__ long long *stat_slot(long long *slots) {
+  return slots + (int64_t)(blockIdx.x & (STAT_SLOTS - 1)) * STAT_STRIDE;
 }
 template <typename T>
-__device__ __forceinline__ void flush_max(T x, T *sh, long long *dst) {
+__device__ __forceinline__ void flush_sum(T x, T *sh, long long *slot) {
+  T r = block_sum(x, sh);
+  if (threadIdx.x == 0 && r) atomicAdd((unsigned long long *)slot, (unsigned long long)r);
+}
+template <typename T>
+__device__ __forceinline__ void flush_max(T x, T *sh, long long *slot) {
   T r = block_max(x, sh);
-  if (threadIdx.x == 0 && r) atomicMax((long long *)dst, (long long)r);
+  if (threadIdx.x == 0 && r) atomicMax((long long *)slot, (long long)r);
 }
 
 template <typename G, typename F>
 __global__ void __launch_bounds__(BLOCK)
-    k_inspect_vc(active_set_t as, G g, F f, long long *info) {
+    k_inspect_vc(active_set_t as, G g, F f, long long *slots) {
   __shared__ long long sh[BLOCK / WAVE];
   const int lane = lane_id();
   const int64_t n = g.nvertexs;
@@ -61,18 +67,19 @@ __global__ void __launch_bounds__(BLOCK)
       as.bitmap.inactive.dg_bits[w] = nm;
     }
   }
-  flush_sum(act, sh, info + 0);
-  flush_sum(inact, sh, info + 1);
-  flush_sum(pushw, sh, info + 2);
-  flush_sum(pullw, sh, info + 3);
-  flush_max(maxa, sh, info + 4);
-  flush_max(maxi, sh, info + 5);
+  long long *slot = stat_slot(slots);
+  flush_sum(act, sh, slot + 0);
+  flush_sum(inact, sh, slot + 1);
+  flush_sum(pushw, sh, slot + 2);
+  flush_sum(pullw, sh, slot + 3);
+  flush_max(maxa, sh, slot + 4);
+  flush_max(maxi, sh, slot + 5);
 }
 
 // Edge-centric inspector: an edge stays active while filter(v0, v1) says so.
 template <typename G, typename F>
 __global__ void __launch_bounds__(BLOCK)
-    k_inspect_ec(active_set_t as, G g, F f, long long *info) {
+    k_inspect_ec(active_set_t as, G g, F f, long long *slots) {
   __shared__ long long sh[BLOCK / WAVE];
   const int lane = lane_id();
   const int64_t m = g.nedges;
@@ -93,7 +100,7 @@ __global__ void __launch_bounds__(BLOCK)
     if (lane == 0) as.bitmap.active.dg_bits[w] = am;
     act += next ? 1 : 0;
   }
-  flush_sum(act, sh, info + 0);
+  flush_sum(act, sh, stat_slot(slots));
 }
 
 }  // namespace gsw

@@ -8,6 +8,7 @@
 // broadcast from LDS. No MFMA: the network is 6942 parameters and one level
 // needs 13.6 kFLOP.
 #pragma once
+#include "../abstraction/statistics.h"
 #include "../model/mlp.h"
 #include "../utils/wave.h"
 
@@ -153,17 +154,42 @@ __device__ __forceinline__ void block_features(const sel_dev_t &d, float act,
   }
 }
 
-// One workgroup, launched on the engine's stream right after the pass that
-// filled the six inspector counters (the inspector, or a TM pull that folded
-// the inspection in): turns them into the 12 features and writes the class
-// the network picks to info[6], which the host reads with the counters.
-// A kernel of its own rather than a tail of the pass, so the pass keeps its
-// register budget (occupancy) when the learned selector is on.
+// One workgroup, launched on the engine's stream right after a pass that
+// added the six inspector counters into the partial slots (the inspector,
+// or a TM pull that folded the inspection in): wavefront 0 reduces the
+// STAT_SLOTS slots (lane s reads slot s) into info[0..6), clears the slots
+// for the next pass and, with the learned selector on (d.w set), turns the
+// counters into the 12 features and writes the class the network picks to
+// info[6], which the host reads with the counters. A kernel of its own
+// rather than a tail of the pass, so the pass keeps its register budget.
 __global__ void __launch_bounds__(BLOCK)
-    k_mlp_select_counts(sel_dev_t d, long long *info) {
+    k_stat_finish(sel_dev_t d, long long *slots, long long *info) {
+  static_assert(STAT_SLOTS == WAVE, "one lane per partial slot");
   __shared__ mlp_block_sh_t s;
-  block_features(d, (float)info[0], (float)info[1], (float)info[2],
-                 (float)info[3], (float)info[4], s);
+  __shared__ long long tot[6];
+  if (threadIdx.x < WAVE) {
+    long long *p = slots + (int64_t)threadIdx.x * STAT_STRIDE;
+    long long v[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = p[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) p[k] = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = wave_sum_any(v[k]);
+    v[4] = wave_max_any(v[4]);
+    v[5] = wave_max_any(v[5]);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        tot[k] = v[k];
+        info[k] = v[k];
+      }
+    }
+  }
+  if (!d.w) return;  // uniform over the workgroup
+  __syncthreads();
+  block_features(d, (float)tot[0], (float)tot[1], (float)tot[2], (float)tot[3],
+                 (float)tot[4], s);
   const int cls = mlp_block_select(d.w, d.mask, s);
   if (threadIdx.x == 0) info[6] = cls;
 }
