@@ -170,7 +170,8 @@ struct executor_t {
     gsw::set_one_async<int32_t>(out_size, 0);
     if (!conf.conf_sorted) {
       hipLaunchKernelGGL(gsw::k_filter_unsorted,
-                         dim3(gsw::grid_for(nw, gsw::BLOCK, 1024)),
+                         dim3(gsw::grid_for((nw + gsw::FILTER_WPT - 1) / gsw::FILTER_WPT,
+                                            gsw::BLOCK, 1024)),
                          dim3(gsw::BLOCK), 0, gsw::stream(), as, target, n, out,
                          out_size);
     } else {

@@ -3,8 +3,8 @@
 // queue_t keeps the reference's two modes (src/data_structures/queue.cuh:
 // 86-188): Normal = ping-pong input/output queues swapped every level;
 // Cached = every level's queue appended to one history array (used by BC's
-// backward "traceback" replay). Differences: pushes are wavefront-aggregated
-// (one atomic per wave) and bounds-checked against the output capacity; an
+// backward "traceback" replay). Differences: pushes are staged per
+// wavefront in LDS (one atomic per flushed block of ids) and bounds-checked against the output capacity; an
 // overflow sets a device flag the host turns into an error instead of
 // writing past the allocation (queue.cuh:65-68 has no check).
 #pragma once
@@ -64,16 +64,18 @@ struct queue_t {
   __device__ __forceinline__ int64_t output_capacity() const {
     return (mode == Normal) ? capacity : capacity - offset_out;
   }
-  // Append u to the output queue (wave-aggregated); no-op during traceback.
-  __device__ __forceinline__ void push(bool want, int u) {
+  // Append u to the output queue, staged per wavefront in LDS (no-op during
+  // traceback): all 64 lanes call it; the entries reach the queue when the
+  // stage fills and at flush_stage().
+  static constexpr int STAGE = 256;  // entries per wavefront
+  using stage_t = gsw::wave_stage_t<STAGE>;
+  __device__ __forceinline__ void push(bool want, int u, stage_t &st) {
     if (mode == Cached && traceback) return;
-    int slot = gsw::wave_append(want, output_size());
-    if (want) {
-      if (slot < output_capacity())
-        output_base()[slot] = u;
-      else
-        *overflow = 1;
-    }
+    st.put(want, u, output_size(), output_base(), output_capacity(), overflow);
+  }
+  __device__ __forceinline__ void flush_stage(stage_t &st) {
+    if (mode == Cached && traceback) return;
+    st.flush(output_size(), output_base(), output_capacity(), overflow);
   }
 
   // ---- host side (a plain array: queue_t is copied byte-wise to kernels) --

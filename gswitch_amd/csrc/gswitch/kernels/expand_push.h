@@ -30,12 +30,6 @@
 
 namespace gsw {
 
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Edge / vertex work counters of an expand: a workgroup reduction, then one
 // atomic per counter into the workgroup's slot of the current record row
 // (slot blockIdx.x % WORK_SLOTS; the slots of a row lie WORK_ROWS rows
@@ -99,36 +93,65 @@ template <typename G, typename F, bool FUSED> struct push_body {
     }
   }
 
-  __device__ __forceinline__ void emit_q(bool q, int u) {
-    if constexpr (FUSED) as.queue.push(q, u);
+  // fused pushes stage the next queue per wavefront in LDS (STAGE_LDS ints
+  // per workgroup); emit_q and emit_flush are called by all 64 lanes
+  using stage_t = queue_t::stage_t;
+  static constexpr int STAGE_LDS = FUSED ? (BLOCK / WAVE) * queue_t::STAGE : 1;
+  __device__ __forceinline__ stage_t stage(int *lds) const {
+    stage_t st;
+    st.buf = FUSED ? lds + (threadIdx.x >> LOG_WAVE) * queue_t::STAGE : lds;
+    return st;
+  }
+  __device__ __forceinline__ void emit_q(bool q, int u, stage_t &st) {
+    if constexpr (FUSED) as.queue.push(q, u, st);
+  }
+  __device__ __forceinline__ void emit_flush(stage_t &st) {
+    if constexpr (FUSED) as.queue.flush_stage(st);
   }
 };
 
 // ---------------------------------------------------------------- TM
 template <class B>
 __global__ void __launch_bounds__(BLOCK) k_push_tm(B body) {
+  __shared__ int s_stage[B::STAGE_LDS];
+  auto st = body.stage(s_stage);
+  const int lane = lane_id();
   const int64_t N = body.as.get_size();
   const Status want = body.conf.want();
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   unsigned long long ne = 0, nv = 0;
-  for (int64_t idx = tid; idx < N; idx += stride) {
-    const int v = body.as.fetch(idx, want);
-    if (v < 0) continue;
-    const int64_t s = body.rows()[v], e = body.rows()[v + 1];
-    ne += e - s;
-    ++nv;
-    for (int64_t j = s; j < e; ++j) {
-      const int u = body.cols()[j];
-      body.emit_q(body(v, u, j), u);
+  // the loops are uniform over the wavefront (staged emits need all lanes)
+  for (int64_t base = tid - lane; base < N; base += stride) {
+    const int64_t idx = base + lane;
+    const int v = (idx < N) ? body.as.fetch(idx, want) : -1;
+    int64_t j = 0, e = 0;
+    if (v >= 0) {
+      j = body.rows()[v];
+      e = body.rows()[v + 1];
+      ne += e - j;
+      ++nv;
+    }
+    while (ballot(j < e)) {
+      int u = -1;
+      bool q = false;
+      if (j < e) {
+        u = body.cols()[j];
+        q = body(v, u, j);
+        ++j;
+      }
+      body.emit_q(q, u, st);
     }
   }
+  body.emit_flush(st);
   add_work(body.as.dg_work, ne, nv);
 }
 
 // ---------------------------------------------------------------- WM
 template <class B>
 __global__ void __launch_bounds__(BLOCK) k_push_wm(B body) {
+  __shared__ int s_stage[B::STAGE_LDS];
+  auto st = body.stage(s_stage);
   __shared__ int s_v[BLOCK];
   __shared__ int s_scan[BLOCK];
   __shared__ int64_t s_start[BLOCK];
@@ -166,16 +189,19 @@ __global__ void __launch_bounds__(BLOCK) k_push_wm(B body) {
         u = body.cols()[e];
         q = body(s_v[off + o], u, e);
       }
-      body.emit_q(q, u);
+      body.emit_q(q, u, st);
     }
     wave_sync();
   }
+  body.emit_flush(st);
   add_work(body.as.dg_work, ne, nv);
 }
 
 // ---------------------------------------------------------------- CM
 template <class B>
 __global__ void __launch_bounds__(BLOCK) k_push_cm(B body) {
+  __shared__ int s_stage[B::STAGE_LDS];
+  auto st = body.stage(s_stage);
   __shared__ int s_v[BLOCK];
   __shared__ int s_scan[BLOCK];
   __shared__ int64_t s_start[BLOCK];
@@ -211,10 +237,11 @@ __global__ void __launch_bounds__(BLOCK) k_push_cm(B body) {
         u = body.cols()[e];
         q = body(s_v[o], u, e);
       }
-      body.emit_q(q, u);
+      body.emit_q(q, u, st);
     }
     __syncthreads();
   }
+  body.emit_flush(st);
   add_work(body.as.dg_work, ne, nv);
 }
 
@@ -294,6 +321,8 @@ __device__ __forceinline__ int64_t strict_next_owner(const int64_t *s_scan,
 template <class B>
 __global__ void __launch_bounds__(BLOCK)
     k_push_strict(B body, const int64_t *__restrict__ scan) {
+  __shared__ int s_stage[B::STAGE_LDS];
+  auto st = body.stage(s_stage);
   __shared__ int64_t s_scan[SCHUNK + 1];
   __shared__ int64_t s_start[SCHUNK + 1];
   __shared__ int s_v[SCHUNK + 1];
@@ -335,12 +364,13 @@ __global__ void __launch_bounds__(BLOCK)
         q = body(v[j], u[j], e[j]);
         ++ne;
       }
-      body.emit_q(q, u[j]);
+      body.emit_q(q, u[j], st);
     }
     const int64_t nk = strict_next_owner(s_scan, k0, c + SCHUNK);
     __syncthreads();
     k0 = nk;
   }
+  body.emit_flush(st);
   add_work(body.as.dg_work, ne, 0);
 }
 
@@ -349,10 +379,17 @@ template <typename G>
 __global__ void __launch_bounds__(BLOCK)
     k_twc_bin(active_set_t as, G g, Status want, bool pull, int small,
               int large) {
+  // one LDS stage per bin and wavefront: one atomic per bin per flush
+  constexpr int CAP = 256;
+  __shared__ int s_stage[3][BLOCK / WAVE][CAP];
+  wave_stage_t<CAP> st[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) st[k].buf = s_stage[k][threadIdx.x >> LOG_WAVE];
   const int64_t N = as.get_size();
   const int64_t *rows = pull ? g.dgr_row_ptr : g.dg_row_ptr;
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t cap = as.bins.cap;  // bins hold every frontier item
   for (int64_t i0 = tid - lane_id(); i0 < N; i0 += stride) {
     const int64_t i = i0 + lane_id();
     int v = (i < N) ? as.fetch(i, want) : -1;
@@ -362,15 +399,19 @@ __global__ void __launch_bounds__(BLOCK)
       b = (d < small) ? 0 : (d < large) ? 1 : 2;
     }
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const int slot = wave_append(b == k, as.bins.sizes + k);
-      if (b == k) as.bins.bin[k][slot] = v;
-    }
+    for (int k = 0; k < 3; ++k)
+      st[k].put(b == k, v, as.bins.sizes + k, as.bins.bin[k], cap,
+                as.queue.overflow);
   }
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    st[k].flush(as.bins.sizes + k, as.bins.bin[k], cap, as.queue.overflow);
 }
 
 template <class B>
 __global__ void __launch_bounds__(BLOCK) k_push_twc(B body) {
+  __shared__ int s_stage[B::STAGE_LDS];
+  auto st = body.stage(s_stage);
   const int lane = lane_id();
   const int32_t *sizes = body.as.bins.sizes;
   unsigned long long ne = 0, nv = 0;
@@ -391,7 +432,7 @@ __global__ void __launch_bounds__(BLOCK) k_push_twc(B body) {
         u = body.cols()[j];
         q = body(v, u, j);
       }
-      body.emit_q(q, u);
+      body.emit_q(q, u, st);
     }
   }
   // medium: one wavefront per vertex
@@ -413,23 +454,36 @@ __global__ void __launch_bounds__(BLOCK) k_push_twc(B body) {
         u = body.cols()[j];
         q = body(v, u, j);
       }
-      body.emit_q(q, u);
+      body.emit_q(q, u, st);
     }
   }
-  // small: one thread per vertex
+  // small: one thread per vertex (loops uniform over the wavefront)
   const int ns = sizes[0];
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = tid; i < ns; i += stride) {
-    const int v = body.as.bins.bin[0][i];
-    const int64_t s = body.rows()[v], e = body.rows()[v + 1];
-    ne += e - s;
-    ++nv;
-    for (int64_t j = s; j < e; ++j) {
-      const int u = body.cols()[j];
-      body.emit_q(body(v, u, j), u);
+  for (int64_t base = tid - lane; base < ns; base += stride) {
+    const int64_t i = base + lane;
+    int v = -1;
+    int64_t j = 0, e = 0;
+    if (i < ns) {
+      v = body.as.bins.bin[0][i];
+      j = body.rows()[v];
+      e = body.rows()[v + 1];
+      ne += e - j;
+      ++nv;
+    }
+    while (ballot(j < e)) {
+      int u = -1;
+      bool q = false;
+      if (j < e) {
+        u = body.cols()[j];
+        q = body(v, u, j);
+        ++j;
+      }
+      body.emit_q(q, u, st);
     }
   }
+  body.emit_flush(st);
   add_work(body.as.dg_work, ne, nv);
 }
 

@@ -37,17 +37,38 @@ __device__ __forceinline__ void write_word_ids(uint64_t m, int64_t w,
   }
 }
 
+// Each thread owns FILTER_WPT consecutive words; a workgroup scan of their
+// popcounts and one atomicAdd per workgroup step reserve the output range
+// (one atomic per 64 Ki vertices: a counter shared by every wavefront of
+// the grid serialises its atomics).
+constexpr int FILTER_WPT = 4;
 __global__ void __launch_bounds__(BLOCK)
     k_filter_unsorted(active_set_t as, Status target, int64_t n,
                       int32_t *out, int32_t *out_size) {
+  __shared__ int sh[BLOCK / WAVE];
+  __shared__ int s_base;
   const int64_t nw = (n + 63) >> 6;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> LOG_WAVE;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> LOG_WAVE;
-  for (int64_t w0 = wave << 6; w0 < nw; w0 += nwaves << 6) {
-    const int64_t w = w0 + lane_id();
-    const uint64_t m = (w < nw) ? target_word(as, w, target, n) : 0ull;
-    const int pos = wave_append_n(__popcll(m), out_size);
-    write_word_ids(m, w, out, pos);
+  const int64_t step = (int64_t)gridDim.x * BLOCK * FILTER_WPT;
+  for (int64_t w0 = (int64_t)blockIdx.x * BLOCK * FILTER_WPT; w0 < nw; w0 += step) {
+    const int64_t w = w0 + (int64_t)threadIdx.x * FILTER_WPT;
+    uint64_t m[FILTER_WPT];
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < FILTER_WPT; ++k) {
+      m[k] = (w + k < nw) ? target_word(as, w + k, target, n) : 0ull;
+      c += __popcll(m[k]);
+    }
+    int tot;
+    int pos = block_exclusive_scan(c, tot, sh);
+    if (threadIdx.x == 0) s_base = tot ? atomicAdd(out_size, tot) : 0;
+    __syncthreads();
+    pos += s_base;
+#pragma unroll
+    for (int k = 0; k < FILTER_WPT; ++k) {
+      write_word_ids(m[k], w + k, out, pos);
+      pos += __popcll(m[k]);
+    }
+    __syncthreads();  // s_base and sh are reused by the next step
   }
 }
 
