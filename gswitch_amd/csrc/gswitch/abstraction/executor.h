@@ -295,8 +295,9 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf,
       twod_ok = !push && conf.conf_fromall && as.fmt == Bitmap && g.twod;
     if (lb == ELB || (lb == TWOD && !twod_ok)) lb = STRICT;
     if (lb == TWOD) {
-      as.workset.ensure(as.size);
-      twod_ensure(g, TWOD_MAX, as.workset.dg_bsum);
+      // slice width from the bytes of source state one message reads
+      constexpr int64_t src_bytes = sizeof(typename F::ra_t) > 4 ? sizeof(typename F::ra_t) : 4;
+      twod_ensure(g, src_bytes);
     }
     // Scratch must exist before any kernel body copies `as` by value.
     if (lb == TWC && push) as.bins.ensure(as.size);
@@ -389,14 +390,26 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf,
           hipLaunchKernelGGL(k_pull_cm<Bd>, dim3(expand_grid(N, BLOCK)),
                              dim3(BLOCK), 0, st, body);
           break;
-        case TWOD:
-          if (conf.mp_pull)
-            hipLaunchKernelGGL(k_pull_mp_twod<Bd>, dim3(MAX_GRID), dim3(BLOCK),
-                               0, st, body, *g.twod);
-          else
-            hipLaunchKernelGGL(k_pull_twod<Bd>, dim3(MAX_GRID), dim3(BLOCK), 0,
-                               st, body, *g.twod);
+        case TWOD: {
+          // the slices one after another (each launch owns its targets)
+          const twod_t &tw = *g.twod;
+          for (int p = 0; p < tw.P; ++p) {
+            const twod_slice_t &sl = tw.s[p];
+            if (sl.ne == 0) continue;
+            if (conf.mp_pull) {
+              int grid = 0;
+              for (int c = 0; c <= TWOD_BINS; ++c) grid += sl.blocks[c];
+              hipLaunchKernelGGL(k_pull_bins<Bd>, dim3(grid), dim3(BLOCK), 0, st,
+                                 body, sl);
+            } else {
+              const int64_t tiles = (sl.ne + sl.nr + SCHUNK - 1) / SCHUNK;
+              const int grid = (int)MIN((int64_t)MAX_GRID, tiles);
+              hipLaunchKernelGGL(k_pull_twod<Bd>, dim3(grid), dim3(BLOCK), 0, st,
+                                 body, sl);
+            }
+          }
           break;
+        }
         default: {  // STRICT
           const int64_t *scan = nullptr;  // bitmap: implicit rows[v] + v
           if (conf.mp_pull && conf.conf_fromall && !conf.conf_pruning &&

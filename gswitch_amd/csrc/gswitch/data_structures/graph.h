@@ -50,19 +50,54 @@ struct graph_attr_t {
   }
 };
 
-// 2-D (source-sliced) layout for from-all pulls, the TWOD variant. The
-// in-adjacency is split into P slices by source id, with slice boundaries at
-// equal edge counts; slice p is a full CSR (rp[p][n+1], col[p][...]) over
-// sources in [bound[p], bound[p+1]). Workgroup b of a TWOD launch serves
-// slice b % P, which is the XCD the dispatcher places it on, so each XCD's
-// gathers stay inside one slice that fits its L2 (cf. chunk_t,
-// src/utils/2d_partition.cuh:13-299, which tiles for shared memory).
-constexpr int TWOD_MAX = 8;
+// Source-sliced layout for from-all pulls, the TWOD variant. The
+// in-adjacency is cut into P slices of consecutive source ids, P chosen so
+// that one slice's source-side data (the message each source emits) spans
+// about TWOD_SLICE_BYTES. Slice p keeps only its non-empty target rows
+// (doubly compressed): tgt[r] is the target vertex of row r (ascending),
+// off[r..r+1) its range in col, and col holds the sources. The slices are
+// pulled one launch after another, so each launch is the only producer for
+// its targets and rows are updated with plain loads and stores (cf.
+// chunk_t, src/utils/2d_partition.cuh:13-299, which tiles for shared
+// memory instead). On MI355X the pull is bound by the L2 request rate of
+// its message gathers (about one request per edge), not by where the lines
+// come from: slices of 2-16 MB time within 7% of each other on R-MAT scale
+// 22, so the default slice is 16 MB (one slice up to 4 Mi vertices of FP32
+// state); GSW_TWOD_SLICE_KB overrides it.
+//
+// For the binned pull the rows of a slice are also listed by length: bin c
+// holds the rows of length in (twod_bin_len(c-1), twod_bin_len(c)], served
+// by groups of twod_bin_group(c) lanes; longer rows are cut into chunks of
+// TWOD_CHUNK edges, one wavefront each. Every group and every chunk is one
+// unrolled step of TWOD_UNROLL edges per lane, so work items are uniform.
+// `blocks[c]` (c = 0..TWOD_BINS-1 bins, TWOD_BINS chunks) is the workgroup
+// count a launch gives each part, in proportion to its wavefront steps.
+constexpr int TWOD_MAX = 64;
+constexpr int64_t TWOD_SLICE_BYTES = 16ll << 20;
+constexpr int TWOD_BINS = 2;
+constexpr int TWOD_UNROLL = 8;  // edges a lane keeps in flight
+__host__ __device__ constexpr int twod_bin_group(int c) { return c == 0 ? 1 : 8; }
+__host__ __device__ constexpr int twod_bin_len(int c) {
+  return twod_bin_group(c) * TWOD_UNROLL;
+}
+constexpr int64_t TWOD_CHUNK = 64 * TWOD_UNROLL;
+struct twod_slice_t {
+  int64_t nr = 0;          // non-empty target rows
+  int64_t ne = 0;          // edges
+  int32_t *tgt = nullptr;  // [nr]
+  int64_t *off = nullptr;  // [nr + 1]
+  int32_t *col = nullptr;  // [ne]
+  int32_t *bin_rows[TWOD_BINS] = {};
+  int64_t bin_n[TWOD_BINS] = {};
+  int32_t *chunk_row = nullptr;    // [n_chunks]
+  int64_t *chunk_first = nullptr;  // [n_chunks] first edge of the chunk
+  int64_t n_chunks = 0;
+  int32_t blocks[TWOD_BINS + 1] = {};
+};
 struct twod_t {
   int P = 0;
-  int64_t *rp[TWOD_MAX] = {};
-  int32_t *col[TWOD_MAX] = {};
-  int32_t bound[TWOD_MAX + 1] = {};
+  int32_t bound[TWOD_MAX + 1] = {};  // slice p: sources [bound[p], bound[p+1])
+  twod_slice_t s[TWOD_MAX];
 };
 
 template <GraphFmt fmt, typename E> struct device_graph_t {};
@@ -128,8 +163,13 @@ template <typename E> struct device_graph_t<CSR, E> {
   void release() {
     if (twod) {
       for (int p = 0; p < twod->P; ++p) {
-        hipFree(twod->rp[p]);
-        hipFree(twod->col[p]);
+        twod_slice_t &sl = twod->s[p];
+        hipFree(sl.tgt);
+        hipFree(sl.off);
+        hipFree(sl.col);
+        for (int c = 0; c < TWOD_BINS; ++c) hipFree(sl.bin_rows[c]);
+        hipFree(sl.chunk_row);
+        hipFree(sl.chunk_first);
       }
       delete twod;
       twod = nullptr;

@@ -33,6 +33,19 @@ template <typename T> __device__ __forceinline__ T shfl_down_any(T x, int d) {
   return b.t;
 }
 
+template <typename T> __device__ __forceinline__ T shfl_xor_any(T x, int d) {
+  constexpr int W = (int)((sizeof(T) + 3) / 4);
+  union U {
+    T t;
+    int i[W];
+    __device__ U() {}
+  } a, b;
+  a.t = x;
+#pragma unroll
+  for (int k = 0; k < W; ++k) b.i[k] = __shfl_xor(a.i[k], d, 64);
+  return b.t;
+}
+
 template <typename G, typename F> struct pull_body {
   using WA = typename F::wa_t;
   active_set_t as;
@@ -331,13 +344,13 @@ __global__ void __launch_bounds__(BLOCK) k_pull_cm(B body) {
 template <class B>
 __device__ __forceinline__ void pull_strict_range(B &body, const slots_t sl,
                                                   const int32_t *cols,
+                                                  const int32_t *tgt, int64_t N,
                                                   int64_t bi, int64_t nb,
                                                   bool allow_whole) {
   using WA = typename B::WA;
   __shared__ int64_t s_scan[SCHUNK + 1];
   __shared__ int64_t s_start[SCHUNK + 1];
   __shared__ int s_v[SCHUNK + 1];
-  const int64_t N = body.as.get_size();
   const Status want = body.conf.want();
   const bool prune = body.conf.conf_pruning;
   const int64_t total = sl.at(N);
@@ -349,7 +362,7 @@ __device__ __forceinline__ void pull_strict_range(B &body, const slots_t sl,
   const int wofs = threadIdx.x & ~(WAVE - 1);
   int64_t k0 = sl.last_le(N + 1, lo);
   for (int64_t c = lo; c < hi; c += SCHUNK) {
-    strict_window(body, sl, N, k0, want, s_scan, s_v, s_start);
+    strict_window(body, sl, N, k0, want, s_scan, s_v, s_start, tgt);
     __syncthreads();
     int o[SITEMS], u[SITEMS];
     int64_t e[SITEMS];
@@ -432,8 +445,10 @@ constexpr int MP_TILE = BLOCK * MP_R;
 
 template <class B>
 __device__ __forceinline__ void pull_mp_range(B &body, const int64_t *rows,
-                                              const int32_t *cols, int64_t bi,
-                                              int64_t nb, bool exclusive) {
+                                              const int32_t *cols,
+                                              const int32_t *tgt, int64_t N,
+                                              int64_t bi, int64_t nb,
+                                              bool exclusive) {
   using WA = typename B::WA;
   __shared__ int s_row[MP_TILE + 2];  // row prefix relative to the tile start
   __shared__ WA s_msg[MP_TILE];
@@ -441,7 +456,6 @@ __device__ __forceinline__ void pull_mp_range(B &body, const int64_t *rows,
   __shared__ int s_tail_row[BLOCK];
   __shared__ int64_t s_k0;
   const int t = threadIdx.x;
-  const int64_t N = body.as.get_size();
   const Status want = body.conf.want();
   const slots_t sl{nullptr, rows};
   const int64_t total = sl.at(N);
@@ -492,7 +506,7 @@ __device__ __forceinline__ void pull_mp_range(B &body, const int64_t *rows,
       bool has = false;
       for (int i = r0; i < r1; ++i) {
         if (i == re) {
-          const int64_t v = k0 + k;
+          const int64_t v = tgt ? (int64_t)tgt[k0 + k] : k0 + k;
           if (rs >= r0) {  // the whole row lies in this lane's run
             if (has && body.as.fetch(v, want) >= 0) {
               if (exclusive) {
@@ -530,7 +544,7 @@ __device__ __forceinline__ void pull_mp_range(B &body, const int64_t *rows,
     __syncthreads();
     // C: heads gather the tails before them
     auto flush = [&](int k, WA val, bool inside) {
-      const int64_t v = k0 + k;
+      const int64_t v = tgt ? (int64_t)tgt[k0 + k] : k0 + k;
       if (body.as.fetch(v, want) < 0) return;
       if (exclusive && inside) {
         WA w = *body.f.wa_of((int)v);
@@ -572,32 +586,170 @@ __device__ __forceinline__ void pull_mp_range(B &body, const int64_t *rows,
 
 template <class B>
 __global__ void __launch_bounds__(BLOCK) k_pull_mp(B body) {
-  pull_mp_range(body, body.rows(), body.cols(), (int64_t)blockIdx.x,
-                (int64_t)gridDim.x, true);
+  pull_mp_range(body, body.rows(), body.cols(), nullptr, body.as.get_size(),
+                (int64_t)blockIdx.x, (int64_t)gridDim.x, true);
+}
+
+// ---------------------------------------------------------------- binned
+// Binned pull over one TWOD slice (twod_slice_t): groups of G lanes reduce
+// one row each (lane j of a group takes edges j, j + G, ...; a butterfly
+// over the group combines them); the slice launch owns every row it
+// writes, so rows are updated with a plain load and store. Rows longer
+// than the last bin come as TWOD_CHUNK-edge chunks, one wavefront each; a
+// row's only chunk is written plainly, several are combined into the target
+// with compAtomic. No LDS, no tile phases. Each
+// lane issues TWOD_UNROLL column loads and then their TWOD_UNROLL message
+// gathers before combining (addresses clamped into the row, so the loads
+// are unconditional): the row walk is bounded by memory-level parallelism,
+// not by one load-to-use latency per edge.
+template <int G, class B>
+__device__ __forceinline__ void bin_walk(B &body, const int32_t *col, int64_t e0,
+                                         int64_t e1, int sub,
+                                         typename B::WA &acc, bool &has) {
+  using WA = typename B::WA;
+  constexpr int U = TWOD_UNROLL;
+  for (int64_t eb = e0 + sub; eb < e1; eb += (int64_t)G * U) {
+    int u[U];
+    int64_t ek[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int64_t e = eb + (int64_t)k * G;
+      ek[k] = e < e1 ? e : e1 - 1;
+      u[k] = col[ek[k]];
+    }
+    WA m[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) m[k] = body.msg(u[k], ek[k]);
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      if (eb + (int64_t)k * G < e1) {
+        if (has)
+          body.f.comp(&acc, m[k], body.g);
+        else
+          acc = m[k];
+        has = true;
+      }
+    }
+  }
+}
+template <class B>
+__device__ __forceinline__ void bin_combine(B &body, typename B::WA &acc,
+                                            bool &has, int group) {
+  for (int d = group >> 1; d > 0; d >>= 1) {
+    const typename B::WA o = shfl_xor_any(acc, d);
+    const int oh = __shfl_xor((int)has, d, 64);
+    if (oh) {
+      if (has)
+        body.f.comp(&acc, o, body.g);
+      else
+        acc = o;
+      has = true;
+    }
+  }
+}
+
+template <int G, class B>
+__device__ __forceinline__ unsigned long long pull_bin_rows(
+    B &body, const twod_slice_t &sl, const int32_t *rows, int64_t nrows,
+    int64_t bi, int64_t nb) {
+  using WA = typename B::WA;
+  constexpr int GPW = WAVE / G;  // groups per wavefront
+  const int lane = lane_id(), sub = lane & (G - 1);
+  const Status want = body.conf.want();
+  const int64_t wave = (bi * BLOCK + threadIdx.x) >> LOG_WAVE;
+  const int64_t nwaves = (nb * BLOCK) >> LOG_WAVE;
+  unsigned long long ne = 0;
+  // uniform over the wavefront: the butterfly needs every lane
+  for (int64_t i0 = wave * GPW; i0 < nrows; i0 += nwaves * GPW) {
+    const int64_t i = i0 + lane / G;
+    WA acc{};
+    bool has = false;
+    int r = -1;
+    if (i < nrows) {
+      r = rows[i];
+      const int64_t e0 = sl.off[r], e1 = sl.off[r + 1];
+      bin_walk<G>(body, sl.col, e0, e1, sub, acc, has);
+      if (sub == 0) ne += (unsigned long long)(e1 - e0);
+    }
+    bin_combine(body, acc, has, G);
+    if (sub == 0 && has) {
+      const int v = sl.tgt[r];
+      if (body.as.fetch(v, want) >= 0) {
+        WA w = *body.f.wa_of(v);
+        body.f.comp(&w, acc, body.g);
+        *body.f.wa_of(v) = w;
+      }
+    }
+  }
+  return ne;
 }
 
 template <class B>
-__global__ void __launch_bounds__(BLOCK) k_pull_mp_twod(B body, twod_t tw) {
-  const int p = (int)(blockIdx.x % (unsigned)tw.P);
-  pull_mp_range(body, tw.rp[p], tw.col[p], (int64_t)(blockIdx.x / tw.P),
-                (int64_t)(gridDim.x / tw.P), false);
+__device__ __forceinline__ unsigned long long pull_bin_chunks(
+    B &body, const twod_slice_t &sl, int64_t bi, int64_t nb) {
+  using WA = typename B::WA;
+  const int lane = lane_id();
+  const Status want = body.conf.want();
+  const int64_t wave = (bi * BLOCK + threadIdx.x) >> LOG_WAVE;
+  const int64_t nwaves = (nb * BLOCK) >> LOG_WAVE;
+  unsigned long long ne = 0;
+  for (int64_t k = wave; k < sl.n_chunks; k += nwaves) {
+    const int r = sl.chunk_row[k];
+    const int64_t e0 = sl.chunk_first[k];
+    const int64_t e1 = MIN(e0 + TWOD_CHUNK, sl.off[r + 1]);
+    WA acc{};
+    bool has = false;
+    bin_walk<WAVE>(body, sl.col, e0, e1, lane, acc, has);
+    bin_combine(body, acc, has, WAVE);
+    if (lane == 0) {
+      ne += (unsigned long long)(e1 - e0);
+      const int v = sl.tgt[r];
+      if (has && body.as.fetch(v, want) >= 0) {
+        if (e0 == sl.off[r] && e1 == sl.off[r + 1]) {  // the row's only chunk
+          WA w = *body.f.wa_of(v);
+          body.f.comp(&w, acc, body.g);
+          *body.f.wa_of(v) = w;
+        } else {
+          body.f.compAtomic(body.f.wa_of(v), acc, body.g);
+        }
+      }
+    }
+  }
+  return ne;
+}
+
+// workgroups [0, blocks[0]) serve bin 0, the next blocks[1] bin 1, the
+// last blocks[2] the chunks of the longer rows
+template <class B>
+__global__ void __launch_bounds__(BLOCK) k_pull_bins(B body, twod_slice_t sl) {
+  static_assert(TWOD_BINS == 2, "two row bins and the chunks");
+  int64_t b = blockIdx.x;
+  unsigned long long ne = 0;
+  if (b < sl.blocks[0]) {
+    ne = pull_bin_rows<twod_bin_group(0)>(body, sl, sl.bin_rows[0], sl.bin_n[0],
+                                          b, sl.blocks[0]);
+  } else if ((b -= sl.blocks[0]) < sl.blocks[1]) {
+    ne = pull_bin_rows<twod_bin_group(1)>(body, sl, sl.bin_rows[1], sl.bin_n[1],
+                                          b, sl.blocks[1]);
+  } else {
+    b -= sl.blocks[1];
+    ne = pull_bin_chunks(body, sl, b, sl.blocks[2]);
+  }
+  add_work(body.as.dg_work, ne, 0);
 }
 
 template <class B>
 __global__ void __launch_bounds__(BLOCK)
     k_pull_strict(B body, const int64_t *__restrict__ scan) {
-  pull_strict_range(body, slots_t{scan, body.rows()}, body.cols(),
-                    (int64_t)blockIdx.x, (int64_t)gridDim.x, true);
+  pull_strict_range(body, slots_t{scan, body.rows()}, body.cols(), nullptr,
+                    body.as.get_size(), (int64_t)blockIdx.x, (int64_t)gridDim.x,
+                    true);
 }
 
-// TWOD: workgroup b serves source slice b % P (its XCD under round-robin
-// dispatch) as workgroup b / P of gridDim.x / P over that slice's slots.
 template <class B>
-__global__ void __launch_bounds__(BLOCK) k_pull_twod(B body, twod_t tw) {
-  const int p = (int)(blockIdx.x % (unsigned)tw.P);
-  pull_strict_range(body, slots_t{nullptr, tw.rp[p]}, tw.col[p],
-                    (int64_t)(blockIdx.x / tw.P), (int64_t)(gridDim.x / tw.P),
-                    false);
+__global__ void __launch_bounds__(BLOCK) k_pull_twod(B body, twod_slice_t sl) {
+  pull_strict_range(body, slots_t{nullptr, sl.off}, sl.col, sl.tgt, sl.nr,
+                    (int64_t)blockIdx.x, (int64_t)gridDim.x, true);
 }
 
 }  // namespace gsw

@@ -295,15 +295,17 @@ template <class B>
 __device__ __forceinline__ void strict_window(B &body, const slots_t &sl,
                                               int64_t N, int64_t k0,
                                               Status want, int64_t *s_scan,
-                                              int *s_v, int64_t *s_start) {
+                                              int *s_v, int64_t *s_start,
+                                              const int32_t *tgt = nullptr) {
+  // tgt (compressed rows): item k is row k of `rows`, target vertex tgt[k]
   for (int t = threadIdx.x; t <= SCHUNK; t += blockDim.x) {
     const int64_t k = k0 + t;
     s_scan[t] = sl.at(k < N ? k : N);
     int v = -1;
     int64_t st = 0;
     if (k < N) {
-      v = body.as.fetch(k, want);
-      if (v >= 0) st = sl.rows[v];
+      v = body.as.fetch(tgt ? (int64_t)tgt[k] : k, want);
+      if (v >= 0) st = sl.rows[tgt ? k : v];
     }
     s_v[t] = v;
     s_start[t] = st;
