@@ -565,7 +565,9 @@ void bfs_level_setup(gsw_graph *G, app_ctx_t<BFS> &c, int root, int cls) {
 // the next level (nothing extra when the expand already inspected it, as a
 // level-synchronous TM pull does); a fused level pays its expand and the
 // read of the next queue length. This level's own inspection belongs to the
-// level before it and is not timed.
+// level before it and is not timed. A fused level whose next frontier the
+// learned engine's gate hands to the inspector (large and grown, the
+// INSPECT_FRACTION / INSPECT_GROWTH test) also pays that inspection.
 double bfs_probe_level(gsw_graph *G, app_ctx_t<BFS> &c, int root, int cls,
                        int lvl) {
   bfs_level_setup(G, c, root, cls);
@@ -586,7 +588,19 @@ double bfs_probe_level(gsw_graph *G, app_ctx_t<BFS> &c, int root, int cls,
   g.update_level();
   en.executor.profile = true;
   en.executor.expand(c.as, g, c.f, en.stats, en.fets, en.conf);
-  if (!fused) en.inspector.inspect(c.as, g, c.f, en.stats, en.fets, en.conf);
+  if (!fused) {
+    en.inspector.inspect(c.as, g, c.f, en.stats, en.fets, en.conf);
+  } else {
+    // a next frontier that the learned engine's gate hands to the inspector
+    // costs that inspection too (the level after a fused choice pays it)
+    const double nq = (double)en.inspector.known_active;
+    const double fr = (double)en.executor.log.back().frontier;
+    if (nq > 0 && nq * gsw_mlp::INSPECT_FRACTION >= (double)c.as.size &&
+        nq >= gsw_mlp::INSPECT_GROWTH * fr) {
+      en.conf.conf_fuse_inspect = false;
+      en.inspector.inspect(c.as, g, c.f, en.stats, en.fets, en.conf);
+    }
+  }
   gsw::sync();
   const double wall = gsw_now_ms() - t0;
   static const bool verbose = std::getenv("GSW_EXPLORE_VERBOSE") != nullptr;

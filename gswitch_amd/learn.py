@@ -238,6 +238,8 @@ def main(argv=None):
     ap.add_argument("cmd", choices=["collect", "train", "evaluate"])
     ap.add_argument("--set", default="train", choices=["train", "eval"])
     ap.add_argument("--count", type=int, default=300)
+    ap.add_argument("--start", type=int, default=0,
+                    help="collect specs[start:start+count] of the set")
     ap.add_argument("--data")
     ap.add_argument("--model", default=MODEL_PATH)
     ap.add_argument("--out", default="gpurun_out/selector")
@@ -247,9 +249,11 @@ def main(argv=None):
     os.makedirs(a.out, exist_ok=True)
     log = lambda s: print(s, flush=True)
     if a.cmd == "collect":
-        specs = training_specs(a.count) if a.set == "train" else eval_specs()
+        specs = training_specs() if a.set == "train" else eval_specs()
+        specs = specs[a.start:a.start + a.count]
         d = collect(specs, reps=a.reps, log=log)
-        np.savez(os.path.join(a.out, a.set + ".npz"), specs=json.dumps(specs), **d)
+        name = a.set if a.start == 0 else "%s_%d" % (a.set, a.start)
+        np.savez(os.path.join(a.out, name + ".npz"), specs=json.dumps(specs), **d)
     elif a.cmd == "train":
         with np.load(a.data, allow_pickle=False) as z:
             d = {k: z[k] for k in z.files}
