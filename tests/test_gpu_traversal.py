@@ -155,6 +155,25 @@ def test_pagerank(gsw, orc, gset, variant, pull_kernel):
         assert max_rel(pr, ref) <= PR_TOL, (variant, gname, max_rel(pr, ref))
 
 
+@pytest.mark.parametrize("slice_kb", [1, 4, 64])
+def test_pagerank_twod_slices(gsw, orc, monkeypatch, slice_kb):
+    """TWOD with several source slices (one launch each): R-MAT 12 has
+    n * 4 B = 16 KB of source state, so 1 KB slices give 16 launches per
+    iteration; its degrees reach 1345, so rows of every length bin and rows
+    cut into several 512-edge chunks (combined with compAtomic) occur."""
+    monkeypatch.setenv("GSW_TWOD_SLICE_KB", str(slice_kb))
+    h = gsw.rmat(12, 16, seed=1)
+    deg = np.diff(h.row_ptr)
+    assert deg.max() > 2 * 512 and (deg <= 8).any() and ((deg > 8) & (deg <= 64)).any()
+    ref = orc.pagerank(h.row_ptr, h.col, 0.85, 50)
+    for pull_kernel in (0, 1):
+        g = gsw.Graph(h)  # slices are built on the graph's first TWOD pull
+        pr = g.pagerank(0.85, 50, configs="Pull-Bitmap-TWOD-Standalone",
+                        pull_kernel=pull_kernel)
+        g.close()
+        assert max_rel(pr, ref) <= PR_TOL, (slice_kb, pull_kernel, max_rel(pr, ref))
+
+
 def test_pagerank_iteration_counts(gsw, orc, gset):
     h, g = gset["er"]
     for k in (0, 1, 2, 5):
