@@ -1251,8 +1251,8 @@ gsw_status gsw_msbfs64(gsw_graph *G, const int32_t *sources, int32_t nsrc,
     const run_opts_t o = to_opts(opts);
     if (!G->ms_buf) {
       H_ERR(hipMalloc((void **)&G->ms_buf, sizeof(uint64_t) * 3 * (n + 1)));
-      H_ERR(hipMalloc((void **)&G->ms_cnt, sizeof(ms_counters_t)));
-      H_ERR(hipHostMalloc((void **)&G->ms_hcnt, sizeof(ms_counters_t)));
+      H_ERR(hipMalloc((void **)&G->ms_cnt, sizeof(ms_counters_t) * MS_SLOTS));
+      H_ERR(hipHostMalloc((void **)&G->ms_hcnt, sizeof(ms_counters_t) * MS_SLOTS));
       H_ERR(hipMalloc((void **)&G->ms_src, sizeof(int32_t) * 64));
     }
     uint64_t *visit = G->ms_buf, *front = G->ms_buf + (n + 1),
@@ -1281,7 +1281,7 @@ gsw_status gsw_msbfs64(gsw_graph *G, const int32_t *sources, int32_t nsrc,
       const bool pull = o.force.has_direction ? (o.force.d == Pull)
                                               : front_deg > (double)m / 32.0;
       fill_async<unsigned long long>((unsigned long long *)G->ms_cnt, 0ull,
-                                     sizeof(ms_counters_t) / 8);
+                                     sizeof(ms_counters_t) / 8 * MS_SLOTS);
       if (pull) {
         hipLaunchKernelGGL(k_msbfs_pull, dim3(grid), dim3(BLOCK), 0, st, rows,
                            cols, n, all, visit, (const uint64_t *)front, next,
@@ -1295,19 +1295,26 @@ gsw_status gsw_msbfs64(gsw_graph *G, const int32_t *sources, int32_t nsrc,
                            n, visit, next, G->ms_cnt);
       }
       H_ERR(hipGetLastError());
-      H_ERR(hipMemcpyAsync(G->ms_hcnt, G->ms_cnt, sizeof(ms_counters_t),
+      H_ERR(hipMemcpyAsync(G->ms_hcnt, G->ms_cnt, sizeof(ms_counters_t) * MS_SLOTS,
                            hipMemcpyDeviceToHost, st));
       sync();
+      ms_counters_t tot{};
+      for (int k = 0; k < MS_SLOTS; ++k) {  // sum the partial slots
+        const ms_counters_t &p = G->ms_hcnt[k];
+        tot.newly += p.newly;
+        tot.newly_deg += p.newly_deg;
+        for (int s = 0; s < 64; ++s) tot.per_src[s] += p.per_src[s];
+      }
       gsw_level_record rec{};
       rec.level = level;
       rec.dir = pull ? 1 : 0;
       rec.fmt = pull ? 1 : 0;
-      rec.frontier = (int64_t)G->ms_hcnt->newly;
-      rec.edges = (int64_t)G->ms_hcnt->newly_deg;
+      rec.frontier = (int64_t)tot.newly;
+      rec.edges = (int64_t)tot.newly_deg;
       G->records.push_back(rec);
-      if (G->ms_hcnt->newly == 0) break;
-      for (int s = 0; s < 64; ++s) sums[s] += (uint64_t)level * G->ms_hcnt->per_src[s];
-      front_deg = (double)G->ms_hcnt->newly_deg;
+      if (tot.newly == 0) break;
+      for (int s = 0; s < 64; ++s) sums[s] += (uint64_t)level * tot.per_src[s];
+      front_deg = (double)tot.newly_deg;
       std::swap(front, next);
     }
     H_ERR(hipEventRecord(G->ev1, st));

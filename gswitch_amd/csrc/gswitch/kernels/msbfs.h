@@ -24,7 +24,13 @@ struct ms_counters_t {
   unsigned long long newly;         // vertices with new bits this level
   unsigned long long newly_deg;     // their degree sum
   unsigned long long per_src[64];   // newly reached vertices per source
+  unsigned long long pad[14];       // one slot = five 128-byte lines
 };
+// Workgroups add their counts into one of MS_SLOTS partial copies (slot
+// blockIdx.x % MS_SLOTS); the host sums the copies. Atomics that every
+// workgroup of the grid sends to the same lines serialise on MI355X.
+constexpr int MS_SLOTS = 64;
+static_assert(sizeof(ms_counters_t) == 640, "slot size");
 
 // per-source counts of the 64 lanes' `bits`; lane s accumulates source s
 __device__ __forceinline__ void ms_count_sources(uint64_t bits,
@@ -37,17 +43,37 @@ __device__ __forceinline__ void ms_count_sources(uint64_t bits,
   }
 }
 
+// Workgroup reduction of the counts, then one set of atomics into the
+// workgroup's slot. Every thread of the workgroup calls it.
 __device__ __forceinline__ void ms_flush(ms_counters_t *c,
                                          unsigned long long newly,
                                          unsigned long long ndeg,
                                          unsigned long long cnt) {
+  constexpr int NW = BLOCK / WAVE;
+  __shared__ unsigned long long s_src[NW][WAVE];
+  __shared__ unsigned long long s_tot[NW][2];
+  const int w = threadIdx.x >> LOG_WAVE, lane = lane_id();
   newly = wave_sum_any(newly);
   ndeg = wave_sum_any(ndeg);
-  if (lane_id() == 0) {
-    if (newly) atomicAdd(&c->newly, newly);
-    if (ndeg) atomicAdd(&c->newly_deg, ndeg);
+  s_src[w][lane] = cnt;
+  if (lane == 0) {
+    s_tot[w][0] = newly;
+    s_tot[w][1] = ndeg;
   }
-  if (cnt) atomicAdd(&c->per_src[lane_id()], cnt);
+  __syncthreads();
+  if (w == 0) {
+    ms_counters_t *slot = c + (blockIdx.x & (MS_SLOTS - 1));
+    unsigned long long t = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) t += s_src[i][lane];
+    if (t) atomicAdd(&slot->per_src[lane], t);
+    if (lane < 2) {
+      unsigned long long x = 0;
+#pragma unroll
+      for (int i = 0; i < NW; ++i) x += s_tot[i][lane];
+      if (x) atomicAdd(lane == 0 ? &slot->newly : &slot->newly_deg, x);
+    }
+  }
 }
 
 // Bottom-up level. Writes next[v] = newly reached bits for every v.

@@ -166,30 +166,6 @@ struct wave_stage_t {
   }
 };
 
-// Wave-aggregated append: every lane with `want` reserves one slot in the
-// array whose length lives at *counter; returns the slot (or -1).
-__device__ __forceinline__ int wave_append(bool want, int *counter) {
-  uint64_t m = ballot(want);
-  if (m == 0) return -1;
-  int leader = __builtin_ctzll(m);
-  int base = 0;
-  if (lane_id() == leader) base = atomicAdd(counter, __popcll(m));
-  base = __shfl(base, leader, 64);
-  return want ? base + rank_below(m) : -1;
-}
-
-// Wave-aggregated append of `cnt` items per lane; returns the lane's first
-// slot. All 64 lanes must call it.
-__device__ __forceinline__ int wave_append_n(int cnt, int *counter) {
-  int tot;
-  int ex = wave_exclusive_scan(cnt, tot);
-  int base = 0;
-  if (tot == 0) return 0;
-  if (lane_id() == 0) base = atomicAdd(counter, tot);
-  base = __shfl(base, 0, 64);
-  return base + ex;
-}
-
 // Index of the last element of sorted `a[0..len)` that is <= x, assuming
 // a[0] <= x (owner lookup for edge-to-vertex mapping).
 template <typename T>
