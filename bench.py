@@ -139,10 +139,11 @@ def bfs_single(gsw, h, root, reps, **kw):
 
 
 def extended(gsw, args):
-    """Single-query BFS at R-MAT 24, 64-source BFS (R-MAT 22 seed 4), a
-    sample of the 20,000-graph ER batch (seed 5), and the learned selector
-    against the rule trees and a fixed strategy. Every comparison checks
-    that the results agree."""
+    """Single-query BFS at R-MAT 24, BFS and CC on the 4096 x 4096 grid
+    (10 % of edges deleted, 1 % long-range, seed 3), 64-source BFS (R-MAT 22
+    seed 4), a sample of the 20,000-graph ER batch (seed 5), and the learned
+    selector against the rule trees and a fixed strategy. Every comparison
+    checks that the results agree."""
     out = {}
     fixed = "Push-Queue-WM-Standalone"
     # --- single query, R-MAT 24
@@ -173,6 +174,50 @@ def extended(gsw, args):
         learned = ms_l
     out["bfs_single"] = single
     del h
+    # --- high-diameter grid: 4096 x 4096, 10 % of edges deleted, 1 % long-range
+    t = time.perf_counter()
+    h = gsw.grid2d(args.grid, args.grid, 0.1, 0.01, 3)
+    log("grid graph: %dx%d n=%d m=%d built in %.1fs" % (args.grid, args.grid, h.n, h.m,
+                                                      time.perf_counter() - t))
+    lvg, ms_g, recs_g, _ = bfs_single(gsw, h, 0, 3)
+    lvg_f, ms_g_fixed, _, _ = bfs_single(gsw, h, 0, 2, configs=fixed)
+    grid = {"n": h.n, "m_undirected": h.m // 2, "levels": len(recs_g),
+            "bfs_ms": round(ms_g, 3),
+            "bfs_gteps": round(graph500_edges(h, lvg) / (ms_g * 1e-3) / 1e9, 2),
+            "bfs_fixed_strategy_ms": round(ms_g_fixed, 3),
+            "bfs_fixed_bit_exact": bool(np.array_equal(lvg, lvg_f))}
+    ms_g_learned = ms_g
+    if learned is not None:
+        lvg_l, ms_g_learned, _, _ = bfs_single(gsw, h, 0, 3, selector=1)
+        grid["bfs_learned_ms"] = round(ms_g_learned, 3)
+        grid["bfs_learned_bit_exact"] = bool(np.array_equal(lvg, lvg_l))
+    g = gsw.Graph(h)
+    lab = g.cc()
+    t_cc = []
+    for _ in range(3):
+        g.cc(out=False)
+        t_cc.append(g.last_stats.device_ms)
+    g.close()
+    grid["cc_ms"] = round(min(t_cc), 3)
+    grid["cc_gteps"] = round((h.m // 2) / (min(t_cc) * 1e-3) / 1e9, 2)
+    # size-independent checks: BFS levels differ by at most one across every
+    # edge and every reached non-root vertex has a parent one level up; CC
+    # labels agree across every edge and each label is its component's
+    # smallest id
+    src = np.repeat(np.arange(h.n, dtype=np.int64), np.diff(h.row_ptr))
+    dst = h.col.astype(np.int64)
+    a, b = lvg[src], lvg[dst]
+    bfs_ok = bool(lvg[0] == 0 and np.all((a < 0) == (b < 0)) and
+                  np.all(np.abs(a - b)[a >= 0] <= 1))
+    has_parent = np.zeros(h.n, bool)
+    has_parent[dst[(a >= 0) & (a == b - 1)]] = True
+    bfs_ok &= bool(np.all(has_parent[(lvg > 0)]))
+    cc_ok = bool(np.all(lab[src] == lab[dst]) and np.all(lab <= np.arange(h.n)) and
+                 np.all(lab[lab] == lab))
+    grid["bfs_invariants"] = bfs_ok
+    grid["cc_invariants"] = cc_ok
+    out["grid"] = grid
+    del h, src, dst, a, b
     # --- 64-source BFS, R-MAT 22 seed 4
     h = gsw.rmat(22, 16, 0.57, 0.19, 0.19, seed=4)
     g = gsw.Graph(h)
@@ -241,11 +286,11 @@ def extended(gsw, args):
                        "bit_exact_vs_per_graph": same}
     g.close()
     # --- aggregate: batched + learned vs plain per-query fixed strategy
-    t_new = ms_batch + ms64 + (learned if learned is not None else ms)
-    t_old = t_loop + t_seq + ms_fixed
+    t_new = ms_batch + ms64 + (learned if learned is not None else ms) + ms_g_learned
+    t_old = t_loop + t_seq + ms_fixed + ms_g_fixed
     out["batched_learned_vs_fixed"] = {
         "speedup": round(t_old / t_new, 2),
-        "note": "same work (ER batch BFS+CC, 64 BFS, one R-MAT BFS); device ms"}
+        "note": "same work (ER batch BFS+CC, 64 BFS, one R-MAT BFS, one grid BFS); device ms"}
     return out
 
 
@@ -262,6 +307,8 @@ def main():
     ap.add_argument("--quick", action="store_true",
                     help="headline workload only (no BFS / batch / selector lines)")
     ap.add_argument("--bfs-scale", type=int, default=24)
+    ap.add_argument("--grid", type=int, default=4096,
+                    help="side of the high-diameter grid")
     ap.add_argument("--batch", type=int, default=1000,
                     help="ER batch sample size (instances 0..batch-1 of 20,000)")
     args = ap.parse_args()

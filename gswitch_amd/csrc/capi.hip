@@ -110,7 +110,11 @@ __global__ void k_bc_backward_init(const bc_fwd_t *fw, bc_bwd_t *bw, int *ra,
     deepest = max(deepest, x.level);
   }
   deepest = gsw::wave_max_any(deepest);
-  if (gsw::lane_id() == 0) atomicMax(max_level, deepest);
+  // read first: most wavefronts hold no new maximum, and atomics from every
+  // wavefront onto one address serialise
+  if (gsw::lane_id() == 0 && deepest > __hip_atomic_load(max_level, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT))
+    atomicMax(max_level, deepest);
 }
 __global__ void k_bc_output(const bc_fwd_t *fw, const bc_bwd_t *bw, int64_t n,
                             int32_t *level, double *nsp, double *dep) {
