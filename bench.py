@@ -139,13 +139,32 @@ def bfs_single(gsw, h, root, reps, **kw):
 
 
 def extended(gsw, args):
-    """Single-query BFS at R-MAT 24, BFS and CC on the 4096 x 4096 grid
+    """Single-query BFS at R-MAT 16 and 24, BFS and CC on the 4096 x 4096 grid
     (10 % of edges deleted, 1 % long-range, seed 3), 64-source BFS (R-MAT 22
     seed 4), a sample of the 20,000-graph ER batch (seed 5), and the learned
     selector against the rule trees and a fixed strategy. Every comparison
     checks that the results agree."""
     out = {}
     fixed = "Push-Queue-WM-Standalone"
+    from gswitch_amd import learn
+    have_model = os.path.exists(learn.MODEL_PATH)
+    if have_model:
+        gsw.selector_load(learn.load_model())
+    # --- single query, R-MAT 16 (the CPU-runnable configuration)
+    h = gsw.rmat(16, 16, 0.57, 0.19, 0.19, seed=1)
+    root = h.max_degree_vertex()
+    lv16, ms16, _, _ = bfs_single(gsw, h, root, 5)
+    lv16_f, ms16_f, _, _ = bfs_single(gsw, h, root, 5, configs=fixed)
+    c1 = {"n": h.n, "m_undirected": h.m // 2, "ms": round(ms16, 3),
+          "gteps": round(graph500_edges(h, lv16) / (ms16 * 1e-3) / 1e9, 2),
+          "fixed_strategy_ms": round(ms16_f, 3),
+          "fixed_bit_exact": bool(np.array_equal(lv16, lv16_f))}
+    if have_model:
+        lv16_l, ms16_l, _, _ = bfs_single(gsw, h, root, 5, selector=1)
+        c1["learned_ms"] = round(ms16_l, 3)
+        c1["learned_bit_exact"] = bool(np.array_equal(lv16, lv16_l))
+    out["bfs_rmat16"] = c1
+    del h
     # --- single query, R-MAT 24
     t = time.perf_counter()
     h = gsw.rmat(args.bfs_scale, 16, 0.57, 0.19, 0.19, seed=1)
@@ -164,10 +183,8 @@ def extended(gsw, args):
               "fixed_strategy_ms": round(ms_fixed, 3),
               "variants": ["%s-%s-%s-%s" % (r["dir"], r["fmt"], r["lb"],
                                             "F" if r["fused"] else "S") for r in recs]}
-    from gswitch_amd import learn
     learned = None
-    if os.path.exists(learn.MODEL_PATH):
-        gsw.selector_load(learn.load_model())
+    if have_model:
         lv2, ms_l, recs_l, _ = bfs_single(gsw, h, root, 5, selector=1)
         single["learned_ms"] = round(ms_l, 3)
         single["learned_bit_exact"] = bool(np.array_equal(lv2, lv))
@@ -406,6 +423,23 @@ def main():
             "sssp_bit_exact": bool(np.array_equal(dist, cdist)),
             "pagerank5_max_rel_err": float(np.max(np.abs(gpr5 - cpr5) / np.abs(cpr5))),
         }
+
+    if not args.quick:
+        # the same SSSP and PageRank under one fixed strategy (no selector)
+        fixed_s, fixed_p = "Push-Queue-WM-Standalone", "Push-Bitmap-WM-Standalone"
+        dist_f = g.sssp(root, configs=fixed_s)
+        s_fixed = g.last_stats.device_ms
+        pr_rule = g.pagerank(DAMPING, PR_ITERS, configs=args.configs_pr)
+        pr_f = g.pagerank(DAMPING, PR_ITERS, configs=fixed_p)
+        p_fixed = g.last_stats.device_ms
+        detail["fixed_strategy"] = {
+            "configs": [fixed_s, fixed_p], "sssp_ms": round(s_fixed, 3),
+            "sssp_bit_exact": bool(np.array_equal(dist_f, dist)),
+            "pagerank_ms": round(p_fixed, 3),
+            "pagerank_max_rel_vs_rule": float(np.max(np.abs(pr_f.astype(np.float64) - pr_rule)
+                                                     / np.abs(pr_rule))),
+            "speedup_rule_vs_fixed": round((s_fixed + p_fixed) /
+                                           ((dev_sssp + dev_pr) / args.steps), 2)}
 
     g.close()
     if not args.quick:
