@@ -104,6 +104,7 @@ _SIGS = {
                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_double)]),
     "gsw_selector_load": (C.c_int32, [_P, C.c_int32]),
+    "gsw_selector_load_app": (C.c_int32, [C.c_int32, _P, C.c_int32]),
     "gsw_mlp_features": (C.c_int32, [_P, _P]),
     "gsw_mlp_forward": (C.c_int32, [_P, _P, C.c_int64, _P]),
     "gsw_mlp_select": (C.c_int32, [_P, _P, C.c_int64, C.c_uint32, _P]),
@@ -113,6 +114,8 @@ _SIGS = {
                                   C.c_float, C.c_uint64, _P, _P]),
     "gsw_explore_bfs": (C.c_int32, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                     _P, _P, _P, _P, C.POINTER(C.c_int32)]),
+    "gsw_explore_sssp": (C.c_int32, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                     _P, _P, _P, _P, C.POINTER(C.c_int32)]),
     "gsw_tree_eval": (C.c_int32, [C.c_char_p, _P, C.c_int32,
                                   C.POINTER(C.c_int32)]),
     "gsw_scan_exclusive_i64": (C.c_int32, [_P, C.c_int64, _P]),
@@ -252,10 +255,16 @@ def _f32(a, n=None):
     return a
 
 
-def selector_load(w):
-    """Install the learned selector's weights (gsw_run_opts.selector = 1)."""
+SELECTOR_APPS = {"bfs": 0, "sssp": 1}
+
+
+def selector_load(w, app="bfs"):
+    """Install an application's learned selector weights, used by its runs
+    with selector=1 ("bfs" or "sssp")."""
+    if app not in SELECTOR_APPS:
+        raise ValueError("no learned selector for %r" % (app,))
     w = _f32(w, MLP_NPARAM)
-    _check(_lib.gsw_selector_load(_ptr(w), MLP_NPARAM))
+    _check(_lib.gsw_selector_load_app(SELECTOR_APPS[app], _ptr(w), MLP_NPARAM))
 
 
 def mlp_features(counts):
@@ -433,14 +442,20 @@ class Graph:
     def explore_bfs(self, root, max_levels=64, stride=1, reps=2):
         """Exhaustive per-level timing of the 30 variant classes for BFS from
         `root`: dict(features[L,12], ms[L,30], rule[L], level[L])."""
+        return self._explore(_lib.gsw_explore_bfs, root, max_levels, stride, reps)
+
+    def explore_sssp(self, root, max_levels=64, stride=1, reps=2):
+        """The same for SSSP from `root` (weighted graphs)."""
+        return self._explore(_lib.gsw_explore_sssp, root, max_levels, stride, reps)
+
+    def _explore(self, fn, root, max_levels, stride, reps):
         feat = np.zeros((max_levels, MLP_NIN), np.float32)
         ms = np.zeros((max_levels, MLP_NOUT), np.float32)
         rule = np.zeros(max_levels, np.int32)
         lvl = np.zeros(max_levels, np.int32)
         k = C.c_int32()
-        _check(_lib.gsw_explore_bfs(self._h, int(root), max_levels, stride, reps,
-                                    _ptr(feat), _ptr(ms), _ptr(rule), _ptr(lvl),
-                                    C.byref(k)))
+        _check(fn(self._h, int(root), max_levels, stride, reps, _ptr(feat), _ptr(ms),
+                  _ptr(rule), _ptr(lvl), C.byref(k)))
         k = k.value
         return dict(features=feat[:k], ms=ms[:k], rule=rule[:k], level=lvl[:k])
 

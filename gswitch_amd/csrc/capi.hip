@@ -274,11 +274,11 @@ run_opts_t to_opts(const gsw_run_opts *o) {
 }
 
 // options of traversals whose variant choice stays with the rule trees (the
-// learned selector is trained on BFS levels)
+// learned selector is trained on BFS and SSSP levels)
 run_opts_t to_opts_rule(const gsw_run_opts *o) {
   run_opts_t r = to_opts(o);
   GSW_REQUIRE(r.selector == 0, GSW_E_UNSUPPORTED,
-              "the learned selector is available for BFS only");
+              "the learned selector is available for BFS and SSSP only");
   return r;
 }
 
@@ -341,18 +341,21 @@ double elapsed(gsw_graph *G) {
 }
 
 // ---- learned selector state (process-wide, one device) ---------------------
-float *g_mlp_dev = nullptr;
-std::vector<float> g_mlp_host;
+// one network per application (GSW_APP_*), each trained on that
+// application's own per-level timings
+float *g_mlp_dev[GSW_APPS] = {};
+std::vector<float> g_mlp_host[GSW_APPS];
 
 template <typename F>
 void maybe_learned(app_ctx_t<F> &c, const run_opts_t &o, const gsw_graph *G,
-                   bool fromall, bool toall) {
+                   bool fromall, bool toall, int app) {
   if (o.selector == 0) return;
   GSW_REQUIRE(o.selector == 1, GSW_E_INVALID, "unknown selector");
-  GSW_REQUIRE(g_mlp_dev, GSW_E_INVALID,
-              "learned selector requested but no weights are loaded");
-  enable_learned(c.en, g_mlp_dev, g_mlp_host.data(), G->attr, (double)G->hg.n,
-                 (double)G->hg.m, fromall, toall);
+  GSW_REQUIRE(g_mlp_dev[app], GSW_E_INVALID,
+              "learned selector requested but no weights are loaded for this "
+              "application");
+  enable_learned(c.en, g_mlp_dev[app], g_mlp_host[app].data(), G->attr,
+                 (double)G->hg.n, (double)G->hg.m, fromall, toall);
 }
 
 app_ctx_t<BFS> &bfs_ctx(gsw_graph *G) {
@@ -422,7 +425,7 @@ int bfs_engine_run(gsw_graph *G, const gsw_run_opts *opts, int32_t root,
   c.en.conf.conf_pruning = true;
   view_t<GU> gv{&G->hg, G->attr, 0.0};
   init_conf(c.en.stats, fets, c.en.conf, gv, c.f);
-  maybe_learned(c, o, G, false, false);
+  maybe_learned(c, o, G, false, false, GSW_APP_BFS);
 
   H_ERR(hipEventRecord(G->ev0, gsw::stream()));
   c.as.reset();
@@ -546,6 +549,89 @@ ins_t ins_of_class(int cls) {
 // Fresh per-level engine state for a BFS level probe (forced class `cls`,
 // or the rule selector when cls < 0): the level is inspected as a
 // standalone level whatever came before it.
+app_ctx_t<SSSP> &sssp_ctx(gsw_graph *G) {
+  if (!G->sssp) {
+    G->sssp = std::make_unique<app_ctx_t<SSSP>>();
+    G->sssp->as.build(G->hg.n, Normal);
+    G->sssp->f.data.build(G->hg.n);
+  }
+  return *G->sssp;
+}
+
+// The SSSP engine before its first level (run_sssp, application/sssp.cu:
+// 106-126): Mono pattern, to-all, window sized by init_conf.
+void sssp_prepare(gsw_graph *G, app_ctx_t<SSSP> &c, const run_opts_t &o,
+                  int root, bool profile) {
+  prepare_engine(c, o, profile);
+  feature_t &fets = c.en.fets;
+  fets.centric = VC;
+  fets.pattern = Mono;
+  fets.fromall = false;
+  fets.toall = true;
+  fets.use_root = root;
+  fets.set_graph((double)G->hg.n, (double)(G->hg.m / 2), G->attr,
+                 G->mean_weight);
+  c.en.conf.conf_window = true;
+  c.f.data.window = window_t{};
+  view_t<GW> gv{&G->hg, G->attr, G->mean_weight};
+  init_conf(c.en.stats, fets, c.en.conf, gv, c.f);
+}
+
+// The engine for SSSP level `lvl` in the state the level loop carries into
+// it (features of the previous level, the window), with class `cls` forced
+// or the rule selector (cls < 0). The level is inspected as a standalone
+// level whatever came before it; level 0 is the first round (the root).
+void sssp_level_setup(gsw_graph *G, app_ctx_t<SSSP> &c, int root, int cls,
+                      int lvl, const feature_t &fets, const window_t &win) {
+  run_opts_t o;
+  if (cls >= 0) o.force = ins_of_class(cls);
+  sssp_prepare(G, c, o, root, false);
+  c.en.fets = fets;
+  c.f.data.window = win;
+  c.en.conf.conf_first_round = (lvl == 0);
+  c.en.conf.conf_fuse_inspect = false;
+  c.en.conf.conf_fusion = false;
+  c.as.reset();
+  if (lvl == 0) c.as.init(root);
+}
+
+// Cost of SSSP level `lvl` with class `cls` from the current distance
+// arrays, attributed as bfs_probe_level attributes BFS levels.
+double sssp_probe_level(gsw_graph *G, app_ctx_t<SSSP> &c, int root, int cls,
+                        int lvl, const feature_t &fets, const window_t &win) {
+  sssp_level_setup(G, c, root, cls, lvl, fets, win);
+  engine_t &en = c.en;
+  GW g = G->dgw;
+  g.level = lvl;
+  const bool fused = gsw_mlp::variant_of_class(cls).fused;
+  en.inspector.inspect(c.as, g, c.f, en.stats, en.fets, en.conf);
+  if (c.as.finish(g, c.f, en.conf)) return 0.0;
+  gsw::sync();
+  double t0 = gsw_now_ms();
+  en.selector.select(en.stats, en.fets, en.conf);
+  en.executor.filter(c.as, g, c.f, en.stats, en.fets, en.conf);
+  if (fused) {
+    gsw::sync();
+    t0 = gsw_now_ms();
+  }
+  g.update_level();
+  en.executor.profile = true;
+  en.executor.expand(c.as, g, c.f, en.stats, en.fets, en.conf);
+  if (!fused) {
+    en.inspector.inspect(c.as, g, c.f, en.stats, en.fets, en.conf);
+  } else {
+    const double nq = (double)en.inspector.known_active;
+    const double fr = (double)en.executor.log.back().frontier;
+    if (nq > 0 && nq * gsw_mlp::INSPECT_FRACTION >= (double)c.as.size &&
+        nq >= gsw_mlp::INSPECT_GROWTH * fr) {
+      en.conf.conf_fuse_inspect = false;
+      en.inspector.inspect(c.as, g, c.f, en.stats, en.fets, en.conf);
+    }
+  }
+  gsw::sync();
+  return gsw_now_ms() - t0;
+}
+
 void bfs_level_setup(gsw_graph *G, app_ctx_t<BFS> &c, int root, int cls) {
   run_opts_t o;
   if (cls >= 0) o.force = ins_of_class(cls);
@@ -1076,27 +1162,11 @@ gsw_status gsw_sssp(gsw_graph *G, int32_t root, const gsw_run_opts *opts,
     GSW_REQUIRE(G, GSW_E_INVALID, "null graph");
     GSW_REQUIRE(G->has_w, GSW_E_INVALID, "SSSP needs a weighted graph");
     GSW_REQUIRE(root >= 0 && root < G->hg.n, GSW_E_INVALID, "root out of range");
-    const run_opts_t o = to_opts_rule(opts);
-    if (!G->sssp) {
-      G->sssp = std::make_unique<app_ctx_t<SSSP>>();
-      G->sssp->as.build(G->hg.n, Normal);
-      G->sssp->f.data.build(G->hg.n);
-    }
-    auto &c = *G->sssp;
+    const run_opts_t o = to_opts(opts);
+    auto &c = sssp_ctx(G);
     arch_once(c);
-    prepare_engine(c, o, !opts || opts->profile);
-    feature_t &fets = c.en.fets;
-    fets.centric = VC;
-    fets.pattern = Mono;
-    fets.fromall = false;
-    fets.toall = true;
-    fets.use_root = root;
-    fets.set_graph((double)G->hg.n, (double)(G->hg.m / 2), G->attr,
-                   G->mean_weight);
-    c.en.conf.conf_window = true;
-    c.f.data.window = window_t{};
-    view_t<GW> gv{&G->hg, G->attr, G->mean_weight};
-    init_conf(c.en.stats, fets, c.en.conf, gv, c.f);
+    sssp_prepare(G, c, o, root, !opts || opts->profile);
+    maybe_learned(c, o, G, false, true, GSW_APP_SSSP);
 
     hipStream_t st = gsw::stream();
     H_ERR(hipEventRecord(G->ev0, st));
@@ -1338,19 +1408,24 @@ gsw_status gsw_msbfs64(gsw_graph *G, const int32_t *sources, int32_t nsrc,
 }
 
 // ---- learned selector -------------------------------------------------------
-gsw_status gsw_selector_load(const float *w, int32_t nparam) {
+gsw_status gsw_selector_load_app(int32_t app, const float *w, int32_t nparam) {
   return guarded([&] {
+    GSW_REQUIRE(app >= 0 && app < GSW_APPS, GSW_E_INVALID, "unknown application");
     GSW_REQUIRE(w && nparam == gsw_mlp::NPARAM, GSW_E_INVALID,
                 "expected 6942 weights");
     require_gpu();
     for (int32_t i = 0; i < nparam; ++i)
       GSW_REQUIRE(std::isfinite(w[i]), GSW_E_INVALID, "non-finite weight");
-    if (!g_mlp_dev)
-      H_ERR(hipMalloc((void **)&g_mlp_dev, sizeof(float) * gsw_mlp::NPARAM));
-    H_ERR(hipMemcpy(g_mlp_dev, w, sizeof(float) * gsw_mlp::NPARAM,
+    if (!g_mlp_dev[app])
+      H_ERR(hipMalloc((void **)&g_mlp_dev[app], sizeof(float) * gsw_mlp::NPARAM));
+    H_ERR(hipMemcpy(g_mlp_dev[app], w, sizeof(float) * gsw_mlp::NPARAM,
                     hipMemcpyHostToDevice));
-    g_mlp_host.assign(w, w + nparam);
+    g_mlp_host[app].assign(w, w + nparam);
   });
+}
+
+gsw_status gsw_selector_load(const float *w, int32_t nparam) {
+  return gsw_selector_load_app(GSW_APP_BFS, w, nparam);
 }
 
 gsw_status gsw_mlp_features(const double *counts, float *x) {
@@ -1626,6 +1701,106 @@ gsw_status gsw_explore_bfs(gsw_graph *G, int32_t root, int32_t max_levels,
       }
       // every class leaves the same levels array: the state of level lvl+1
       ++written;
+    }
+    *n_out = written;
+  });
+}
+
+gsw_status gsw_explore_sssp(gsw_graph *G, int32_t root, int32_t max_levels,
+                            int32_t stride, int32_t reps, float *feat_out,
+                            float *ms_out, int32_t *rule_out, int32_t *level_out,
+                            int32_t *n_out) {
+  return guarded([&] {
+    using namespace gsw_mlp;
+    GSW_REQUIRE(G && n_out && max_levels >= 1 && stride >= 1 && reps >= 1,
+                GSW_E_INVALID, "invalid arguments");
+    GSW_REQUIRE(G->has_w, GSW_E_INVALID, "SSSP needs a weighted graph");
+    GSW_REQUIRE(root >= 0 && root < G->hg.n, GSW_E_INVALID, "root out of range");
+    auto &c = sssp_ctx(G);
+    arch_once(c);
+    const int64_t n = G->hg.n;
+    dev_bufs_t d;
+    uint32_t *snap_wa = d.get<uint32_t>((size_t)n);
+    uint32_t *snap_ra = d.get<uint32_t>((size_t)n);
+    hipStream_t st = gsw::stream();
+    // the state the level loop starts from
+    sssp_prepare(G, c, run_opts_t{}, root, false);
+    feature_t fets = c.en.fets;
+    window_t win = c.f.data.window;
+    hipLaunchKernelGGL(k_init_sssp, dim3(gsw::grid_for(n)), dim3(gsw::BLOCK), 0, st,
+                       c.f.data.dg_wa, c.f.data.dg_ra, n, root);
+    H_ERR(hipGetLastError());
+    auto save = [&] {
+      H_ERR(hipMemcpyAsync(snap_wa, c.f.data.dg_wa, sizeof(uint32_t) * n,
+                           hipMemcpyDeviceToDevice, st));
+      H_ERR(hipMemcpyAsync(snap_ra, c.f.data.dg_ra, sizeof(uint32_t) * n,
+                           hipMemcpyDeviceToDevice, st));
+    };
+    auto restore = [&] {
+      H_ERR(hipMemcpyAsync(c.f.data.dg_wa, snap_wa, sizeof(uint32_t) * n,
+                           hipMemcpyDeviceToDevice, st));
+      H_ERR(hipMemcpyAsync(c.f.data.dg_ra, snap_ra, sizeof(uint32_t) * n,
+                           hipMemcpyDeviceToDevice, st));
+    };
+    int written = 0;
+    for (int lvl = 0; lvl < (1 << 20) && written < max_levels; ++lvl) {
+      save();
+      // the level as the rule selector runs it
+      sssp_level_setup(G, c, root, -1, lvl, fets, win);
+      GW g = G->dgw;
+      g.level = lvl;
+      c.en.inspector.inspect(c.as, g, c.f, c.en.stats, c.en.fets, c.en.conf);
+      if (c.as.finish(g, c.f, c.en.conf)) break;  // no pending update left
+      if (lvl > 0 && c.en.fets.active_vertex <= 0) {
+        // the window advanced over a drained frontier: the next level
+        // inspects with the new window
+        fets = c.en.fets;
+        win = c.f.data.window;
+        continue;
+      }
+      if ((lvl % stride) == 0) {
+        const feature_t lf = c.en.fets;
+        counts_t k;
+        k.n = (float)n;
+        k.m = (float)G->hg.m;
+        k.avg_deg = (float)G->attr.avg_deg;
+        k.std_deg = (float)G->attr.std_deg;
+        k.gini = (float)G->attr.GI;
+        k.level = (float)lvl;
+        k.active = (float)lf.active_vertex;
+        k.inactive = (float)lf.unactive_vertex;
+        k.push_wl = (float)lf.push_workload;
+        k.pull_wl = (float)lf.pull_workload;
+        k.max_deg_active = (float)lf.cur_max_deg_active;
+        if (feat_out) features(k, feat_out + (size_t)written * NIN);
+        c.en.selector.select(c.en.stats, c.en.fets, c.en.conf);
+        const config_t &rc = c.en.conf;
+        if (rule_out)
+          rule_out[written] = class_of(rc.conf_dir, rc.conf_asfmt, rc.conf_sorted,
+                                       rc.conf_fuse_inspect, rc.conf_lb);
+        if (level_out) level_out[written] = lvl;
+        for (int cls = 0; cls < NOUT; ++cls) {
+          double best = 1e300;
+          for (int r = 0; r < reps; ++r) {
+            restore();
+            best = std::min(best, sssp_probe_level(G, c, root, cls, lvl, fets, win));
+          }
+          if (ms_out) ms_out[(size_t)written * NOUT + cls] = (float)best;
+        }
+        ++written;
+        // back to the level's start for the rule's run of it
+        restore();
+        sssp_level_setup(G, c, root, -1, lvl, fets, win);
+        c.en.inspector.inspect(c.as, g, c.f, c.en.stats, c.en.fets, c.en.conf);
+        c.as.finish(g, c.f, c.en.conf);
+      }
+      c.en.selector.select(c.en.stats, c.en.fets, c.en.conf);
+      c.en.executor.filter(c.as, g, c.f, c.en.stats, c.en.fets, c.en.conf);
+      g.update_level();
+      c.en.executor.expand(c.as, g, c.f, c.en.stats, c.en.fets, c.en.conf);
+      gsw::sync();
+      fets = c.en.fets;
+      win = c.f.data.window;
     }
     *n_out = written;
   });

@@ -113,10 +113,14 @@ struct executor_t {
       if (tofilter) {
         if (!(as.queue.mode == Cached && as.queue.traceback))
           run_filter(as, conf);
-      } else if (!conf.conf_pruning) {
+      }
+      // a fused level of a non-idempotent algorithm de-duplicates its
+      // pushes within the level only: start from a clean `visited`, also on
+      // the level that switches into fusion (an earlier fused phase may have
+      // left marks behind)
+      if (conf.conf_fuse_inspect && !conf.conf_pruning)
         gsw::fill_async<uint64_t>(as.bitmap.visited.dg_bits, 0,
                                   as.bitmap.visited.n_words + 1);
-      }
       as.swap_queues();
     }
     conf.thdnum = gsw::BLOCK;
@@ -238,6 +242,7 @@ struct executor_t {
         if (nq > 0 && (double)nq * gsw_mlp::INSPECT_FRACTION >= (double)as.size &&
             (double)nq >= gsw_mlp::INSPECT_GROWTH * (double)conf.mlp_fused_ref) {
           conf.conf_fuse_inspect = false;
+          conf.conf_fusion = false;
           as.swap_queues();
         }
       }

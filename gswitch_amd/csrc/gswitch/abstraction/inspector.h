@@ -52,8 +52,14 @@ struct inspector_t {
   void inspect(active_set_t &as, device_graph_t<CSR, E> &g, F &f,
                stat_t &stat, feature_t &fets, config_t &conf) {
     stat.last_filter_time = gsw_now_ms();
-    f.data.window.adjust(fets.active_vertex, fets.cur_avg_deg_active,
-                         conf.conf_fusion);
+    // The window moves only on inspected levels: a fused phase keeps the
+    // window its queue was built with, so every in-window update is queued
+    // and finish() may absorb the stale marks when the phase drains. (The
+    // rule trees' fused phases sit at the window limit anyway, where the
+    // reference's per-level adjust changes nothing.)
+    if (!conf.conf_fuse_inspect)
+      f.data.window.adjust(fets.active_vertex, fets.cur_avg_deg_active,
+                           conf.conf_fusion);
     bool need = !conf.conf_fuse_inspect;
     const bool pre = conf.conf_pre_inspected;
     conf.conf_pre_inspected = false;

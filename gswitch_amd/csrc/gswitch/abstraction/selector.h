@@ -184,6 +184,10 @@ struct selector_t {
       conf.conf_asfmt = v.fmt;
       conf.conf_sorted = v.sorted;
     }
+    // the fusion mode flag follows the choice: with a stepping window, a
+    // drained fused frontier must re-filter every vertex and leave fusion
+    // before the window advances (active_set_t::finish)
+    conf.conf_fusion = conf.conf_fuse_inspect;
     conf.conf_lb = v.lb;
     if (conf.force.has_direction) conf.conf_dir = conf.force.d;
     if (conf.force.has_asfmt) conf.conf_asfmt = conf.force.asf;

@@ -81,11 +81,11 @@ inline int class_of(Direction dir, ASFmt fmt, bool sorted, bool fused, LB lb) {
   return g * NLB + index_of_lb(lb);
 }
 
-// classes that make sense for an algorithm: from-all / to-all traversals
-// and pulls have no fused form
+// classes that make sense for an algorithm: a from-all and to-all one
+// (PageRank-like) has no fused form, as init_conf keeps it standalone
 inline uint32_t valid_mask(bool fromall, bool toall) {
   uint32_t m = ALL_CLASSES;
-  if (fromall || toall) m &= ~(((1u << NLB) - 1) << (3 * NLB));
+  if (fromall && toall) m &= ~(((1u << NLB) - 1) << (3 * NLB));
   return m;
 }
 

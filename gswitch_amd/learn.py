@@ -1,21 +1,22 @@
-"""Training pipeline of the learned variant selector (BFS).
+"""Training pipeline of the learned variant selectors (BFS and SSSP, one
+network each).
 
-1. `collect`: the engine times all 30 variant classes at every probed BFS
-   level of seeded synthetic graphs (gsw_explore_bfs: snapshot, run each
-   class, restore) and records the level's 12 features and the class the
-   rule trees would pick.
+1. `collect`: the engine times all 30 variant classes at every probed level
+   of seeded synthetic graphs (gsw_explore_bfs / gsw_explore_sssp: snapshot,
+   run each class, restore) and records the level's 12 features and the
+   class the rule trees would pick.
 2. `train`: targets are log2(time / fastest time) per class, clipped at
    CLIP; the 12 -> 64 -> 64 -> 30 network is fitted by the library's own
    HIP forward / backward / Adam kernels (gsw_mlp_train).
 3. `evaluate`: on held-out graphs, per-level regret of the network's and
    the rule trees' choices against the exhaustive optimum, and end-to-end
-   BFS time under the rule trees, the network, and fixed strategies.
+   time under the rule trees, the network, and fixed strategies.
 
-Run as a module on a GPU machine:
-    python -m gswitch_amd.learn collect --set train --out DIR
-    python -m gswitch_amd.learn train --data DIR/train.npz --out DIR
-    python -m gswitch_amd.learn collect --set eval --out DIR
-    python -m gswitch_amd.learn evaluate --data DIR/eval.npz --model M --out DIR
+Run as a module on a GPU machine (--app bfs | sssp, default bfs):
+    python -m gswitch_amd.learn collect --app A --set train --out DIR
+    python -m gswitch_amd.learn train --app A --data DIR/train.npz --out DIR
+    python -m gswitch_amd.learn collect --app A --set eval --out DIR
+    python -m gswitch_amd.learn evaluate --app A --data DIR/eval.npz --model M --out DIR
 """
 import argparse
 import json
@@ -28,7 +29,14 @@ import numpy as np
 import gswitch_amd as gsw
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-MODEL_PATH = os.path.join(HERE, "model", "selector_bfs.npz")
+APPS = ("bfs", "sssp")
+
+
+def model_path(app="bfs"):
+    return os.path.join(HERE, "model", "selector_%s.npz" % app)
+
+
+MODEL_PATH = model_path("bfs")
 CLIP = 6.0
 TRAIN_SEED, EVAL_SEED = 20240501, 20240777
 
@@ -85,17 +93,24 @@ def eval_specs(seed=EVAL_SEED):
     return out
 
 
-def build(spec):
+def build(spec, weighted=False):
     k = spec["kind"]
     if k == "rmat":
         return gsw.rmat(spec["scale"], spec["ef"], spec["a"], spec["b"], spec["b"],
-                        seed=spec["seed"])
+                        seed=spec["seed"], weighted=weighted)
     if k == "ba":
-        return gsw.barabasi_albert(spec["n"], spec["m"], seed=spec["seed"])
+        return gsw.barabasi_albert(spec["n"], spec["m"], seed=spec["seed"],
+                                   weighted=weighted)
     if k == "er":
-        return gsw.erdos_renyi(spec["n"], spec["d"], seed=spec["seed"])
+        return gsw.erdos_renyi(spec["n"], spec["d"], seed=spec["seed"],
+                               weighted=weighted)
     return gsw.grid2d(spec["rows"], spec["cols"], spec["p"], spec["lr"],
-                      seed=spec["seed"])
+                      seed=spec["seed"], weighted=weighted)
+
+
+def run(g, app, root, **kw):
+    """One traversal of `app` from `root` (levels or distances)."""
+    return g.bfs(root, **kw) if app == "bfs" else g.sssp(root, **kw)
 
 
 def pick_root(spec, h):
@@ -128,24 +143,29 @@ def class_of_record(r):
     return gsw.MLP_CLASSES.index(name)
 
 
-def rule_trajectory(g, root):
-    """The rule selector's actual class per BFS level (fused levels
-    included), keyed by the level of the expanded frontier."""
-    g.bfs(root, out=False, profile=True)
+def rule_trajectory(g, root, app="bfs"):
+    """The rule selector's actual class per level (fused levels included),
+    keyed by the level of the expanded frontier."""
+    run(g, app, root, out=False, profile=True)
     return {r["level"] - 1: class_of_record(r) for r in g.records()}
 
 
-def collect(specs, reps=2, max_levels=32, log=print):
+def collect(specs, reps=2, max_levels=32, log=print, app="bfs"):
     rows = dict(features=[], ms=[], rule=[], level=[], graph=[])
     t0 = time.time()
     for gi, spec in enumerate(specs):
-        h = build(spec)
+        h = build(spec, weighted=(app == "sssp"))
         g = gsw.Graph(h)
         root = pick_root(spec, h)
-        depth = int(g.bfs(root).max()) + 1
-        traj = rule_trajectory(g, root)
-        ex = g.explore_bfs(root, max_levels=max_levels,
-                           stride=level_stride(depth, max_levels), reps=reps)
+        if app == "bfs":
+            depth = int(g.bfs(root).max()) + 1
+        else:
+            g.sssp(root, out=False)
+            depth = int(g.last_stats.levels) + 1
+        traj = rule_trajectory(g, root, app)
+        explore = g.explore_bfs if app == "bfs" else g.explore_sssp
+        ex = explore(root, max_levels=max_levels,
+                     stride=level_stride(depth, max_levels), reps=reps)
         # the rule's class in its own run (fused state included) where known
         ex["rule"] = np.array([traj.get(int(l), int(c)) for l, c in
                                zip(ex["level"], ex["rule"])], np.int32)
@@ -205,24 +225,24 @@ def load_model(path=MODEL_PATH):
 FIXED = ["Push-Queue-WM-Standalone", "Push-Queue-WM-Fused", None]
 
 
-def end_to_end(specs, w, reps=3, log=print):
-    """Median BFS device time per graph under the rule trees, the network
-    and fixed strategies (results checked identical)."""
-    gsw.selector_load(w)
+def end_to_end(specs, w, reps=3, log=print, app="bfs"):
+    """Median device time per graph under the rule trees, the network and
+    fixed strategies (results checked identical)."""
+    gsw.selector_load(w, app=app)
     out = []
     for spec in specs:
-        h = build(spec)
+        h = build(spec, weighted=(app == "sssp"))
         g = gsw.Graph(h)
         root = pick_root(spec, h)
-        ref = g.bfs(root)
+        ref = run(g, app, root)
         rec = dict(kind=spec["kind"], n=h.n, m=h.m)
         for name, kw in [("rule", {}), ("mlp", dict(selector=1)),
                          ("fixed_wm", dict(configs=FIXED[0])),
                          ("fixed_fused", dict(configs=FIXED[1]))]:
             ts = []
             for _ in range(reps):
-                lv = g.bfs(root, profile=False, **kw)
-                assert np.array_equal(lv, ref), (name, spec)
+                res = run(g, app, root, profile=False, **kw)
+                assert np.array_equal(res, ref), (name, spec)
                 ts.append(g.last_stats.device_ms)
             rec[name] = float(np.median(ts))
         g.close()
@@ -236,12 +256,14 @@ def end_to_end(specs, w, reps=3, log=print):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("cmd", choices=["collect", "train", "evaluate"])
+    ap.add_argument("--app", default="bfs", choices=APPS)
     ap.add_argument("--set", default="train", choices=["train", "eval"])
     ap.add_argument("--count", type=int, default=300)
     ap.add_argument("--start", type=int, default=0,
                     help="collect specs[start:start+count] of the set")
     ap.add_argument("--data")
-    ap.add_argument("--model", default=MODEL_PATH)
+    ap.add_argument("--model", default=None,
+                    help="model file (default: the committed one of --app)")
     ap.add_argument("--out", default="gpurun_out/selector")
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--epochs", type=int, default=400)
@@ -251,7 +273,7 @@ def main(argv=None):
     if a.cmd == "collect":
         specs = training_specs() if a.set == "train" else eval_specs()
         specs = specs[a.start:a.start + a.count]
-        d = collect(specs, reps=a.reps, log=log)
+        d = collect(specs, reps=a.reps, log=log, app=a.app)
         name = a.set if a.start == 0 else "%s_%d" % (a.set, a.start)
         np.savez(os.path.join(a.out, name + ".npz"), specs=json.dumps(specs), **d)
     elif a.cmd == "train":
@@ -262,20 +284,21 @@ def main(argv=None):
         cls = gsw.mlp_select(w, d["features"])
         log("train regret: mlp %.3f rule %.3f" % (per_level_regret(d["ms"], cls),
                                                   per_level_regret(d["ms"], d["rule"])))
-        save_model(os.path.join(a.out, "selector_bfs.npz"), w,
-                   dict(samples=int(len(d["ms"])), epochs=a.epochs,
+        save_model(os.path.join(a.out, "selector_%s.npz" % a.app), w,
+                   dict(app=a.app, samples=int(len(d["ms"])), epochs=a.epochs,
                         loss_first=float(curve[0]), loss_last=float(curve[-1])))
     else:
         with np.load(a.data, allow_pickle=False) as z:
             d = {k: z[k] for k in z.files}
-        w = load_model(a.model)
+        w = load_model(a.model or model_path(a.app))
         cls = gsw.mlp_select(w, d["features"])
         res = dict(levels=int(len(d["ms"])),
                    regret_mlp=per_level_regret(d["ms"], cls),
                    regret_rule=per_level_regret(d["ms"], d["rule"]))
         log("eval per-level regret: mlp %.3f rule %.3f" % (res["regret_mlp"],
                                                            res["regret_rule"]))
-        res["end_to_end"] = end_to_end(json.loads(str(d["specs"])), w, log=log)
+        res["end_to_end"] = end_to_end(json.loads(str(d["specs"])), w, log=log,
+                                       app=a.app)
         with open(os.path.join(a.out, "eval.json"), "w") as f:
             json.dump(res, f, indent=1)
 

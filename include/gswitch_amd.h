@@ -216,7 +216,13 @@ gsw_status gsw_tree_eval(const char *name, const double *fv, int32_t nfv,
 #define GSW_MLP_NPARAM 6942
 #define GSW_MLP_NIN 12
 #define GSW_MLP_NOUT 30
-/* Install the weights used when gsw_run_opts.selector == 1. */
+/* Applications with a learned selector of their own. */
+#define GSW_APP_BFS 0
+#define GSW_APP_SSSP 1
+#define GSW_APPS 2
+/* Install the weights that `app`'s runs use when gsw_run_opts.selector == 1. */
+gsw_status gsw_selector_load_app(int32_t app, const float *w, int32_t nparam);
+/* = gsw_selector_load_app(GSW_APP_BFS, w, nparam). */
 gsw_status gsw_selector_load(const float *w, int32_t nparam);
 /* The 12 inputs from counts[11] = {n, m, avg deg, std deg, Gini, level,
  * active, inactive, push workload, pull workload, max active degree}:
@@ -248,6 +254,13 @@ gsw_status gsw_mlp_train(const float *x, const float *t, const float *mask,
  * next queue length), and the class the rule selector would pick.
  * *n_out = levels written. */
 gsw_status gsw_explore_bfs(gsw_graph *g, int32_t root, int32_t max_levels,
+                           int32_t stride, int32_t reps, float *feat_out,
+                           float *ms_out, int32_t *rule_out, int32_t *level_out,
+                           int32_t *n_out);
+/* The same for SSSP from `root` on a weighted graph: the state carried from
+ * level to level is the two distance arrays, the stepping window and the
+ * features; levels where only the window advances are not probed. */
+gsw_status gsw_explore_sssp(gsw_graph *g, int32_t root, int32_t max_levels,
                            int32_t stride, int32_t reps, float *feat_out,
                            float *ms_out, int32_t *rule_out, int32_t *level_out,
                            int32_t *n_out);

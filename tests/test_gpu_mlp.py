@@ -137,14 +137,61 @@ def test_learned_policy_bit_exact(gsw, orc, name, make):
 def test_learned_policy_errors(gsw):
     h = gsw.rmat(10, 8, seed=3, weighted=True)
     g = gsw.Graph(h)
+    with pytest.raises(gsw.GswError):  # no learned selector for CC / PageRank
+        g.cc(selector=1)
     with pytest.raises(gsw.GswError):
-        g.sssp(0, selector=1)
+        g.pagerank(0.85, 3, selector=1)
     with pytest.raises(ValueError):
         gsw.selector_load(np.zeros(10, np.float32))
+    with pytest.raises(ValueError):
+        gsw.selector_load(np.zeros(NP, np.float32), app="pagerank")
     bad = np.zeros(NP, np.float32)
     assert gsw._lib.gsw_selector_load(bad.ctypes.data, 10) == 1  # GSW_E_INVALID
+    assert gsw._lib.gsw_selector_load_app(2, bad.ctypes.data, NP) == 1
     bad[7] = np.nan
     assert gsw._lib.gsw_selector_load(bad.ctypes.data, NP) == 1
+
+
+SSSP_GRAPHS = [("rmat", lambda gsw: gsw.rmat(12, 16, seed=2, weighted=True)),
+               ("grid", lambda gsw: gsw.grid2d(50, 50, 0.1, 0.01, 3, weighted=True)),
+               ("er", lambda gsw: gsw.erdos_renyi(4000, 4.0, 8, weighted=True))]
+
+
+@pytest.mark.parametrize("name,make", SSSP_GRAPHS)
+def test_learned_sssp_bit_exact(gsw, orc, name, make):
+    h = make(gsw)
+    g = gsw.Graph(h)
+    root = h.max_degree_vertex()
+    exp = orc.spfa(h.row_ptr, h.col, h.w, root)
+    seen = set()
+    for seed in range(6):  # random networks visit many classes and switches
+        w = np.random.default_rng(100 + seed).normal(0, 0.5, NP).astype(np.float32)
+        gsw.selector_load(w, app="sssp")
+        assert np.array_equal(g.sssp(root, selector=1), exp), seed
+        for r in g.records():
+            seen.add((r["dir"], r["fmt"], r["lb"], r["fused"]))
+    assert len(seen) >= 3
+
+
+def test_explore_sssp(gsw, orc):
+    h = gsw.rmat(11, 16, seed=2, weighted=True)
+    g = gsw.Graph(h)
+    root = h.max_degree_vertex()
+    ex = g.explore_sssp(root, max_levels=64, stride=1, reps=1)
+    L = len(ex["level"])
+    assert L >= 3 and np.all(np.diff(ex["level"]) > 0) and ex["level"][0] == 0
+    assert np.all(ex["ms"] > 0) and np.isfinite(ex["ms"]).all()
+    assert np.all((ex["rule"] >= 0) & (ex["rule"] < 30))
+    assert np.isfinite(ex["features"]).all()
+    # level 0 is the root alone: active = 1, push workload = its degree
+    c = (h.n, h.m, h.degrees.mean(), h.degrees.std(ddof=1),
+         gsw.degree_stats(h.row_ptr)["GI"], 0, 1, h.n - 1, h.degrees[root],
+         h.m - h.degrees[root], h.degrees[root])
+    x0 = gsw.mlp_features(c)
+    for j in (0, 1, 2, 3, 4, 5, 6, 8, 10, 11):
+        np.testing.assert_allclose(ex["features"][0][j], x0[j], rtol=1e-5, atol=1e-6)
+    # the probes leave the graph usable: a later run is still bit-exact
+    assert np.array_equal(g.sssp(root), orc.spfa(h.row_ptr, h.col, h.w, root))
 
 
 def test_trained_model_file(gsw, orc):
