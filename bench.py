@@ -5,8 +5,10 @@ Workload (one "step"): SSSP from the highest-degree vertex plus a
 50-iteration PageRank (damping 0.85) on R-MAT scale 22, edge factor 16,
 (a, b, c, d) = (0.57, 0.19, 0.19, 0.05), seed 2, symmetric int32 CSR with
 uint32 weights in [1, 64] -- the SSSP / PageRank configuration of the
-engine's headline metric. The graph is resident in HBM before the timed
-region; runs keep their results on the device.
+engine's headline metric. SSSP picks its variants with the learned selector
+(the committed network; --rule-only for the rule trees), PageRank with the
+rule trees. The graph is resident in HBM before the timed region; runs keep
+their results on the device.
 
 GTEPS (Graph500 convention): SSSP counts sum(deg(v), v reached)/2,
 PageRank counts m_undirected per iteration; value = total / wall time.
@@ -319,6 +321,8 @@ def main():
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--rule-only", action="store_true",
+                    help="headline SSSP under the rule trees even with a trained network")
     ap.add_argument("--configs-sssp", default=None)
     ap.add_argument("--configs-pr", default=None)
     ap.add_argument("--quick", action="store_true",
@@ -349,8 +353,17 @@ def main():
     copy_gbs = gsw.stream_copy(1 << 30, 10)
     log("stream copy: %.0f GB/s" % copy_gbs)
 
+    # SSSP runs under its learned selector when a trained network is
+    # committed (and no variant is forced); PageRank keeps the rule trees
+    from gswitch_amd import learn
+    sel = {}
+    if os.path.exists(learn.model_path("sssp")) and args.configs_sssp is None \
+            and not args.rule_only:
+        gsw.selector_load(learn.load_model(learn.model_path("sssp")), app="sssp")
+        sel = {"selector": 1}
+
     def step():
-        g.sssp(root, configs=args.configs_sssp, out=False)
+        g.sssp(root, configs=args.configs_sssp, out=False, **sel)
         s_ms = g.last_stats.device_ms
         s_rec = g.records()
         g.pagerank(DAMPING, PR_ITERS, configs=args.configs_pr, out=False)
@@ -364,7 +377,7 @@ def main():
     dev_sssp = dev_pr = 0.0
     for _ in range(args.steps):
         t0 = time.perf_counter()
-        g.sssp(root, configs=args.configs_sssp, out=False)
+        g.sssp(root, configs=args.configs_sssp, out=False, **sel)
         t1 = time.perf_counter()
         s_ms, s_rec = g.last_stats.device_ms, g.records()
         t2 = time.perf_counter()
@@ -378,7 +391,7 @@ def main():
         accumulate(acc, "pagerank", p_rec, n, m)
 
     # traversed edges (results fetched outside the timed region)
-    dist = g.sssp(root, configs=args.configs_sssp)
+    dist = g.sssp(root, configs=args.configs_sssp, **sel)
     reached = dist != np.uint32(0xFFFFFFFF)
     sssp_edges = int(np.diff(h.row_ptr)[reached].sum()) // 2
     pr_edges = (m // 2) * PR_ITERS
@@ -440,6 +453,15 @@ def main():
                                                      / np.abs(pr_rule))),
             "speedup_rule_vs_fixed": round((s_fixed + p_fixed) /
                                            ((dev_sssp + dev_pr) / args.steps), 2)}
+        # the same SSSP under the rule trees
+        t_r = []
+        for _ in range(args.steps):
+            dist_r = g.sssp(root, configs=args.configs_sssp)
+            t_r.append(g.last_stats.device_ms)
+        detail["sssp_selectors"] = {"headline": "learned" if sel else "rule",
+                                    "headline_ms": round(dev_sssp / args.steps, 3),
+                                    "rule_ms": round(min(t_r), 3),
+                                    "rule_bit_exact": bool(np.array_equal(dist_r, dist))}
 
     g.close()
     if not args.quick:
@@ -462,7 +484,8 @@ def main():
                                % (args.scale, args.seed),
                    "scale": args.scale, "edge_factor": 16, "n": n,
                    "m_undirected": m // 2, "root": root,
-                   "selector": "rule", "parallelism": "single-gpu"},
+                   "selector": ("learned (SSSP), rule (PageRank)" if sel else "rule"),
+                   "parallelism": "single-gpu"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "detail": detail,

@@ -245,8 +245,13 @@ int run_vc(engine_t &en, active_set_t &as, G &g, F &f, const run_opts_t &o,
            Stop stop_after_inspect) {
   int level = 0;
   for (level = 0; level < o.max_levels; ++level) {
+    as.window_moved = false;
     en.inspector.inspect(as, g, f, en.stats, en.fets, en.conf);
     if (as.finish(g, f, en.conf)) break;
+    // the inspection drained and the window advanced: inspect again rather
+    // than expand an empty frontier (a fused choice for it would let the
+    // next finish() absorb the new window's updates unexpanded)
+    if (as.take_window_moved()) continue;
     if (stop_after_inspect(level)) break;
     en.selector.select(en.stats, en.fets, en.conf);
     en.executor.filter(as, g, f, en.stats, en.fets, en.conf);

@@ -1743,7 +1743,10 @@ gsw_status gsw_explore_sssp(gsw_graph *G, int32_t root, int32_t max_levels,
                            hipMemcpyDeviceToDevice, st));
     };
     int written = 0;
-    for (int lvl = 0; lvl < (1 << 20) && written < max_levels; ++lvl) {
+    // lvl counts expanded levels, as the level loop's graph level does
+    // (an iteration that only advances the window expands nothing)
+    int lvl = 0;
+    for (int it = 0; it < (1 << 20) && written < max_levels; ++it) {
       save();
       // the level as the rule selector runs it
       sssp_level_setup(G, c, root, -1, lvl, fets, win);
@@ -1801,6 +1804,7 @@ gsw_status gsw_explore_sssp(gsw_graph *G, int32_t root, int32_t max_levels,
       gsw::sync();
       fets = c.en.fets;
       win = c.f.data.window;
+      ++lvl;
     }
     *n_out = written;
   });

@@ -119,6 +119,7 @@ struct active_set_t {
   int32_t *dg_block_count = nullptr;          // sorted-filter scratch [1025]
   int32_t *h_ctrl = nullptr;                  // pinned mirror of dg_ctrl
   bool converged = false;                     // host view of the halt flag
+  bool window_moved = false;                  // finish() advanced the window
 
   int64_t build(int64_t n, QueueMode qmode) {
     size = n;
@@ -171,6 +172,7 @@ struct active_set_t {
     gsw::fill_async<unsigned long long>(dg_work_base, 0ull, WORK_TOTAL);
     dg_work = dg_work_base;
     converged = false;
+    window_moved = false;
   }
 
   // root >= 0: only `root` is active; ALL_ACTIVE: every vertex; ALL_INACTIVE:
@@ -247,6 +249,13 @@ struct active_set_t {
   }
 
   // route the next expand's work counters to record row `i`
+  // true once after finish() advanced the stepping window
+  bool take_window_moved() {
+    const bool m = window_moved;
+    window_moved = false;
+    return m;
+  }
+
   void select_work_row(int i) {
     dg_work = dg_work_base + (int64_t)(i < WORK_ROWS ? i : WORK_ROWS - 1) * WORK_N;
     // past the last row every expand reuses it: start it from zero

@@ -85,7 +85,9 @@ void reset_conf(stat_t &stats, feature_t &fets, config_t &conf, Gr &g, F &f) {
 // exit, src/data_structures/active_set.cuh:37-43, notification.cuh:73-89):
 // with the SSSP window, a drained frontier only ends the run when no vertex
 // still has a pending update; otherwise the window advances and the loop
-// continues (in standalone mode if it was fused).
+// continues (in standalone mode if it was fused). `window_moved` tells a
+// level loop that called this after an inspection to inspect again (with
+// the new window) instead of expanding the drained frontier.
 template <typename G, typename F>
 bool active_set_t::finish(G &g, F &f, config_t &conf) {
   if (!converged) return false;
@@ -106,6 +108,7 @@ bool active_set_t::finish(G &g, F &f, config_t &conf) {
     gsw::sync();
     if (h_ctrl[CTRL_FLAG]) {
       converged = false;
+      window_moved = true;
       f.data.window.reset();
       if (conf.conf_fusion) {
         conf.conf_fuse_inspect = false;

@@ -194,6 +194,20 @@ def test_explore_sssp(gsw, orc):
     assert np.array_equal(g.sssp(root), orc.spfa(h.row_ptr, h.col, h.w, root))
 
 
+def test_trained_sssp_model_file(gsw, orc):
+    from gswitch_amd import learn
+    path = learn.model_path("sssp")
+    if not os.path.exists(path):
+        pytest.skip("no trained SSSP selector committed")
+    gsw.selector_load(learn.load_model(path), app="sssp")
+    for h in (gsw.rmat(14, 16, seed=11, weighted=True),
+              gsw.grid2d(120, 120, 0.1, 0.01, 4, weighted=True)):
+        g = gsw.Graph(h)
+        root = h.max_degree_vertex()
+        assert np.array_equal(g.sssp(root, selector=1),
+                              orc.spfa(h.row_ptr, h.col, h.w, root))
+
+
 def test_trained_model_file(gsw, orc):
     from gswitch_amd import learn
     if not os.path.exists(learn.MODEL_PATH):
