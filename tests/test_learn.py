@@ -1,0 +1,105 @@
+"""Host-side logic of the selector training pipeline (gswitch_amd/learn.py),
+on the CPU: class numbering of engine records, label targets, regret, level
+stride, seeded graph specs, model files, and the committed training sets."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import gswitch_amd as gsw
+from gswitch_amd import learn
+
+
+def rec(d, f, lb, fused=False, sorted_=False):
+    return {"dir": d, "fmt": f, "lb": lb, "fused": fused, "sorted": sorted_}
+
+
+def test_class_numbering():
+    # class = group * 5 + lb, lb in TM WM CM STRICT TWC
+    assert learn.class_of_record(rec("Push", "Queue", "TM")) == 0
+    assert learn.class_of_record(rec("Push", "Queue", "TWC", sorted_=True)) == 9
+    assert learn.class_of_record(rec("Push", "Bitmap", "CM")) == 12
+    assert learn.class_of_record(rec("Push", "Queue", "WM", fused=True)) == 16
+    assert learn.class_of_record(rec("Pull", "Queue", "STRICT")) == 23
+    assert learn.class_of_record(rec("Pull", "Bitmap", "TM")) == 25
+    # edge-balanced relatives count as STRICT
+    assert learn.class_of_record(rec("Pull", "Bitmap", "TWOD")) == 28
+    # a fused level keeps its class whatever the queue's sortedness
+    assert learn.class_of_record(rec("Push", "Queue", "CM", True, True)) == 17
+    names = gsw.MLP_CLASSES
+    assert len(names) == 30 and len(set(names)) == 30
+
+
+def test_targets_and_regret():
+    ms = np.array([[1.0, 2.0, 4.0, 1000.0], [3.0, 1.5, 1.5, 6.0]], np.float32)
+    y, m = learn.targets(ms)
+    np.testing.assert_allclose(y[0], [0, 1, 2, learn.CLIP], atol=1e-6)
+    np.testing.assert_allclose(y[1], [1, 0, 0, 2], atol=1e-6)
+    assert np.all(m == 1)
+    best = ms.argmin(axis=1)
+    assert learn.per_level_regret(ms, best) == pytest.approx(1.0)
+    # choosing class 0 everywhere: (1 + 3) / (1 + 1.5)
+    assert learn.per_level_regret(ms, np.zeros(2, int)) == pytest.approx(4 / 2.5)
+
+
+def test_level_stride():
+    assert learn.level_stride(5, 32) == 1
+    assert learn.level_stride(32, 32) == 1
+    assert learn.level_stride(33, 32) == 2
+    assert learn.level_stride(1000, 32) == 32
+
+
+def test_specs_are_seeded_and_prefix_stable():
+    a, b = learn.training_specs(), learn.training_specs()
+    assert a == b and len(a) == 300
+    assert learn.training_specs(150) == a[:150]
+    kinds = [s["kind"] for s in a]
+    assert kinds.count("rmat") == 120 and kinds.count("grid") == 60
+    for s in a:
+        if s["kind"] == "rmat":
+            assert 14 <= s["scale"] <= 22 and 0.45 <= s["a"] <= 0.65
+            assert s["b"] == pytest.approx((1 - s["a"] - 0.05) / 2)
+    e = learn.eval_specs()
+    assert len(e) == 40 and e == learn.eval_specs()
+    assert {s["scale"] for s in e if s["kind"] == "rmat"} == {20, 21, 22, 23, 24}
+    # held-out graphs are not training graphs
+    assert not ({s["seed"] for s in a} & {s["seed"] for s in e})
+
+
+def test_model_file_roundtrip(tmp_path):
+    w = learn.init_weights(seed=3)
+    assert w.shape == (gsw.MLP_NPARAM,) and np.isfinite(w).all()
+    np.testing.assert_array_equal(w, learn.init_weights(seed=3))
+    p = str(tmp_path / "m.npz")
+    learn.save_model(p, w, {"app": "sssp"})
+    np.testing.assert_array_equal(learn.load_model(p), w)
+    bad = w.copy()
+    bad[5] = np.nan
+    learn.save_model(p, bad, {})
+    with pytest.raises(ValueError):
+        learn.load_model(p)
+    learn.save_model(p, w[:100], {})
+    with pytest.raises(ValueError):
+        learn.load_model(p)
+
+
+@pytest.mark.parametrize("app", learn.APPS)
+def test_committed_training_sets(orc, app):
+    path = os.path.join(os.path.dirname(learn.MODEL_PATH), "selector_%s_train.npz" % app)
+    if not os.path.exists(path):
+        pytest.skip("no committed training set")
+    with np.load(path, allow_pickle=False) as z:
+        d = {k: z[k] for k in z.files}
+    n = len(d["ms"])
+    assert d["features"].shape == (n, 12) and d["ms"].shape == (n, 30)
+    assert np.all(d["ms"] > 0) and np.isfinite(d["features"]).all()
+    assert np.all((d["rule"] >= 0) & (d["rule"] < 30))
+    assert len(np.unique(d["graph"])) == len(json.loads(str(d["specs"]))) == 300
+    # the committed network's choices (FP64 oracle network) cost less than
+    # the rule's over its own training levels
+    w = learn.load_model(learn.model_path(app))
+    choice = orc.mlp_forward(w, d["features"]).argmin(axis=1)
+    r_mlp = learn.per_level_regret(d["ms"], choice)
+    r_rule = learn.per_level_regret(d["ms"], d["rule"])
+    assert r_mlp < r_rule and r_mlp < 1.2, (app, r_mlp, r_rule)
