@@ -179,3 +179,36 @@ def test_pagerank_iteration_counts(gsw, orc, gset):
     for k in (0, 1, 2, 5):
         pr = g.pagerank(0.85, k)
         assert max_rel(pr, orc.pagerank(h.row_ptr, h.col, 0.85, k)) <= PR_TOL, k
+
+
+# ---------------------------------------------------------------- degenerate
+@pytest.mark.parametrize("n", [1, 5])
+def test_graphs_without_edges(gsw, orc, n):
+    """Vertices and no edges: every traversal ends at once with the root
+    alone reached, whatever the variant."""
+    rp = np.zeros(n + 1, np.int64)
+    col = np.zeros(0, np.int32)
+    g = gsw.Graph(row_ptr=rp, col=col)
+    exp = np.full(n, -1, np.int32)
+    exp[n - 1] = 0
+    assert np.array_equal(g.bfs(n - 1), exp)
+    for v in ["Push-Queue-STRICT-Standalone", "Pull-Bitmap-TM-Standalone",
+              "Push-Queue-WM-Fused", "Push-Queue-TWC-Standalone"]:
+        assert np.array_equal(g.bfs(n - 1, configs=v), exp), v
+    assert np.array_equal(g.cc(), np.arange(n, dtype=np.int32))
+    pr = g.pagerank(0.85, 10)
+    assert max_rel(pr, orc.pagerank(rp, col, 0.85, 10)) <= PR_TOL
+    pr2 = g.pagerank(0.85, 10, configs="Pull-Bitmap-TWOD-Standalone", pull_kernel=1)
+    assert max_rel(pr2, orc.pagerank(rp, col, 0.85, 10)) <= PR_TOL
+    lv, nsp, dep = g.bc(0)
+    rl, rn, rd = orc.bc(rp, col, 0)
+    assert np.array_equal(lv, rl) and np.allclose(nsp, rn) and np.allclose(dep, rd)
+    masks, _ = g.msbfs64(np.arange(min(n, 64), dtype=np.int32))
+    assert np.array_equal(masks, (np.uint64(1) << np.arange(n, dtype=np.uint64)))
+    multi = g.bfs_multi(np.array([0], np.int32), configs="Push-x-WM-x", capture=2)
+    exp0 = np.full(n, -1, np.int32)
+    exp0[0] = 0
+    assert np.array_equal(multi, exp0)
+    ex = g.explore_bfs(0, max_levels=4, stride=1, reps=1)
+    assert len(ex["level"]) <= 1
+    g.close()
