@@ -497,7 +497,7 @@ int bfs_captured_run(gsw_graph *G, int64_t k, LB lb, int small, int large,
                              0, st, body);
         } else {
           hipLaunchKernelGGL(gsw::k_push_wm<Bd>, dim3(grid), dim3(gsw::BLOCK), 0,
-                             st, body);
+                             st, body, gsw::WAVE);
         }
         hipLaunchKernelGGL(k_level_advance, dim3(1), dim3(64), 0, st,
                            G->d_level, as.queue.sizes, j & 1);

@@ -44,6 +44,7 @@ struct config_t {
   bool conf_fuse_inspect = false;
   bool conf_lite = false;
   bool conf_window = false;
+  int wm_vpw = 64;  // frontier items per wavefront of the WM push
   bool conf_compensation = false;
   bool conf_super_fusion = false;
   bool conf_fusion = false;

@@ -208,6 +208,15 @@ struct executor_t {
     rec.active = insp ? insp->known_active : 0;
     rec.inactive = insp ? insp->known_inactive : 0;
     as.select_work_row((int)log.size());
+    {
+      // WM push: about 64 edges per wavefront at the frontier's mean degree
+      // (the last inspected one on fused levels)
+      const double d = fets.cur_avg_deg_active > 0 ? fets.cur_avg_deg_active
+                                                   : fets.avg_deg;
+      int vpw = gsw::WAVE;
+      while (vpw > 8 && vpw * d > 64.0) vpw >>= 1;
+      conf.wm_vpw = vpw;
+    }
     int e0 = -1, e1 = -1;
     if (profile) {
       e0 = ev_next;
@@ -320,8 +329,9 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf,
                                dim3(BLOCK), 0, st, body);
             break;
           case WM:
-            hipLaunchKernelGGL(k_push_wm<Bd>, dim3(expand_grid(N, BLOCK)),
-                               dim3(BLOCK), 0, st, body);
+            hipLaunchKernelGGL(k_push_wm<Bd>,
+                               dim3(expand_grid(N * (WAVE / conf.wm_vpw), BLOCK)),
+                               dim3(BLOCK), 0, st, body, conf.wm_vpw);
             break;
           case CM:
             hipLaunchKernelGGL(k_push_cm<Bd>, dim3(expand_grid(N, BLOCK)),

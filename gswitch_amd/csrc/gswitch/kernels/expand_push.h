@@ -149,7 +149,10 @@ __global__ void __launch_bounds__(BLOCK) k_push_tm(B body) {
 
 // ---------------------------------------------------------------- WM
 template <class B>
-__global__ void __launch_bounds__(BLOCK) k_push_wm(B body) {
+// A wavefront takes `vpw` frontier items (64, 32, 16 or 8) and walks their
+// edges 64 at a time: with a small mean degree fewer items per wavefront
+// means more wavefronts in flight and fewer dependent 64-edge steps each.
+__global__ void __launch_bounds__(BLOCK) k_push_wm(B body, int vpw) {
   __shared__ int s_stage[B::STAGE_LDS];
   auto st = body.stage(s_stage);
   __shared__ int s_v[BLOCK];
@@ -162,9 +165,9 @@ __global__ void __launch_bounds__(BLOCK) k_push_wm(B body) {
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> LOG_WAVE;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> LOG_WAVE;
   unsigned long long ne = 0, nv = 0;
-  for (int64_t base = wave << 6; base < N; base += nwaves << 6) {
+  for (int64_t base = wave * vpw; base < N; base += nwaves * vpw) {
     const int64_t idx = base + lane;
-    const int v = (idx < N) ? body.as.fetch(idx, want) : -1;
+    const int v = (lane < vpw && idx < N) ? body.as.fetch(idx, want) : -1;
     int64_t start = 0;
     int deg = 0;
     if (v >= 0) {
