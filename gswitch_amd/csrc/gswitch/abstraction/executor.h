@@ -209,12 +209,14 @@ struct executor_t {
     rec.inactive = insp ? insp->known_inactive : 0;
     as.select_work_row((int)log.size());
     {
-      // WM push: about 64 edges per wavefront at the frontier's mean degree
-      // (the last inspected one on fused levels)
+      // WM push: about 128 edges per wavefront at the frontier's mean degree
+      // (the last inspected one on fused levels); measured against 64 edges
+      // and against filling the capped grid first, this target keeps
+      // long-range grids at fixed-64 speed and still halves ER levels
       const double d = fets.cur_avg_deg_active > 0 ? fets.cur_avg_deg_active
                                                    : fets.avg_deg;
       int vpw = gsw::WAVE;
-      while (vpw > 8 && vpw * d > 64.0) vpw >>= 1;
+      while (vpw > 8 && vpw * d > 128.0) vpw >>= 1;
       conf.wm_vpw = vpw;
     }
     int e0 = -1, e1 = -1;
