@@ -261,49 +261,67 @@ def extended(gsw, args):
                       "bit_exact_vs_64_bfs": bool(np.array_equal(masks, ref))}
     g.close()
     del h
-    # --- ER batch sample (instances 0 .. count-1 of the seed-5 batch)
-    t = time.perf_counter()
-    h, off = gsw.erdos_renyi_batch(args.batch, 1000, 200000, 2.0, 32.0, seed=5)
-    log("batch: %d graphs n=%d m=%d in %.1fs" % (args.batch, h.n, h.m,
-                                                 time.perf_counter() - t))
-    g = gsw.Graph(h)
-    roots = off[:-1].astype(np.int32)
-    tune = g.autotune_bfs(roots, capture=8)
-    kw = dict(configs="Push-x-%s-x" % tune["lb"], capture=8, grid=tune["grid"],
-              twc_small=tune["twc_small"], twc_large=tune["twc_large"])
-    lvb = g.bfs_multi(roots, **kw)
-    bms = []
-    for _ in range(3):
-        g.bfs_multi(roots, out=False, profile=False, **kw)
-        t_b = g.last_stats.device_ms
-        g.cc(out=False, profile=False)
-        bms.append(t_b + g.last_stats.device_ms)
-    ms_batch = min(bms)
-    labb = g.cc()
-    e_bfs = graph500_edges(h, lvb)
-    e_cc = h.m // 2
-    # per-graph launch loop with a fixed strategy (upload outside the timing)
-    t_loop = 0.0
+    # --- ER batch: instances 0 .. batch-1 of the seed-5 batch, built and run
+    # as consecutive pieces of batch_piece instances (one union each, far
+    # below 2^31 directed entries); the bins are autotuned on the first piece
+    tune, kw = None, None
+    ms_batch = t_loop = t_gen = 0.0
+    e_bfs = e_cc = n_tot = m_tot = pieces = 0
     same = True
-    for i in range(args.batch):
-        lo, hi = int(off[i]), int(off[i + 1])
-        rp = h.row_ptr[lo:hi + 1] - h.row_ptr[lo]
-        col = h.col[h.row_ptr[lo]:h.row_ptr[hi]] - lo
-        gi = gsw.Graph(row_ptr=rp, col=col)
-        lvi = gi.bfs(0, configs=fixed, profile=False)
-        t_loop += gi.last_stats.device_ms
-        labi = gi.cc(profile=False)
-        t_loop += gi.last_stats.device_ms
-        same &= bool(np.array_equal(lvi, lvb[lo:hi])) and \
-            bool(np.array_equal(labi, labb[lo:hi] - lo))
-        gi.close()
-    out["er_batch"] = {"graphs": args.batch, "n": h.n, "m_undirected": h.m // 2,
-                       "autotune": tune, "batched_ms": round(ms_batch, 3),
+    for first in range(0, args.batch, args.batch_piece):
+        k = min(args.batch_piece, args.batch - first)
+        t = time.perf_counter()
+        h, off = gsw.erdos_renyi_batch(k, 1000, 200000, 2.0, 32.0, seed=5, first=first)
+        t_gen += time.perf_counter() - t
+        g = gsw.Graph(h)
+        roots = off[:-1].astype(np.int32)
+        if tune is None:
+            tune = g.autotune_bfs(roots, capture=8)
+            kw = dict(configs="Push-x-%s-x" % tune["lb"], capture=8, grid=tune["grid"],
+                      twc_small=tune["twc_small"], twc_large=tune["twc_large"])
+        lvb = g.bfs_multi(roots, **kw)
+        bms = []
+        for _ in range(3):
+            g.bfs_multi(roots, out=False, profile=False, **kw)
+            t_b = g.last_stats.device_ms
+            g.cc(out=False, profile=False)
+            bms.append(t_b + g.last_stats.device_ms)
+        ms_piece = min(bms)
+        ms_batch += ms_piece
+        labb = g.cc()
+        e_bfs += graph500_edges(h, lvb)
+        e_cc += h.m // 2
+        n_tot += h.n
+        m_tot += h.m // 2
+        # per-graph launch loop with a fixed strategy (upload outside the timing)
+        t_piece = 0.0
+        for i in range(k):
+            lo, hi = int(off[i]), int(off[i + 1])
+            rp = h.row_ptr[lo:hi + 1] - h.row_ptr[lo]
+            col = h.col[h.row_ptr[lo]:h.row_ptr[hi]] - lo
+            gi = gsw.Graph(row_ptr=rp, col=col)
+            lvi = gi.bfs(0, configs=fixed, profile=False)
+            t_piece += gi.last_stats.device_ms
+            labi = gi.cc(profile=False)
+            t_piece += gi.last_stats.device_ms
+            same &= bool(np.array_equal(lvi, lvb[lo:hi])) and \
+                bool(np.array_equal(labi, labb[lo:hi] - lo))
+            gi.close()
+        t_loop += t_piece
+        pieces += 1
+        g.close()
+        del h, g
+        log("batch piece %d: instances %d..%d n=%d batched %.2f ms, loop %.1f ms, "
+            "bit-exact so far %s" % (pieces, first, first + k - 1, n_tot, ms_piece,
+                                     t_piece, same))
+    log("batch: %d graphs in %d pieces, host build %.1fs" % (args.batch, pieces, t_gen))
+    out["er_batch"] = {"graphs": args.batch, "pieces": pieces, "n": n_tot,
+                       "m_undirected": m_tot, "autotune": tune,
+                       "batched_ms": round(ms_batch, 3),
                        "batched_gteps": round((e_bfs + e_cc) / (ms_batch * 1e-3) / 1e9, 2),
                        "per_graph_loop_ms": round(t_loop, 3),
                        "speedup": round(t_loop / ms_batch, 2),
                        "bit_exact_vs_per_graph": same}
-    g.close()
     # --- aggregate: batched + learned vs plain per-query fixed strategy
     t_new = ms_batch + ms64 + (learned if learned is not None else ms) + ms_g_learned
     t_old = t_loop + t_seq + ms_fixed + ms_g_fixed
@@ -330,8 +348,10 @@ def main():
     ap.add_argument("--bfs-scale", type=int, default=24)
     ap.add_argument("--grid", type=int, default=4096,
                     help="side of the high-diameter grid")
-    ap.add_argument("--batch", type=int, default=1000,
-                    help="ER batch sample size (instances 0..batch-1 of 20,000)")
+    ap.add_argument("--batch", type=int, default=20000,
+                    help="ER batch size (instances 0..batch-1 of the seed-5 batch)")
+    ap.add_argument("--batch-piece", type=int, default=1000,
+                    help="instances per union built and run at once")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU "

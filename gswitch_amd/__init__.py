@@ -74,6 +74,9 @@ _SIGS = {
     "gsw_gen_er_batch": (C.c_int32, [C.c_int32, C.c_int64, C.c_int64, C.c_double,
                                      C.c_double, C.c_uint64, C.c_int32,
                                      C.POINTER(_P), _P]),
+    "gsw_gen_er_batch_range": (C.c_int32, [C.c_int64, C.c_int32, C.c_int64, C.c_int64,
+                                           C.c_double, C.c_double, C.c_uint64,
+                                           C.c_int32, C.POINTER(_P), _P]),
     "gsw_host_graph_info": (C.c_int32, [_P, C.POINTER(C.c_int64),
                                         C.POINTER(C.c_int64),
                                         C.POINTER(C.c_int32),
@@ -205,12 +208,13 @@ def barabasi_albert(n, m, seed=6, weighted=False):
 
 
 def erdos_renyi_batch(count, nmin=1000, nmax=200000, dmin=2.0, dmax=32.0, seed=5,
-                      weighted=False):
-    """`count` ER instances as one disjoint union: (HostGraph, offsets[count+1])."""
-    off = np.empty(count + 1, np.int64)
+                      weighted=False, first=0):
+    """Instances first .. first+count-1 of an ER batch as one disjoint union
+    numbered from vertex 0: (HostGraph, offsets[count+1])."""
+    off = np.empty(max(count, 0) + 1, np.int64)
     h = _P()
-    _check(_lib.gsw_gen_er_batch(count, nmin, nmax, dmin, dmax, seed, int(weighted),
-                                 C.byref(h), _ptr(off)))
+    _check(_lib.gsw_gen_er_batch_range(first, count, nmin, nmax, dmin, dmax, seed,
+                                       int(weighted), C.byref(h), _ptr(off)))
     return HostGraph(h), off
 
 

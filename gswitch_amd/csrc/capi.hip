@@ -811,20 +811,28 @@ gsw_status gsw_gen_ba(int64_t n, int32_t m, uint64_t seed, int32_t weighted,
   });
 }
 
+gsw_status gsw_gen_er_batch_range(int64_t first, int32_t count, int64_t nmin,
+                                  int64_t nmax, double dmin, double dmax,
+                                  uint64_t seed, int32_t weighted,
+                                  gsw_host_graph **out, int64_t *offsets_out) {
+  return guarded([&] {
+    GSW_REQUIRE(out && offsets_out && first >= 0 && count >= 1 && nmin >= 1 &&
+                    nmax >= nmin && nmax <= INT32_MAX && dmin >= 0 && dmax >= dmin,
+                GSW_E_INVALID, "invalid ER batch parameters");
+    std::vector<int32_t> s, d;
+    std::vector<int64_t> off;
+    gsw_host::gen_er_batch(first, count, nmin, nmax, dmin, dmax, seed, s, d, off);
+    finish_gen(off.back(), s, d, seed, weighted, out);
+    std::memcpy(offsets_out, off.data(), sizeof(int64_t) * off.size());
+  });
+}
+
 gsw_status gsw_gen_er_batch(int32_t count, int64_t nmin, int64_t nmax,
                             double dmin, double dmax, uint64_t seed,
                             int32_t weighted, gsw_host_graph **out,
                             int64_t *offsets_out) {
-  return guarded([&] {
-    GSW_REQUIRE(out && offsets_out && count >= 1 && nmin >= 1 && nmax >= nmin &&
-                    nmax <= INT32_MAX && dmin >= 0 && dmax >= dmin,
-                GSW_E_INVALID, "invalid ER batch parameters");
-    std::vector<int32_t> s, d;
-    std::vector<int64_t> off;
-    gsw_host::gen_er_batch(count, nmin, nmax, dmin, dmax, seed, s, d, off);
-    finish_gen(off.back(), s, d, seed, weighted, out);
-    std::memcpy(offsets_out, off.data(), sizeof(int64_t) * off.size());
-  });
+  return gsw_gen_er_batch_range(0, count, nmin, nmax, dmin, dmax, seed,
+                                weighted, out, offsets_out);
 }
 
 gsw_status gsw_host_graph_info(const gsw_host_graph *g, int64_t *n, int64_t *m,

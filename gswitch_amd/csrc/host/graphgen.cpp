@@ -301,11 +301,14 @@ namespace gsw_host {
 // draw(batch_key, 2i + 1) as uniforms in [0, 1); n_i = floor(exp(ln nmin +
 // u1 (ln nmax - ln nmin))), d_i = dmin + u2 (dmax - dmin); its edges are
 // gen_er(n_i, d_i, seed_i) with seed_i = stream_key(seed, 1000 + i), shifted
-// by the instance's vertex offset.
-void gen_er_batch(int count, int64_t nmin, int64_t nmax, double dmin,
-                  double dmax, uint64_t seed, std::vector<int32_t> &src,
-                  std::vector<int32_t> &dst, std::vector<int64_t> &offsets) {
-  if (count < 1 || nmin < 1 || nmax < nmin)
+// by the instance's vertex offset. Instances first .. first+count-1 are
+// built, numbered from vertex 0: a batch built in consecutive ranges is the
+// whole batch cut into pieces.
+void gen_er_batch(int64_t first, int count, int64_t nmin, int64_t nmax,
+                  double dmin, double dmax, uint64_t seed,
+                  std::vector<int32_t> &src, std::vector<int32_t> &dst,
+                  std::vector<int64_t> &offsets) {
+  if (first < 0 || count < 1 || nmin < 1 || nmax < nmin)
     throw std::invalid_argument("invalid batch parameters");
   const uint64_t key = stream_key(seed, STREAM_BATCH);
   std::vector<int64_t> ns(count);
@@ -313,8 +316,9 @@ void gen_er_batch(int count, int64_t nmin, int64_t nmax, double dmin,
   offsets.assign(count + 1, 0);
   const double l0 = std::log((double)nmin), l1 = std::log((double)nmax);
   for (int i = 0; i < count; ++i) {
-    const double u1 = (double)(draw(key, 2 * (uint64_t)i) >> 11) * 0x1.0p-53;
-    const double u2 = (double)(draw(key, 2 * (uint64_t)i + 1) >> 11) * 0x1.0p-53;
+    const uint64_t k = (uint64_t)(first + i);
+    const double u1 = (double)(draw(key, 2 * k) >> 11) * 0x1.0p-53;
+    const double u2 = (double)(draw(key, 2 * k + 1) >> 11) * 0x1.0p-53;
     ns[i] = std::max<int64_t>(1, (int64_t)std::floor(std::exp(l0 + u1 * (l1 - l0))));
     ns[i] = std::min(ns[i], nmax);
     ds[i] = dmin + u2 * (dmax - dmin);
@@ -332,7 +336,7 @@ void gen_er_batch(int count, int64_t nmin, int64_t nmax, double dmin,
   run_threads(T, [&](int t) {
     std::vector<int32_t> s, d;
     for (int i = t; i < count; i += T) {
-      gen_er(ns[i], ds[i], stream_key(seed, 1000 + (uint64_t)i), s, d);
+      gen_er(ns[i], ds[i], stream_key(seed, 1000 + (uint64_t)(first + i)), s, d);
       for (size_t j = 0; j < s.size(); ++j) {
         src[eo[i] + j] = (int32_t)(s[j] + offsets[i]);
         dst[eo[i] + j] = (int32_t)(d[j] + offsets[i]);

@@ -82,9 +82,10 @@ void gen_ba(int64_t n, int m, uint64_t seed, std::vector<int32_t> &src,
             std::vector<int32_t> &dst);
 // `count` Erdos-Renyi instances as one disjoint union (see gsw_gen_er_batch);
 // offsets receives count + 1 vertex offsets.
-void gen_er_batch(int count, int64_t nmin, int64_t nmax, double dmin,
-                  double dmax, uint64_t seed, std::vector<int32_t> &src,
-                  std::vector<int32_t> &dst, std::vector<int64_t> &offsets);
+void gen_er_batch(int64_t first, int count, int64_t nmin, int64_t nmax,
+                  double dmin, double dmax, uint64_t seed,
+                  std::vector<int32_t> &src, std::vector<int32_t> &dst,
+                  std::vector<int64_t> &offsets);
 
 int num_threads();
 

@@ -195,6 +195,13 @@ gsw_status gsw_gen_er_batch(int32_t count, int64_t nmin, int64_t nmax,
                             double dmin, double dmax, uint64_t seed,
                             int32_t weighted, gsw_host_graph **out,
                             int64_t *offsets_out);
+/* Instances first .. first+count-1 of the same batch, numbered from vertex
+ * 0 (offsets_out[0] = 0): consecutive ranges cut the whole batch into
+ * pieces, so a batch too large for one union is built and run piecewise. */
+gsw_status gsw_gen_er_batch_range(int64_t first, int32_t count, int64_t nmin,
+                                  int64_t nmax, double dmin, double dmax,
+                                  uint64_t seed, int32_t weighted,
+                                  gsw_host_graph **out, int64_t *offsets_out);
 /* Records of the last run on `g`; *n = number copied (<= cap). */
 gsw_status gsw_last_records(const gsw_graph *g, gsw_level_record *out,
                             int32_t cap, int32_t *n);

@@ -184,3 +184,18 @@ def test_er_batch_is_union_of_instances(gsw):
         gsw.erdos_renyi_batch(0)
     with pytest.raises(gsw.GswError):
         gsw.erdos_renyi_batch(3, nmin=10, nmax=5)
+    with pytest.raises(gsw.GswError):
+        gsw.erdos_renyi_batch(3, first=-1)
+
+
+def test_er_batch_ranges_cut_the_batch(gsw):
+    """Consecutive instance ranges are the whole batch cut into pieces."""
+    count, nmin, nmax = 10, 50, 3000
+    h, off = gsw.erdos_renyi_batch(count, nmin, nmax, 2.0, 32.0, seed=5)
+    for first, k in [(0, 4), (4, 1), (5, 5)]:
+        p, po = gsw.erdos_renyi_batch(k, nmin, nmax, 2.0, 32.0, seed=5, first=first)
+        lo, hi = int(off[first]), int(off[first + k])
+        assert np.array_equal(po, off[first:first + k + 1] - lo)
+        assert p.n == hi - lo
+        assert np.array_equal(p.row_ptr, h.row_ptr[lo:hi + 1] - h.row_ptr[lo])
+        assert np.array_equal(p.col, h.col[h.row_ptr[lo]:h.row_ptr[hi]] - lo)
