@@ -242,6 +242,7 @@ void upload_graph(gsw_graph *G) {
     G->dgw.dg_adj_list = G->dg.dg_adj_list;
     G->dgw.dgr_row_ptr = G->dg.dgr_row_ptr;
     G->dgw.dgr_adj_list = G->dg.dgr_adj_list;
+    G->dgw.dgr_first = G->dg.dgr_first;
     G->dgw.twod = nullptr;  // TWOD slices are built for unweighted views only
     GSW_REQUIRE(!G->hg.directed, GSW_E_UNSUPPORTED,
                 "weighted directed graphs are not supported");

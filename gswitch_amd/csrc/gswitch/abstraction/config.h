@@ -45,6 +45,8 @@ struct config_t {
   bool conf_lite = false;
   bool conf_window = false;
   int wm_vpw = 64;  // frontier items per wavefront of the WM push
+  // TM pulls take each target's first in-edge from the per-vertex copy
+  bool pull_first = false;
   bool conf_compensation = false;
   bool conf_super_fusion = false;
   bool conf_fusion = false;

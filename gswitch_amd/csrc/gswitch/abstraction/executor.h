@@ -218,6 +218,11 @@ struct executor_t {
       int vpw = gsw::WAVE;
       while (vpw > 8 && vpw * d > 128.0) vpw >>= 1;
       conf.wm_vpw = vpw;
+      // early-exit TM pulls probe each target's first in-edge from the
+      // per-vertex copy when a random in-edge is likely to end the walk:
+      // the frontier holds >= 10% of the edges (on R-MAT, where the first
+      // in-neighbour is a hub, far more walks end there)
+      conf.pull_first = conf.conf_pruning && fets.push_workload_ratio >= 0.1;
     }
     int e0 = -1, e1 = -1;
     if (profile) {

@@ -118,6 +118,11 @@ template <typename E> struct device_graph_t<CSR, E> {
   int64_t *dgr_row_ptr = nullptr;
   int32_t *dgr_adj_list = nullptr;
   E *dgr_edgedata = nullptr;
+  // first in-neighbour of every vertex (dgr_adj_list[dgr_row_ptr[v]], -1
+  // without in-edges): the bottom-up pull's first probe of a wavefront of
+  // consecutive targets is one coalesced load instead of one scattered
+  // adjacency load per lane
+  int32_t *dgr_first = nullptr;
   // host-side handle of the TWOD slices (kernels/twod.h builds them on first
   // use); shared by every copy of this struct, owned by the one that built it
   twod_t *twod = nullptr;
@@ -157,6 +162,17 @@ template <typename E> struct device_graph_t<CSR, E> {
       dgr_adj_list = dg_adj_list;
       dgr_edgedata = dg_edgedata;
     }
+    {
+      const std::vector<int64_t> &rp = directed ? hg.r_row_ptr : hg.row_ptr;
+      const std::vector<int32_t> &cl = directed ? hg.r_col : hg.col;
+      std::vector<int32_t> first((size_t)nvertexs + 1, -1);
+      for (int64_t v = 0; v < nvertexs; ++v)
+        if (rp[v] < rp[v + 1]) first[v] = cl[rp[v]];
+      H_ERR(hipMalloc((void **)&dgr_first, sizeof(int32_t) * (nvertexs + 1)));
+      H_ERR(hipMemcpy(dgr_first, first.data(), sizeof(int32_t) * (nvertexs + 1),
+                      hipMemcpyHostToDevice));
+      bytes += sizeof(int32_t) * (nvertexs + 1);
+    }
     return bytes;
   }
 
@@ -181,7 +197,9 @@ template <typename E> struct device_graph_t<CSR, E> {
     }
     hipFree(dg_row_ptr);
     hipFree(dg_adj_list);
+    hipFree(dgr_first);
     if (has_edata) hipFree(dg_edgedata);
+    dgr_first = nullptr;
     dg_row_ptr = dgr_row_ptr = nullptr;
     dg_adj_list = dgr_adj_list = nullptr;
     dg_edgedata = dgr_edgedata = nullptr;

@@ -55,6 +55,9 @@ template <typename G, typename F> struct pull_body {
 
   __device__ __forceinline__ const int64_t *rows() const { return g.dgr_row_ptr; }
   __device__ __forceinline__ const int32_t *cols() const { return g.dgr_adj_list; }
+  // cols()[rows()[v]] for a target with in-edges, read from the per-vertex
+  // copy (coalesced across a wavefront of consecutive targets)
+  __device__ __forceinline__ int first_col(int v) const { return g.dgr_first[v]; }
   __device__ __forceinline__ bool source_ok(int u) const {
     return conf.conf_fromall || as.bitmap.is_active(u);
   }
@@ -102,8 +105,21 @@ __global__ void __launch_bounds__(BLOCK) k_pull_tm(B body) {
     const int64_t s = body.rows()[v], e = body.rows()[v + 1];
     WA acc;
     bool any = false;
-    int64_t j = s;
-    for (; j < e; ++j) {
+    int64_t j = s, end = e;
+    if (body.conf.pull_first && j < e) {
+      // the first in-edge from the per-vertex copy; an early exit there
+      // leaves the walk below empty
+      const int u = body.first_col(v);
+      if (body.source_ok(u)) {
+        const WA m = body.msg(u, j);
+        acc = *body.f.wa_of(v);
+        any = true;
+        body.f.comp(&acc, m, body.g);
+        if (prune) end = j + 1;
+      }
+      ++j;
+    }
+    for (; j < end; ++j) {
       const int u = body.cols()[j];
       if (!body.source_ok(u)) continue;
       const WA m = body.msg(u, j);
@@ -153,8 +169,19 @@ __global__ void __launch_bounds__(BLOCK)
       ++nv;
       const int64_t s = body.rows()[v], e = body.rows()[v + 1];
       WA acc;
-      int64_t j = s;
-      for (; j < e; ++j) {
+      int64_t j = s, end = e;
+      if (body.conf.pull_first && j < e) {
+        const int u = body.first_col((int)v);
+        if (body.source_ok(u)) {
+          const WA m = body.msg(u, j);
+          acc = *body.f.wa_of(v);
+          found = true;
+          body.f.comp(&acc, m, body.g);
+          if (prune) end = j + 1;
+        }
+        ++j;
+      }
+      for (; j < end; ++j) {
         const int u = body.cols()[j];
         if (!body.source_ok(u)) continue;
         const WA m = body.msg(u, j);
