@@ -264,7 +264,7 @@ def main(argv=None):
     ap.add_argument("--data")
     ap.add_argument("--model", default=None,
                     help="model file (default: the committed one of --app)")
-    ap.add_argument("--out", default="gpurun_out/selector")
+    ap.add_argument("--out", default="selector_out")
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--epochs", type=int, default=400)
     a = ap.parse_args(argv)
