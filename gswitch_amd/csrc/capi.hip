@@ -431,9 +431,9 @@ int bfs_engine_run(gsw_graph *G, const gsw_run_opts *opts, int32_t root,
   H_ERR(hipEventRecord(G->ev0, gsw::stream()));
   c.as.reset();
   if (d_roots)
-    c.as.init_roots(d_roots, k);
+    c.as.init_roots(d_roots, k, true);
   else
-    c.as.init(root);
+    c.as.init(root, true);
   init_bfs_levels(G, c.f.data.dg_wa, root, d_roots, k);
   GU g = G->dg;
   const int levels = run_vc(c.en, c.as, g, c.f, o, [](int) { return false; });
@@ -593,7 +593,7 @@ void sssp_level_setup(gsw_graph *G, app_ctx_t<SSSP> &c, int root, int cls,
   c.en.conf.conf_fuse_inspect = false;
   c.en.conf.conf_fusion = false;
   c.as.reset();
-  if (lvl == 0) c.as.init(root);
+  if (lvl == 0) c.as.init(root, true);
 }
 
 // Cost of SSSP level `lvl` with class `cls` from the current distance
@@ -1092,7 +1092,7 @@ gsw_status gsw_bc(gsw_graph *G, int32_t root, const gsw_run_opts *opts,
     init_conf(c.en.stats, fets, c.en.conf, gv, c.f);
     H_ERR(hipEventRecord(G->ev0, st));
     c.as.reset();
-    c.as.init(root);
+    c.as.init(root, true);
     hipLaunchKernelGGL(k_init_bc, dim3(gsw::grid_for(n)), dim3(gsw::BLOCK), 0, st,
                        c.f.data.dg_wa, n, root);
     GU g = G->dg;
@@ -1122,7 +1122,7 @@ gsw_status gsw_bc(gsw_graph *G, int32_t root, const gsw_run_opts *opts,
     c.en.conf.force.fusion = Standalone;
     c.en.conf.conf_first_round = false;
     c.as.reset();
-    c.as.init(ALL_INACTIVE);
+    c.as.init(ALL_INACTIVE, true);
     g.level = deepest;
     for (int l = 0; l < o.max_levels && g.level >= 0; ++l) {
       c.en.inspector.inspect(c.as, g, h, c.en.stats, fets, c.en.conf);
@@ -1180,7 +1180,7 @@ gsw_status gsw_sssp(gsw_graph *G, int32_t root, const gsw_run_opts *opts,
     hipStream_t st = gsw::stream();
     H_ERR(hipEventRecord(G->ev0, st));
     c.as.reset();
-    c.as.init(root);
+    c.as.init(root, true);
     hipLaunchKernelGGL(k_init_sssp, dim3(gsw::grid_for(G->hg.n)),
                        dim3(gsw::BLOCK), 0, st, c.f.data.dg_wa, c.f.data.dg_ra,
                        G->hg.n, root);

@@ -11,6 +11,8 @@
 //    the TWC expand, allocated on first use;
 //  * STRICT's workset holds int64 prefix offsets.
 #pragma once
+#include <algorithm>
+
 #include "../abstraction/config.h"
 #include "../utils/runtime.h"
 #include "bitmap.h"
@@ -84,6 +86,35 @@ constexpr int WORK_SLOTS = 16;
 constexpr int64_t WORK_SLOT_STRIDE = (int64_t)WORK_ROWS * WORK_N;
 constexpr int64_t WORK_TOTAL = WORK_SLOT_STRIDE * WORK_SLOTS;
 
+// A traversal's reset in one launch: zeroes the three frontier bitmaps
+// (n_bits words each), the control block, the work counters and the first
+// nq queue lengths; n_all is the largest of these counts.
+__global__ void __launch_bounds__(256)
+    k_reset_frontier(uint64_t *visited, uint64_t *active, uint64_t *inactive,
+                     int64_t n_bits, int32_t *ctrl, unsigned long long *work,
+                     int32_t *qsizes, int64_t nq, int64_t n_all) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_all;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (i < n_bits) {
+      visited[i] = 0;
+      active[i] = 0;
+      inactive[i] = 0;
+    }
+    if (i < CTRL_N) ctrl[i] = 0;
+    if (i < WORK_TOTAL) work[i] = 0;
+    if (i < nq) qsizes[i] = 0;
+  }
+}
+
+// one root active (and not inactive) on cleared frontier bitmaps
+__global__ void __launch_bounds__(64)
+    k_set_root(uint64_t *active, uint64_t *inactive, int64_t root) {
+  if (threadIdx.x == 0) {
+    active[root >> 6] = 1ull << (root & 63);
+    inactive[root >> 6] = 1ull << (root & 63);
+  }
+}
+
 // zero record row `row` in every slot
 __global__ void __launch_bounds__(64)
     k_work_clear_row(unsigned long long *row) {
@@ -153,23 +184,28 @@ struct active_set_t {
   // history and its cursor, as queue_t::reset does in the reference).
   void reset(bool traceback = false) {
     queue.traceback = traceback;
+    int64_t nq = 0;  // queue lengths to clear
     if (queue.mode == Normal) {
-      gsw::fill_async(queue.sizes, 0, queue.max_levels + 2);
+      nq = queue.max_levels + 2;
       queue.host_sizes[0] = queue.host_sizes[1] = 0;
     }
     if (!traceback) {
       queue.cursor = 0;
       queue.offset_in = queue.offset_out = 0;
       if (queue.mode == Cached) {
-        gsw::fill_async(queue.sizes, 0, queue.max_levels + 2);
+        nq = queue.max_levels + 2;
         for (int i = 0; i < queue.max_levels + 2; ++i) queue.host_sizes[i] = 0;
       }
     }
-    gsw::fill_async<uint64_t>(bitmap.visited.dg_bits, 0, bitmap.visited.n_words + 1);
-    gsw::fill_async<uint64_t>(bitmap.active.dg_bits, 0, bitmap.active.n_words + 1);
-    gsw::fill_async<uint64_t>(bitmap.inactive.dg_bits, 0, bitmap.inactive.n_words + 1);
-    gsw::fill_async(dg_ctrl, 0, CTRL_N);
-    gsw::fill_async<unsigned long long>(dg_work_base, 0ull, WORK_TOTAL);
+    const int64_t nb = bitmap.visited.n_words + 1;
+    GSW_REQUIRE(bitmap.active.n_words + 1 == nb && bitmap.inactive.n_words + 1 == nb,
+                gsw::E_INTERNAL_CHECK, "frontier bitmaps of different sizes");
+    const int64_t n_all = std::max(std::max(nb, nq), std::max((int64_t)CTRL_N, WORK_TOTAL));
+    hipLaunchKernelGGL(k_reset_frontier, dim3(gsw::grid_for(n_all, 256)), dim3(256),
+                       0, gsw::stream(), bitmap.visited.dg_bits,
+                       bitmap.active.dg_bits, bitmap.inactive.dg_bits, nb, dg_ctrl,
+                       dg_work_base, queue.sizes, nq, n_all);
+    H_ERR(hipGetLastError());
     dg_work = dg_work_base;
     converged = false;
     window_moved = false;
@@ -177,20 +213,22 @@ struct active_set_t {
 
   // root >= 0: only `root` is active; ALL_ACTIVE: every vertex; ALL_INACTIVE:
   // none (bitmap_t::init, src/data_structures/bitmap.cuh:133-144).
-  void init(int64_t root = ALL_ACTIVE) {
+  // after_reset: called right after reset(), which left both bitmaps zero.
+  void init(int64_t root = ALL_ACTIVE, bool after_reset = false) {
     const int64_t nw = bitmap.active.n_words;
     if (root >= 0) {
       GSW_REQUIRE(root < size, gsw::E_INVALID, "root out of range");
-      gsw::fill_async<uint64_t>(bitmap.active.dg_bits, 0, nw + 1);
-      gsw::fill_async<uint64_t>(bitmap.inactive.dg_bits, 0, nw + 1);
-      gsw::set_one_async<uint64_t>(bitmap.active.dg_bits + (root >> 6),
-                                   1ull << (root & 63));
-      gsw::set_one_async<uint64_t>(bitmap.inactive.dg_bits + (root >> 6),
-                                   1ull << (root & 63));
+      if (!after_reset) {
+        gsw::fill_async<uint64_t>(bitmap.active.dg_bits, 0, nw + 1);
+        gsw::fill_async<uint64_t>(bitmap.inactive.dg_bits, 0, nw + 1);
+      }
+      hipLaunchKernelGGL(k_set_root, dim3(1), dim3(64), 0, gsw::stream(),
+                         bitmap.active.dg_bits, bitmap.inactive.dg_bits, root);
+      H_ERR(hipGetLastError());
     } else if (root == ALL_ACTIVE) {
       gsw::fill_async<uint64_t>(bitmap.active.dg_bits, ~0ull, nw + 1);
       gsw::fill_async<uint64_t>(bitmap.inactive.dg_bits, ~0ull, nw + 1);
-    } else {
+    } else if (!after_reset) {
       gsw::fill_async<uint64_t>(bitmap.active.dg_bits, 0, nw + 1);
       gsw::fill_async<uint64_t>(bitmap.inactive.dg_bits, 0, nw + 1);
     }
@@ -198,10 +236,12 @@ struct active_set_t {
 
   // Several roots at once (a batch of disjoint instances, or multi-source
   // traversals): `d_roots` holds k distinct vertex ids on the device.
-  void init_roots(const int32_t *d_roots, int64_t k) {
+  void init_roots(const int32_t *d_roots, int64_t k, bool after_reset = false) {
     const int64_t nw = bitmap.active.n_words;
-    gsw::fill_async<uint64_t>(bitmap.active.dg_bits, 0, nw + 1);
-    gsw::fill_async<uint64_t>(bitmap.inactive.dg_bits, 0, nw + 1);
+    if (!after_reset) {
+      gsw::fill_async<uint64_t>(bitmap.active.dg_bits, 0, nw + 1);
+      gsw::fill_async<uint64_t>(bitmap.inactive.dg_bits, 0, nw + 1);
+    }
     if (k > 0) {
       hipLaunchKernelGGL(k_set_roots, dim3(gsw::grid_for(k)), dim3(256), 0,
                          gsw::stream(), bitmap.active.dg_bits,
