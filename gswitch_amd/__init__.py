@@ -71,6 +71,8 @@ _SIGS = {
                                C.POINTER(_P)]),
     "gsw_gen_ba": (C.c_int32, [C.c_int64, C.c_int32, C.c_uint64, C.c_int32,
                                C.POINTER(_P)]),
+    "gsw_read_mtx": (C.c_int32, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                 C.POINTER(_P)]),
     "gsw_gen_er_batch": (C.c_int32, [C.c_int32, C.c_int64, C.c_int64, C.c_double,
                                      C.c_double, C.c_uint64, C.c_int32,
                                      C.POINTER(_P), _P]),
@@ -205,6 +207,18 @@ def erdos_renyi(n, mean_degree, seed=5, weighted=False):
 
 def barabasi_albert(n, m, seed=6, weighted=False):
     return _gen(_lib.gsw_gen_ba, n, m, seed, int(weighted))
+
+
+def read_mtx(path, directed=False, weighted=False, header=False, rinse=None):
+    """Matrix Market / edge-list file -> HostGraph (edgelist_t::read_mtx +
+    host_graph_t<CSR>::build, src/utils/fileIO.cuh:73-332). rinse=None does
+    what the reference does: drop degree-zero vertices and renumber, for
+    undirected unweighted graphs only."""
+    h = _P()
+    r = -1 if rinse is None else int(bool(rinse))
+    _check(_lib.gsw_read_mtx(os.fsencode(path), int(directed), int(weighted), int(header),
+                             r, C.byref(h)))
+    return HostGraph(h)
 
 
 def erdos_renyi_batch(count, nmin=1000, nmax=200000, dmin=2.0, dmax=32.0, seed=5,

@@ -22,6 +22,7 @@ ARCH = os.environ.get("GSW_OFFLOAD_ARCH", "gfx950")
 SOURCES = [
     os.path.join(CSRC, "capi.hip"),
     os.path.join(CSRC, "host", "graphgen.cpp"),
+    os.path.join(CSRC, "host", "mtx.cpp"),
 ]
 
 DEVICE_FLAGS = [

@@ -790,6 +790,24 @@ gsw_status gsw_gen_grid2d(int64_t rows, int64_t cols, double p_delete,
   });
 }
 
+gsw_status gsw_read_mtx(const char *path, int32_t directed, int32_t with_weight,
+                        int32_t with_header, int32_t rinse,
+                        gsw_host_graph **out) {
+  return guarded([&] {
+    GSW_REQUIRE(out && path, GSW_E_INVALID, "null argument");
+    const bool r = rinse < 0 ? (!directed && !with_weight) : rinse != 0;
+    auto *h = new gsw_host_graph();
+    try {
+      gsw_host::read_mtx(path, directed != 0, with_weight != 0, with_header != 0, r,
+                         h->g);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
 gsw_status gsw_gen_er(int64_t n, double mean_degree, uint64_t seed,
                       int32_t weighted, gsw_host_graph **out) {
   return guarded([&] {

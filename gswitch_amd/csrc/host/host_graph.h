@@ -9,6 +9,7 @@
 // stable. Offsets are int64 (row_ptr[n+1]) rather than int32 start_pos[n].
 #pragma once
 #include <cstdint>
+#include <string>
 #include <vector>
 
 namespace gsw_host {
@@ -86,6 +87,10 @@ void gen_er_batch(int64_t first, int count, int64_t nmin, int64_t nmax,
                   double dmin, double dmax, uint64_t seed,
                   std::vector<int32_t> &src, std::vector<int32_t> &dst,
                   std::vector<int64_t> &offsets);
+
+// Matrix Market / edge-list file -> CSR (see mtx.cpp for the format rules).
+void read_mtx(const std::string &path, bool directed, bool with_weight,
+              bool with_header, bool rinse, HostGraph &g);
 
 int num_threads();
 

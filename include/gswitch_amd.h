@@ -62,6 +62,18 @@ gsw_status gsw_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b,
 gsw_status gsw_gen_grid2d(int64_t rows, int64_t cols, double p_delete,
                           double long_range_frac, uint64_t seed,
                           int32_t weighted, gsw_host_graph **out);
+/* Matrix Market / edge-list file -> host CSR. Replaces
+ * edgelist_t::read_mtx + host_graph_t<CSR>::build (+ rinse)
+ * (src/utils/fileIO.cuh:73-146, 210-332, 399-443) behind build_graph<C,E>(path,
+ * fets, with_header, with_weight, directed) (graph.cuh:268-280): comment lines
+ * skipped, optional "rows cols nnz" header ignored, self-loops dropped, ids
+ * shifted by the smallest id, a final record without a newline dropped,
+ * integer weights when with_weight. rinse: 1 drops degree-zero vertices and
+ * renumbers the rest (undirected only), 0 keeps every id, -1 does what the
+ * reference does (rinse an undirected unweighted graph only). */
+gsw_status gsw_read_mtx(const char *path, int32_t directed, int32_t with_weight,
+                        int32_t with_header, int32_t rinse,
+                        gsw_host_graph **out);
 gsw_status gsw_gen_er(int64_t n, double mean_degree, uint64_t seed,
                       int32_t weighted, gsw_host_graph **out);
 gsw_status gsw_gen_ba(int64_t n, int32_t m, uint64_t seed, int32_t weighted,

@@ -199,3 +199,111 @@ def test_er_batch_ranges_cut_the_batch(gsw):
         assert p.n == hi - lo
         assert np.array_equal(p.row_ptr, h.row_ptr[lo:hi + 1] - h.row_ptr[lo])
         assert np.array_equal(p.col, h.col[h.row_ptr[lo]:h.row_ptr[hi]] - lo)
+
+
+# ---------------------------------------------------------------- file ingest
+def _write(tmp_path, name, text):
+    p = tmp_path / name
+    p.write_bytes(text.encode())
+    return str(p)
+
+
+def test_read_mtx_matrix_market(gsw, tmp_path):
+    """Banner, comments and the size line skipped (with header); ids shifted
+    to 0; both directions stored; rows sorted; duplicates and self-loops
+    dropped."""
+    p = _write(tmp_path, "a.mtx", "%%MatrixMarket matrix coordinate pattern symmetric\n"
+               "% a comment\n4 4 6\n1 2\n1 3\n3 2\n2 1\n4 4\n3 4\n")
+    h = gsw.read_mtx(p, header=True)
+    assert h.n == 4 and not h.directed and not h.weighted
+    assert list(h.row_ptr) == [0, 2, 4, 7, 8]
+    assert list(h.col) == [1, 2, 0, 2, 0, 1, 3, 2]
+
+
+def test_read_mtx_header_is_an_edge_without_the_flag(gsw, tmp_path):
+    # without with_header the size line "3 3 1" is a self-loop on id 3:
+    # dropped, but it widens the id range to 1..3
+    p = _write(tmp_path, "b.mtx", "3 3 1\n1 2\n")
+    h = gsw.read_mtx(p, rinse=False)
+    assert h.n == 3 and list(h.row_ptr) == [0, 1, 2, 2] and list(h.col) == [1, 0]
+
+
+def test_read_mtx_rinse_and_id_shift(gsw, tmp_path):
+    # ids 10..15; 12 and 14 have no edge (14 only in a self-loop)
+    p = _write(tmp_path, "c.txt", "10 11\n11 13\n14 14\n13 15\n15 10\n")
+    keep = gsw.read_mtx(p, rinse=False)
+    assert keep.n == 6 and list(np.diff(keep.row_ptr)) == [2, 2, 0, 2, 0, 2]
+    r = gsw.read_mtx(p)  # the reference's default for an undirected unweighted graph
+    assert r.n == 4
+    # kept ids 10, 11, 13, 15 -> 0, 1, 2, 3
+    assert list(r.row_ptr) == [0, 2, 4, 6, 8]
+    assert list(r.col) == [1, 3, 0, 2, 1, 3, 0, 2]
+
+
+def test_read_mtx_last_line_without_newline(gsw, tmp_path):
+    """The reference's stream loop drops a final record not ended by a
+    newline; an unweighted record with extra tokens still counts its ids."""
+    h = gsw.read_mtx(_write(tmp_path, "d.txt", "1 2\n2 3"), rinse=False)
+    assert h.n == 2 and list(h.col) == [1, 0]
+    h = gsw.read_mtx(_write(tmp_path, "e.txt", "1 2\n2 5 9"), rinse=False)
+    assert h.n == 5 and list(h.col) == [1, 0]
+    h = gsw.read_mtx(_write(tmp_path, "f.txt", "1 2\n2 3\n"), rinse=False)
+    assert h.n == 3 and list(h.col) == [1, 0, 2, 1]
+
+
+def test_read_mtx_weights_and_directed(gsw, tmp_path):
+    p = _write(tmp_path, "g.txt", "0 1 7\n1 2 3\n0 1 5\n2 0 9\n")
+    h = gsw.read_mtx(p, weighted=True)
+    assert h.weighted and h.n == 3
+    assert list(h.row_ptr) == [0, 2, 4, 6]
+    assert list(h.col) == [1, 2, 0, 2, 0, 1]
+    # duplicate 0-1 keeps its smaller weight (shortest paths are unchanged)
+    assert list(h.w) == [5, 9, 5, 3, 9, 3]
+    d = gsw.read_mtx(p, directed=True)
+    assert d.directed and d.n == 3
+    assert list(d.row_ptr) == [0, 1, 2, 3] and list(d.col) == [1, 2, 0]
+    with pytest.raises(gsw.GswError):
+        gsw.read_mtx(p, directed=True, rinse=True)
+
+
+def test_read_mtx_errors(gsw, tmp_path):
+    with pytest.raises(gsw.GswError):
+        gsw.read_mtx(str(tmp_path / "missing.mtx"))
+    with pytest.raises(gsw.GswError):
+        gsw.read_mtx(_write(tmp_path, "h.txt", "% only a comment\n"))
+    with pytest.raises(gsw.GswError):
+        gsw.read_mtx(_write(tmp_path, "i.txt", "1 1\n2 2\n"))  # self-loops only
+    with pytest.raises(gsw.GswError):
+        gsw.read_mtx(_write(tmp_path, "j.txt", "1 2 0.5\n2 3 1\n"), weighted=True)
+
+
+def write_mtx(path, h, base=1):
+    """Each undirected edge once (u < v), Matrix Market coordinate layout."""
+    u = np.repeat(np.arange(h.n), np.diff(h.row_ptr))
+    keep = u < h.col
+    with open(path, "w") as f:
+        f.write("%%MatrixMarket matrix coordinate pattern symmetric\n")
+        f.write("%d %d %d\n" % (h.n, h.n, int(keep.sum())))
+        np.savetxt(f, np.stack([u[keep] + base, h.col[keep] + base], 1), fmt="%d")
+
+
+def test_read_mtx_round_trip(gsw, tmp_path):
+    h = gsw.rmat(10, 8, seed=3)
+    p = str(tmp_path / "r.mtx")
+    write_mtx(p, h)
+    # vertex 0 of R-MAT has edges, so the id shift is 1 (1-based file); the
+    # id range ends at the largest id in an edge (n = vmax - vmin + 1), so
+    # keeping every id gives back the generator's CSR up to that id
+    live = np.flatnonzero(np.diff(h.row_ptr) > 0)
+    assert live[0] == 0
+    back = gsw.read_mtx(p, header=True, rinse=False)
+    assert back.n == live[-1] + 1 < h.n
+    assert np.array_equal(back.row_ptr, h.row_ptr[:back.n + 1])
+    assert np.array_equal(back.col, h.col)
+    # rinse: isolated vertices removed, the rest renumbered in order
+    r = gsw.read_mtx(p, header=True)
+    assert r.n == len(live)
+    assert np.array_equal(np.diff(r.row_ptr), np.diff(h.row_ptr)[live])
+    rename = np.full(h.n, -1, np.int64)
+    rename[live] = np.arange(len(live))
+    assert np.array_equal(r.col, rename[h.col])
