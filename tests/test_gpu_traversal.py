@@ -228,3 +228,16 @@ def test_traversal_on_a_matrix_market_file(gsw, orc, tmp_path):
     assert np.array_equal(g.bfs(root), orc.bfs(h.row_ptr, h.col, root))
     assert np.array_equal(g.cc(), orc.cc(h.row_ptr, h.col))
     g.close()
+
+
+@pytest.mark.parametrize("app", ["bfs", "sssp", "cc", "pagerank", "bc"])
+def test_command_line_driver(gsw, tmp_path, capsys, app):
+    """The command-line driver runs each application on a Matrix Market
+    file and its --validation passes."""
+    from test_host_graph import write_mtx
+    from gswitch_amd import cli
+    p = str(tmp_path / "g.mtx")
+    write_mtx(p, gsw.rmat(11, 16, seed=6))
+    assert cli.main([app, p, "-H", "-V"]) == 0
+    out = capsys.readouterr().out
+    assert ("validation: PASS" in out) == (app != "bc")
