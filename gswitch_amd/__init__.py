@@ -71,6 +71,7 @@ _SIGS = {
                                C.POINTER(_P)]),
     "gsw_gen_ba": (C.c_int32, [C.c_int64, C.c_int32, C.c_uint64, C.c_int32,
                                C.POINTER(_P)]),
+    "gsw_record_features": (C.c_int32, [C.c_char_p]),
     "gsw_read_mtx": (C.c_int32, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                  C.POINTER(_P)]),
     "gsw_gen_er_batch": (C.c_int32, [C.c_int32, C.c_int64, C.c_int64, C.c_double,
@@ -207,6 +208,25 @@ def erdos_renyi(n, mean_degree, seed=5, weighted=False):
 
 def barabasi_albert(n, m, seed=6, weighted=False):
     return _gen(_lib.gsw_gen_ba, n, m, seed, int(weighted))
+
+
+def record_features(path):
+    """Append the reference's per-level training records (JSON objects with
+    keys "0".."22") to `path` from now on, at most 100; None turns it off."""
+    _check(_lib.gsw_record_features(None if path is None else os.fsencode(path)))
+
+
+def read_records(path):
+    """The JSON objects a record file holds, in order."""
+    import json
+    txt, dec, out, i = open(path).read(), json.JSONDecoder(), [], 0
+    while True:
+        while i < len(txt) and txt[i].isspace():
+            i += 1
+        if i >= len(txt):
+            return out
+        obj, i = dec.raw_decode(txt, i)
+        out.append(obj)
 
 
 def read_mtx(path, directed=False, weighted=False, header=False, rinse=None):

@@ -257,6 +257,7 @@ int run_vc(engine_t &en, active_set_t &as, G &g, F &f, const run_opts_t &o,
     en.executor.filter(as, g, f, en.stats, en.fets, en.conf);
     g.update_level();
     en.executor.expand(as, g, f, en.stats, en.fets, en.conf);
+    en.fets.record();  // a training record when a record file is set
     if (o.verbose) {
       const auto &r = en.executor.log.back();
       fprintf(stderr, "level %d frontier %lld ", r.level, r.frontier);

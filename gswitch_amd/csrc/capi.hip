@@ -790,6 +790,14 @@ gsw_status gsw_gen_grid2d(int64_t rows, int64_t cols, double p_delete,
   });
 }
 
+gsw_status gsw_record_features(const char *path) {
+  return guarded([&] {
+    record_sink_t &k = record_sink();
+    k.path = path ? path : "";
+    k.written = 0;
+  });
+}
+
 gsw_status gsw_read_mtx(const char *path, int32_t directed, int32_t with_weight,
                         int32_t with_header, int32_t rinse,
                         gsw_host_graph **out) {

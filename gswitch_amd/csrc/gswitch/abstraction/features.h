@@ -5,7 +5,22 @@
 // and history (last/avg filter and expand time). cross() folds the
 // active and inactive sides together for from-all / to-all algorithms.
 #pragma once
+#include <fstream>
+#include <sstream>
+#include <string>
+
 #include "../utils/common.h"
+
+// Destination of feature_t::record(): a file the records are appended to
+// ("" = off) and how many were written since it was set.
+struct record_sink_t {
+  std::string path;
+  int written = 0;
+};
+inline record_sink_t &record_sink() {
+  static record_sink_t s;
+  return s;
+}
 
 struct feature_t {
   // Algorithm
@@ -69,6 +84,29 @@ struct feature_t {
     fv[19] = DIV(last_filter_time, avg_filter_time);
     fv[20] = DIV(last_expand_time, avg_expand_time);
     cross();
+  }
+
+  // One training record in the reference's format (feature_t::record,
+  // src/abstraction/features.cuh:86-94 with json_t::dump,
+  // src/utils/json.cuh:31-44): keys "0".."20" = fv, "21" = last filter
+  // time, "22" = last expand time (ms), values printed as an ostream prints
+  // a double, the object appended to the file; at most 100 records per
+  // file setting, as json_t stops after 100 dumps.
+  void record() {
+    record_sink_t &k = record_sink();
+    if (k.path.empty() || k.written >= 100) return;
+    flatten();
+    std::ostringstream o;
+    o << "{\n";
+    for (int i = 0; i < 23; ++i) {
+      const double x = i < 21 ? fv[i] : (i == 21 ? last_filter_time : last_expand_time);
+      if (i > 0) o << ",\n";
+      o << "\"" << i << "\": " << x;
+    }
+    o << "\n}\n";
+    std::ofstream out(k.path, std::ios::app);
+    out << o.str();
+    ++k.written;
   }
 
   void cross() {

@@ -62,6 +62,12 @@ gsw_status gsw_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b,
 gsw_status gsw_gen_grid2d(int64_t rows, int64_t cols, double p_delete,
                           double long_range_frac, uint64_t seed,
                           int32_t weighted, gsw_host_graph **out);
+/* Training records in the reference's format (feature_t::record,
+ * src/abstraction/features.cuh:86-104, enabled there by --json=<path>):
+ * from now on every expanded level of a vertex-centric run appends one JSON
+ * object {"0": fv[0], ..., "20": fv[20], "21": last filter ms, "22": last
+ * expand ms} to `path`, at most 100 per call. NULL or "" turns it off. */
+gsw_status gsw_record_features(const char *path);
 /* Matrix Market / edge-list file -> host CSR. Replaces
  * edgelist_t::read_mtx + host_graph_t<CSR>::build (+ rinse)
  * (src/utils/fileIO.cuh:73-146, 210-332, 399-443) behind build_graph<C,E>(path,

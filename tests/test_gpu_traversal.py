@@ -241,3 +241,30 @@ def test_command_line_driver(gsw, tmp_path, capsys, app):
     assert cli.main([app, p, "-H", "-V"]) == 0
     out = capsys.readouterr().out
     assert ("validation: PASS" in out) == (app != "bc")
+
+
+def test_training_records(gsw, gset, tmp_path):
+    """gsw_record_features appends one reference-format record per expanded
+    level (keys "0".."22": the 21 features, then filter and expand times),
+    at most 100 per setting."""
+    h, g = gset["rmat12"]
+    root = h.max_degree_vertex()
+    p = str(tmp_path / "rec.json")
+    gsw.record_features(p)
+    try:
+        g.bfs(root)
+        levels = g.records()
+        recs = gsw.read_records(p)
+        assert len(recs) == len(levels) > 0
+        for r, lv in zip(recs, levels):
+            assert sorted(r, key=int) == [str(i) for i in range(23)]
+            assert r["0"] == h.n and r["1"] == h.m // 2
+            if not lv["fused"]:
+                assert r["7"] == lv["active"], (r["7"], lv)
+        for _ in range(100 // len(levels) + 1):
+            g.bfs(root, out=False)
+        assert len(gsw.read_records(p)) == 100
+    finally:
+        gsw.record_features(None)
+    g.bfs(root, out=False)
+    assert len(gsw.read_records(p)) == 100

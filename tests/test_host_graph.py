@@ -307,3 +307,14 @@ def test_read_mtx_round_trip(gsw, tmp_path):
     rename = np.full(h.n, -1, np.int64)
     rename[live] = np.arange(len(live))
     assert np.array_equal(r.col, rename[h.col])
+
+
+def test_record_file_parsing(gsw, tmp_path):
+    """Record files hold concatenated JSON objects (the reference's json_t
+    dump layout); setting and clearing the file needs no GPU."""
+    p = tmp_path / "r.json"
+    p.write_text('{\n"0": 5,\n"22": 0.25\n}\n{\n"0": 6,\n"22": 1e-05\n}\n')
+    recs = gsw.read_records(str(p))
+    assert [r["0"] for r in recs] == [5, 6] and recs[1]["22"] == 1e-05
+    gsw.record_features(str(tmp_path / "x.json"))
+    gsw.record_features(None)
