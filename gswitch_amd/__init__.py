@@ -98,6 +98,7 @@ _SIGS = {
     "gsw_cc": (C.c_int32, [_P, C.POINTER(RunOpts), _P, C.POINTER(RunStats)]),
     "gsw_pagerank": (C.c_int32, [_P, C.c_float, C.c_int32, C.c_float,
                                  C.POINTER(RunOpts), _P, C.POINTER(RunStats)]),
+    "gsw_gc": (C.c_int32, [_P, C.POINTER(RunOpts), _P, C.POINTER(RunStats)]),
     "gsw_bc": (C.c_int32, [_P, C.c_int32, C.POINTER(RunOpts), _P, _P, _P,
                            C.POINTER(RunStats)]),
     "gsw_last_records": (C.c_int32, [_P, _P, C.c_int32, C.POINTER(C.c_int32)]),
@@ -440,6 +441,15 @@ class Graph:
                                  C.byref(st)))
         self.last_stats = st
         return pr
+
+    def gc(self, configs=None, out=True, **kw):
+        """Graph colouring (Jones-Plassmann, id priorities): colour per vertex."""
+        o = self._opts(configs, kw)
+        st = RunStats()
+        col = np.empty(self.n, np.int32) if out else None
+        _check(_lib.gsw_gc(self._h, C.byref(o), _ptr(col), C.byref(st)))
+        self.last_stats = st
+        return col
 
     def bc(self, root, configs=None, out=True, **kw):
         """Betweenness dependencies from `root`: (levels, nsp, dep)."""

@@ -26,7 +26,8 @@ BFS levels grow by at most one along every edge out of a reached vertex and
 every reached vertex has an in-neighbour one level up; SSSP distances satisfy every edge's triangle
 inequality and every reached vertex has a tight in-edge; CC labels agree
 across every edge, no label exceeds its vertex id and every label is a
-vertex labelled by itself; PageRank's residual (the change one more
+vertex labelled by itself; GC colours every vertex and no edge joins two vertices of
+one colour; PageRank's residual (the change one more
 iteration would make) is small after 30 or more iterations. These are
 necessary conditions, not a second computation.
 """
@@ -38,7 +39,7 @@ import numpy as np
 
 import gswitch_amd as gsw
 
-APPS = ("bfs", "sssp", "cc", "pagerank", "bc")
+APPS = ("bfs", "sssp", "cc", "pagerank", "bc", "gc")
 
 
 def edge_sources(h):
@@ -96,6 +97,13 @@ def check_cc(h, lab):
     if np.any(lab[u] != lab[v]) or np.any(lab > ids) or np.any(lab[lab] != lab):
         return False
     return True
+
+
+def check_gc(h, color):
+    """Every vertex coloured and no edge joins two vertices of one colour
+    (the reference's validation, application/gc.cu:45-74)."""
+    u, v = edge_sources(h), h.col
+    return bool(np.all(color >= 1) and np.all(color[u] != color[v]))
 
 
 def pagerank_residual(h, pr, damping):
@@ -165,6 +173,10 @@ def main(argv=None):
                 # each iteration shrinks the change by about the damping
                 # factor: 0.85^30 ~ 8e-3
                 ok = r < 0.05
+    elif a.app == "gc":
+        res = g.gc(configs=a.configs)
+        if a.validation:
+            ok = check_gc(h, res)
     else:
         res = g.bc(root, configs=a.configs)
     st = g.last_stats

@@ -110,6 +110,36 @@ struct PR : Functor<VC, double, float, Empty> {
   }
 };
 
+// ---- graph colouring (application/gc.cu:15-43) ---------------------------
+// Jones-Plassmann rounds with vertex ids as priorities: each round every
+// uncoloured vertex pulls the largest id among its uncoloured neighbours
+// (WA, reset to -1 by the filter); an uncoloured vertex whose own id is
+// larger takes the round number as its colour (RA) and leaves the frontier.
+// Colours start at 1; coloured vertices are Fixed.
+struct Color : Functor<VC, int, int, Empty> {
+  __device__ Status filter(int v, GU g) {
+    int *color = ra_of(v);
+    int *maxid = wa_of(v);
+    if (*color != -1) return Fixed;
+    if (*maxid < v) {
+      *color = g.get_level();
+      return Fixed;
+    }
+    *maxid = -1;
+    return Active;
+  }
+  __device__ int emit(int v, Empty *e, GU g) { return *ra_of(v) == -1 ? v : -1; }
+  __device__ bool cond(int v, int msg, GU g) { return *ra_of(v) == -1; }
+  __device__ bool comp(int *v, int msg, GU g) {
+    if (*v < msg) *v = msg;
+    return true;
+  }
+  __device__ bool compAtomic(int *v, int msg, GU g) {
+    atomicMax(v, msg);
+    return true;
+  }
+};
+
 // ---- betweenness centrality from one root (application/bc.cu:16-94) -----
 // Forward pass: level + number of shortest paths (nsp); backward pass:
 // Brandes dependency accumulated from the deepest level up. nsp is an FP64

@@ -167,6 +167,7 @@ struct gsw_graph {
   std::unique_ptr<app_ctx_t<SSSP>> sssp;
   std::unique_ptr<app_ctx_t<CC>> cc;
   std::unique_ptr<app_ctx_t<PR>> pr;
+  std::unique_ptr<app_ctx_t<Color>> gc;
   // BC: the forward functor owns the frontier and engine; the backward
   // functor has its own vertex arrays
   std::unique_ptr<app_ctx_t<BCF>> bc;
@@ -208,6 +209,7 @@ struct gsw_graph {
     if (sssp) sssp->release();
     if (cc) cc->release();
     if (pr) pr->release();
+    if (gc) gc->release();
     if (bc) bc->release();
     if (bcb_built) bcb.data.release();
     if (d_bc_out) hipFree(d_bc_out);
@@ -1334,6 +1336,54 @@ gsw_status gsw_pagerank(gsw_graph *G, float damping, int32_t iters, float eps,
     collect_records(G, c.en.executor, c.as);
     if (pr_out)
       H_ERR(hipMemcpy(pr_out, G->d_out_f, sizeof(float) * G->hg.n,
+                      hipMemcpyDeviceToHost));
+    if (stats) {
+      stats->device_ms = ms;
+      stats->levels = levels;
+      stats->n_records = (int32_t)G->records.size();
+      stats->reached = 0;
+    }
+  });
+}
+
+// run_gc (application/gc.cu:76-98, main :100-127): from-all, to-all,
+// associative, Fixed vertices leave the frontier for good.
+gsw_status gsw_gc(gsw_graph *G, const gsw_run_opts *opts, int32_t *colors_out,
+                  gsw_run_stats *stats) {
+  return guarded([&] {
+    GSW_REQUIRE(G, GSW_E_INVALID, "null graph");
+    const run_opts_t o = to_opts_rule(opts);
+    if (!G->gc) {
+      G->gc = std::make_unique<app_ctx_t<Color>>();
+      G->gc->as.build(G->hg.n, Normal);
+      G->gc->f.data.build(G->hg.n);
+    }
+    auto &c = *G->gc;
+    arch_once(c);
+    prepare_engine(c, o, !opts || opts->profile);
+    feature_t &fets = c.en.fets;
+    fets.centric = VC;
+    fets.pattern = ASSO;
+    fets.fromall = true;
+    fets.toall = true;
+    c.en.conf.conf_fixed = true;
+    fets.set_graph((double)G->hg.n, (double)(G->hg.m / 2), G->attr);
+    view_t<GU> gv{&G->hg, G->attr, 0.0};
+    init_conf(c.en.stats, fets, c.en.conf, gv, c.f);
+
+    hipStream_t st = gsw::stream();
+    H_ERR(hipEventRecord(G->ev0, st));
+    c.as.reset();
+    c.as.init(ALL_ACTIVE);
+    gsw::fill_async<int32_t>(c.f.data.dg_wa, -1, G->hg.n);
+    gsw::fill_async<int32_t>(c.f.data.dg_ra, -1, G->hg.n);
+    GU g = G->dg;
+    const int levels = run_vc(c.en, c.as, g, c.f, o, [](int) { return false; });
+    H_ERR(hipEventRecord(G->ev1, st));
+    const double ms = elapsed(G);
+    collect_records(G, c.en.executor, c.as);
+    if (colors_out)
+      H_ERR(hipMemcpy(colors_out, c.f.data.dg_ra, sizeof(int32_t) * G->hg.n,
                       hipMemcpyDeviceToHost));
     if (stats) {
       stats->device_ms = ms;

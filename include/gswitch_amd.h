@@ -170,6 +170,11 @@ gsw_status gsw_cc(gsw_graph *g, const gsw_run_opts *opts, int32_t *labels_out,
 gsw_status gsw_pagerank(gsw_graph *g, float damping, int32_t iters, float eps,
                         const gsw_run_opts *opts, float *pr_out,
                         gsw_run_stats *stats);
+/* Graph colouring, run_gc (application/gc.cu:15-127): Jones-Plassmann rounds
+ * with vertex ids as priorities; colors_out[v] = the round (from 1) in which
+ * v's id first exceeded every uncoloured neighbour's. */
+gsw_status gsw_gc(gsw_graph *g, const gsw_run_opts *opts, int32_t *colors_out,
+                  gsw_run_stats *stats);
 /* Betweenness dependencies from one root (Brandes): levels_out = BFS level
  * (-1 unreached), nsp_out = number of shortest root->v paths, dep_out =
  * dependency of the root on v. Any output may be null. Replaces run_bc

@@ -29,6 +29,7 @@ for name, args, res in [
     ("orc_bellman_ford", [C.c_int64, _P, _P, _P, C.c_int32, _P], None),
     ("orc_cc", [C.c_int64, _P, _P, _P], None),
     ("orc_cc_unionfind", [C.c_int64, _P, _P, _P], None),
+    ("orc_gc", [C.c_int64, _P, _P, _P], None),
     ("orc_pagerank", [C.c_int64, _P, _P, C.c_double, C.c_int, _P], None),
     ("orc_bc", [C.c_int64, _P, _P, C.c_int32, _P, _P, _P], None),
     ("orc_msbfs64", [C.c_int64, _P, _P, _P, C.c_int, _P], None),
@@ -79,6 +80,14 @@ def cc(row_ptr, col):
     rp, cl = _csr(row_ptr, col)
     out = np.empty(len(rp) - 1, np.int32)
     _lib.orc_cc(len(rp) - 1, _p(rp), _p(cl), _p(out))
+    return out
+
+
+def gc(row_ptr, col):
+    """Jones-Plassmann colouring rounds with id priorities (colours from 1)."""
+    rp, cl = _csr(row_ptr, col)
+    out = np.empty(len(rp) - 1, np.int32)
+    _lib.orc_gc(len(rp) - 1, _p(rp), _p(cl), _p(out))
     return out
 
 

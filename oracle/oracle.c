@@ -226,6 +226,34 @@ void orc_pagerank(int64_t n, const int64_t *row_ptr, const int32_t *col,
   free(contrib);
 }
 
+/* ---- graph colouring: application/gc.cu:15-98 -----------------------------
+ * Jones-Plassmann rounds r = 1, 2, ...: from the colours at the start of the
+ * round, every uncoloured v finds the largest id among its uncoloured
+ * neighbours; v takes colour r when its own id is larger. */
+void orc_gc(int64_t n, const int64_t *row_ptr, const int32_t *col,
+            int32_t *color) {
+  int32_t *maxid = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  for (int64_t v = 0; v < n; ++v) color[v] = -1;
+  int64_t left = n;
+  for (int32_t r = 1; left > 0; ++r) {
+    for (int64_t v = 0; v < n; ++v) {
+      if (color[v] != -1) continue;
+      int32_t m = -1;
+      for (int64_t j = row_ptr[v]; j < row_ptr[v + 1]; ++j) {
+        const int32_t u = col[j];
+        if (color[u] == -1 && u > m) m = u;
+      }
+      maxid[v] = m;
+    }
+    for (int64_t v = 0; v < n; ++v)
+      if (color[v] == -1 && maxid[v] < v) {
+        color[v] = r;
+        --left;
+      }
+  }
+  free(maxid);
+}
+
 /* ---- Brandes BC from one root: application/bc.cu:162-214 ----------------
  * Forward: BFS; every edge v->u with level[u] == level[v]+1 adds nsp[v] to
  * nsp[u]. Backward, in reverse BFS order: dep[v] *= nsp[v]; every edge v->u

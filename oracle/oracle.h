@@ -47,6 +47,9 @@ void orc_cc(int64_t n, const int64_t *row_ptr, const int32_t *col,
 
 /* Independent check for orc_cc: union-find over the edge list with the
  * root chosen as the smaller id. */
+/* Graph colouring rounds (application/gc.cu:15-98), colours from 1. */
+void orc_gc(int64_t n, const int64_t *row_ptr, const int32_t *col,
+            int32_t *color);
 void orc_cc_unionfind(int64_t n, const int64_t *row_ptr, const int32_t *col,
                       int32_t *label);
 

@@ -78,3 +78,17 @@ def test_cli_reads_the_file_before_any_gpu_call(gsw, tmp_path):
     if gsw.device_count() < 1:
         with pytest.raises(gsw.GswError):  # no GPU here: fails loudly
             cli.main(["bfs", str(p)])
+
+
+def test_check_gc(gsw, orc):
+    h = gsw.erdos_renyi(2000, 6.0, 9)
+    c = orc.gc(h.row_ptr, h.col)
+    assert cli.check_gc(h, c)
+    u = np.repeat(np.arange(h.n), np.diff(h.row_ptr))
+    e = int(np.flatnonzero(u < h.col)[0])
+    bad = c.copy()
+    bad[h.col[e]] = bad[u[e]]  # two neighbours share a colour
+    assert not cli.check_gc(h, bad)
+    bad = c.copy()
+    bad[5] = -1  # uncoloured
+    assert not cli.check_gc(h, bad)

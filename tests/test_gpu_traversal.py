@@ -230,7 +230,7 @@ def test_traversal_on_a_matrix_market_file(gsw, orc, tmp_path):
     g.close()
 
 
-@pytest.mark.parametrize("app", ["bfs", "sssp", "cc", "pagerank", "bc"])
+@pytest.mark.parametrize("app", ["bfs", "sssp", "cc", "pagerank", "bc", "gc"])
 def test_command_line_driver(gsw, tmp_path, capsys, app):
     """The command-line driver runs each application on a Matrix Market
     file and its --validation passes."""
@@ -268,3 +268,19 @@ def test_training_records(gsw, gset, tmp_path):
         gsw.record_features(None)
     g.bfs(root, out=False)
     assert len(gsw.read_records(p)) == 100
+
+
+# ---------------------------------------------------------------- colouring
+@pytest.mark.parametrize("variant", [None, "Pull-Queue-TM-Standalone",
+                                     "Pull-Bitmap-TWOD-Standalone",
+                                     "Pull-Bitmap-STRICT-Standalone",
+                                     "Pull-Queue-WM-Standalone",
+                                     "Push-Queue-WM-Standalone",
+                                     "Push-Bitmap-CM-Standalone"])
+def test_gc(gsw, orc, gset, variant):
+    """Graph colouring (Jones-Plassmann rounds, id priorities) bit-exact
+    against the oracle, whatever the variant."""
+    for gname in ["rmat12", "grid", "er", "ba"]:
+        h, g = gset[gname]
+        assert np.array_equal(g.gc(configs=variant), orc.gc(h.row_ptr, h.col)), \
+            (variant, gname)

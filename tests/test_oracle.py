@@ -235,3 +235,47 @@ def test_traversed_edges(orc):
     rp, col = csr_from_edges(6, [(0, 1), (1, 2), (3, 4)])
     lv = orc.bfs(rp, col, 0)
     assert orc.traversed_edges(rp, lv) == 2
+
+
+# ---------------------------------------------------------------- colouring
+def gc_recurrence(n, row_ptr, col):
+    """Independent form of id-priority Jones-Plassmann: a vertex is coloured
+    in the round after all its higher-id neighbours, so colour(v) = 1 + the
+    largest colour among neighbours with a larger id (0 if none); computed
+    in decreasing id order."""
+    color = np.zeros(n, np.int64)
+    for v in range(n - 1, -1, -1):
+        nb = col[row_ptr[v]:row_ptr[v + 1]]
+        hi = nb[nb > v]
+        color[v] = 1 + (color[hi].max() if len(hi) else 0)
+    return color
+
+
+def test_gc_closed_forms(orc):
+    # path 0-1-...-(n-1): only the last vertex beats its neighbours, then the
+    # one before it, ...: colour(v) = n - v
+    n = 9
+    rp, col = csr_from_edges(n, [(i, i + 1) for i in range(n - 1)])
+    assert list(orc.gc(rp, col)) == [n - v for v in range(n)]
+    # star centred on 0: the leaves all colour in round 1, the centre in 2
+    rp, col = csr_from_edges(6, [(0, i) for i in range(1, 6)])
+    assert list(orc.gc(rp, col)) == [2, 1, 1, 1, 1, 1]
+    # complete graph: one vertex per round, highest id first
+    k = 5
+    rp, col = csr_from_edges(k, [(i, j) for i in range(k) for j in range(i + 1, k)])
+    assert list(orc.gc(rp, col)) == [k - v for v in range(k)]
+    # isolated vertices colour in round 1
+    rp, col = csr_from_edges(3, [])
+    assert list(orc.gc(rp, col)) == [1, 1, 1]
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_gc_matches_recurrence_and_is_proper(orc, seed):
+    rng = np.random.default_rng(seed)
+    n = 300
+    e = rng.integers(0, n, size=(900, 2))
+    rp, col = csr_from_edges(n, [tuple(x) for x in e if x[0] != x[1]])
+    c = orc.gc(rp, col)
+    assert np.array_equal(c, gc_recurrence(n, rp, col))
+    u = np.repeat(np.arange(n), np.diff(rp))
+    assert np.all(c >= 1) and np.all(c[u] != c[col])
