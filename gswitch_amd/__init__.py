@@ -99,6 +99,8 @@ _SIGS = {
     "gsw_pagerank": (C.c_int32, [_P, C.c_float, C.c_int32, C.c_float,
                                  C.POINTER(RunOpts), _P, C.POINTER(RunStats)]),
     "gsw_gc": (C.c_int32, [_P, C.POINTER(RunOpts), _P, C.POINTER(RunStats)]),
+    "gsw_delta_pagerank": (C.c_int32, [_P, C.c_float, C.c_double, C.c_int32,
+                                       C.POINTER(RunOpts), _P, C.POINTER(RunStats)]),
     "gsw_bc": (C.c_int32, [_P, C.c_int32, C.POINTER(RunOpts), _P, _P, _P,
                            C.POINTER(RunStats)]),
     "gsw_last_records": (C.c_int32, [_P, _P, C.c_int32, C.POINTER(C.c_int32)]),
@@ -439,6 +441,18 @@ class Graph:
         pr = np.empty(self.n, np.float32) if out else None
         _check(_lib.gsw_pagerank(self._h, damping, iters, eps, C.byref(o), _ptr(pr),
                                  C.byref(st)))
+        self.last_stats = st
+        return pr
+
+    def delta_pagerank(self, damping=0.85, eps=1e-9, max_iters=1000, configs=None,
+                       out=True, **kw):
+        """Delta-PageRank: the converged PageRank values (same form as
+        pagerank()), sending only changes larger than eps."""
+        o = self._opts(configs, kw)
+        st = RunStats()
+        pr = np.empty(self.n, np.float32) if out else None
+        _check(_lib.gsw_delta_pagerank(self._h, damping, eps, max_iters, C.byref(o),
+                                       _ptr(pr), C.byref(st)))
         self.last_stats = st
         return pr
 

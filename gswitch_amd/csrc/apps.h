@@ -110,6 +110,47 @@ struct PR : Functor<VC, double, float, Empty> {
   }
 };
 
+// ---- Delta-PageRank (reference docs/index.md:175-179, not in its code) ----
+// Each vertex keeps its value and the delta it sends this round (already
+// divided by its out-degree) in RA; WA accumulates the damped deltas that
+// arrive. The filter applies an accumulation once it exceeds eps (the vertex
+// becomes active and sends it on next round) and otherwise keeps it
+// accumulating, so no mass is lost and the values converge to the PageRank
+// fixed point value = (1 - d) + d * sum value[u] / degree[u].
+struct dpr_t {
+  double pr;
+  double send;
+};
+struct DeltaPR : Functor<VC, double, dpr_t, Empty> {
+  float damp;
+  double eps;
+  __device__ Status filter(int v, GU g) {
+    double *acc = wa_of(v);
+    dpr_t *s = ra_of(v);
+    const double a = *acc;
+    if (a > eps || a < -eps) {
+      const int od = g.get_out_degree(v);
+      s->pr += a;
+      s->send = od ? a / (double)od : 0.0;
+      *acc = 0;
+      return Active;
+    }
+    s->send = 0;
+    return Inactive;
+  }
+  __device__ double emit(int v, Empty *e, GU g) {
+    return (double)damp * ra_of(v)->send;
+  }
+  __device__ bool comp(double *v, double msg, GU g) {
+    *v += msg;
+    return true;
+  }
+  __device__ bool compAtomic(double *v, double msg, GU g) {
+    atomicAdd(v, msg);
+    return true;
+  }
+};
+
 // ---- graph colouring (application/gc.cu:15-43) ---------------------------
 // Jones-Plassmann rounds with vertex ids as priorities: each round every
 // uncoloured vertex pulls the largest id among its uncoloured neighbours

@@ -86,6 +86,21 @@ __global__ void k_init_pr(double *wa, float *ra, const int64_t *row_ptr,
     ra[v] = od ? (1.0f - damp) / (float)od : (1.0f - damp);
   }
 }
+__global__ void k_init_dpr(double *wa, dpr_t *ra, const int64_t *row_ptr,
+                           int64_t n, float damp) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t od = row_ptr[v + 1] - row_ptr[v];
+    wa[v] = 0.0;
+    ra[v].pr = 1.0 - (double)damp;
+    ra[v].send = od ? (1.0 - (double)damp) / (double)od : 0.0;
+  }
+}
+__global__ void k_dpr_output(const dpr_t *ra, int64_t n, float *out) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    out[v] = (float)ra[v].pr;
+}
 __global__ void k_init_bc(bc_fwd_t *wa, int64_t n, int root) {
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += (int64_t)gridDim.x * blockDim.x) {
@@ -168,6 +183,7 @@ struct gsw_graph {
   std::unique_ptr<app_ctx_t<CC>> cc;
   std::unique_ptr<app_ctx_t<PR>> pr;
   std::unique_ptr<app_ctx_t<Color>> gc;
+  std::unique_ptr<app_ctx_t<DeltaPR>> dpr;
   // BC: the forward functor owns the frontier and engine; the backward
   // functor has its own vertex arrays
   std::unique_ptr<app_ctx_t<BCF>> bc;
@@ -210,6 +226,7 @@ struct gsw_graph {
     if (cc) cc->release();
     if (pr) pr->release();
     if (gc) gc->release();
+    if (dpr) dpr->release();
     if (bc) bc->release();
     if (bcb_built) bcb.data.release();
     if (d_bc_out) hipFree(d_bc_out);
@@ -1331,6 +1348,69 @@ gsw_status gsw_pagerank(gsw_graph *G, float damping, int32_t iters, float eps,
     hipLaunchKernelGGL(k_pr_output, dim3(gsw::grid_for(G->hg.n)),
                        dim3(gsw::BLOCK), 0, st, c.f.data.dg_ra,
                        G->dg.dg_row_ptr, G->hg.n, G->d_out_f);
+    H_ERR(hipEventRecord(G->ev1, st));
+    const double ms = elapsed(G);
+    collect_records(G, c.en.executor, c.as);
+    if (pr_out)
+      H_ERR(hipMemcpy(pr_out, G->d_out_f, sizeof(float) * G->hg.n,
+                      hipMemcpyDeviceToHost));
+    if (stats) {
+      stats->device_ms = ms;
+      stats->levels = levels;
+      stats->n_records = (int32_t)G->records.size();
+      stats->reached = 0;
+    }
+  });
+}
+
+// Delta-PageRank: every vertex starts active with value 1 - d and sends it;
+// the run ends when no accumulation exceeds eps (or after max_iters rounds).
+gsw_status gsw_delta_pagerank(gsw_graph *G, float damping, double eps,
+                              int32_t max_iters, const gsw_run_opts *opts,
+                              float *pr_out, gsw_run_stats *stats) {
+  return guarded([&] {
+    GSW_REQUIRE(G, GSW_E_INVALID, "null graph");
+    GSW_REQUIRE(damping > 0 && damping < 1 && eps > 0 && max_iters >= 1, GSW_E_INVALID,
+                "Delta-PageRank needs 0 < damping < 1, eps > 0, max_iters >= 1");
+    const run_opts_t o = to_opts_rule(opts);
+    if (!G->dpr) {
+      G->dpr = std::make_unique<app_ctx_t<DeltaPR>>();
+      G->dpr->as.build(G->hg.n, Normal);
+      G->dpr->f.data.build(G->hg.n);
+    }
+    auto &c = *G->dpr;
+    arch_once(c);
+    prepare_engine(c, o, !opts || opts->profile);
+    c.f.damp = damping;
+    c.f.eps = eps;
+    feature_t &fets = c.en.fets;
+    fets.centric = VC;
+    fets.pattern = ASSO;
+    fets.fromall = false;
+    fets.toall = true;
+    fets.set_graph((double)G->hg.n, (double)(G->hg.m / 2), G->attr);
+    view_t<GU> gv{&G->hg, G->attr, 0.0};
+    init_conf(c.en.stats, fets, c.en.conf, gv, c.f);
+    // the filter applies the accumulated changes, so every level must be
+    // inspected: no fused levels (a fused level would skip the filter)
+    GSW_REQUIRE(!(c.en.conf.force.has_fusion && c.en.conf.force.fusion == Fused),
+                GSW_E_INVALID, "Delta-PageRank runs standalone levels only");
+    c.en.conf.conf_fuse_inspect = false;
+    c.en.conf.conf_fusion = false;
+
+    hipStream_t st = gsw::stream();
+    H_ERR(hipEventRecord(G->ev0, st));
+    c.as.reset();
+    c.as.init(ALL_ACTIVE);
+    hipLaunchKernelGGL(k_init_dpr, dim3(gsw::grid_for(G->hg.n)), dim3(gsw::BLOCK),
+                       0, st, c.f.data.dg_wa, c.f.data.dg_ra, G->dg.dg_row_ptr,
+                       G->hg.n, damping);
+    GU g = G->dg;
+    const int cap = max_iters;
+    const int levels = run_vc(c.en, c.as, g, c.f, o,
+                              [cap](int level) { return level >= cap; });
+    hipLaunchKernelGGL(k_dpr_output, dim3(gsw::grid_for(G->hg.n)), dim3(gsw::BLOCK),
+                       0, st, c.f.data.dg_ra, G->hg.n, G->d_out_f);
     H_ERR(hipEventRecord(G->ev1, st));
     const double ms = elapsed(G);
     collect_records(G, c.en.executor, c.as);

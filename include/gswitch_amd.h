@@ -170,6 +170,15 @@ gsw_status gsw_cc(gsw_graph *g, const gsw_run_opts *opts, int32_t *labels_out,
 gsw_status gsw_pagerank(gsw_graph *g, float damping, int32_t iters, float eps,
                         const gsw_run_opts *opts, float *pr_out,
                         gsw_run_stats *stats);
+/* Delta-PageRank, described in the reference's documentation
+ * (docs/index.md:175-179) but absent from its code: vertices send only the
+ * change of their value; an accumulation of incoming changes is applied
+ * (and sent on) once it exceeds eps. Converges to the fixed point of
+ * gsw_pagerank's iteration, value = (1 - d) + d * sum value[u] / deg[u];
+ * stops when no accumulation exceeds eps or after max_iters rounds. */
+gsw_status gsw_delta_pagerank(gsw_graph *g, float damping, double eps,
+                              int32_t max_iters, const gsw_run_opts *opts,
+                              float *pr_out, gsw_run_stats *stats);
 /* Graph colouring, run_gc (application/gc.cu:15-127): Jones-Plassmann rounds
  * with vertex ids as priorities; colors_out[v] = the round (from 1) in which
  * v's id first exceeded every uncoloured neighbour's. */

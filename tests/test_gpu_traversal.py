@@ -284,3 +284,24 @@ def test_gc(gsw, orc, gset, variant):
         h, g = gset[gname]
         assert np.array_equal(g.gc(configs=variant), orc.gc(h.row_ptr, h.col)), \
             (variant, gname)
+
+
+# ---------------------------------------------------------------- Delta-PageRank
+@pytest.mark.parametrize("variant", [None, "Push-Queue-WM-Standalone",
+                                     "Push-Bitmap-STRICT-Standalone",
+                                     "Pull-Bitmap-TM-Standalone",
+                                     "Pull-Queue-CM-Standalone"])
+def test_delta_pagerank(gsw, orc, gset, variant):
+    """Delta-PageRank reaches the PageRank fixed point: within 1e-5 max
+    relative error of the FP64 oracle iterated to convergence (300
+    iterations: 0.85^300 ~ 1e-21), in fewer rounds than that."""
+    for gname in ["rmat12", "ba", "grid", "er"]:
+        h, g = gset[gname]
+        pr = g.delta_pagerank(0.85, 1e-10, configs=variant)
+        ref = orc.pagerank(h.row_ptr, h.col, 0.85, 300)
+        assert max_rel(pr, ref) <= PR_TOL, (variant, gname, max_rel(pr, ref))
+        assert 0 < g.last_stats.levels < 300, (variant, gname, g.last_stats.levels)
+    with pytest.raises(gsw.GswError):
+        g.delta_pagerank(0.85, 0.0)
+    with pytest.raises(gsw.GswError):  # its filter applies the changes: never fused
+        g.delta_pagerank(0.85, 1e-10, configs="Push-Queue-WM-Fused")
