@@ -12,6 +12,8 @@
     -v, --verbose      per-level records
     -V, --validation   check the result's defining properties on the host
     -D, --device       accepted for compatibility; the library uses GPU 0
+    -j, --json         append the per-level training records (the reference's
+                       feature_t::record format) to this file
         --iters        PageRank iterations (default 50), --damping (0.85)
 
 The graph is read as the reference reads it (gsw_read_mtx: comments and the
@@ -148,6 +150,8 @@ def main(argv=None):
         g = gsw.Graph(row_ptr=h.row_ptr, col=h.col, w=w)
     else:
         g = gsw.Graph(h)
+    if a.json:
+        gsw.record_features(a.json)
     root = a.src if a.src >= 0 else h.max_degree_vertex()
     if a.app in ("bfs", "sssp", "bc") and not 0 <= root < h.n:
         raise SystemExit("root %d out of range [0, %d)" % (root, h.n))
@@ -187,6 +191,8 @@ def main(argv=None):
     if ok is not None:
         print("validation: %s" % ("PASS" if ok else "FAIL"))
     g.close()
+    if a.json:
+        gsw.record_features(None)
     return 0 if ok in (None, True) else 1
 
 
