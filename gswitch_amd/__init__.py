@@ -72,6 +72,8 @@ _SIGS = {
     "gsw_gen_ba": (C.c_int32, [C.c_int64, C.c_int32, C.c_uint64, C.c_int32,
                                C.POINTER(_P)]),
     "gsw_record_features": (C.c_int32, [C.c_char_p]),
+    "gsw_host_graph_save": (C.c_int32, [_P, C.c_char_p]),
+    "gsw_host_graph_load": (C.c_int32, [C.c_char_p, C.POINTER(_P)]),
     "gsw_read_mtx": (C.c_int32, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                  C.POINTER(_P)]),
     "gsw_gen_er_batch": (C.c_int32, [C.c_int32, C.c_int64, C.c_int64, C.c_double,
@@ -185,6 +187,10 @@ class HostGraph:
     def degrees(self):
         return np.diff(self.row_ptr)
 
+    def save(self, path):
+        """Write this CSR to a binary cache file (load it with load_graph)."""
+        _check(_lib.gsw_host_graph_save(self._h, os.fsencode(path)))
+
     def max_degree_vertex(self):
         """Highest-degree vertex, ties broken by the smallest id."""
         return int(np.argmax(self.degrees))
@@ -211,6 +217,13 @@ def erdos_renyi(n, mean_degree, seed=5, weighted=False):
 
 def barabasi_albert(n, m, seed=6, weighted=False):
     return _gen(_lib.gsw_gen_ba, n, m, seed, int(weighted))
+
+
+def load_graph(path):
+    """A HostGraph from a binary CSR cache file written by HostGraph.save."""
+    h = _P()
+    _check(_lib.gsw_host_graph_load(os.fsencode(path), C.byref(h)))
+    return HostGraph(h)
 
 
 def record_features(path):

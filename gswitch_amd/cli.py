@@ -14,6 +14,9 @@
     -D, --device       accepted for compatibility; the library uses GPU 0
     -j, --json         append the per-level training records (the reference's
                        feature_t::record format) to this file
+        --cache        keep the parsed CSR in .csr_<file name> next to the file
+                       and load it from there on later runs (the reference's
+                       store_to_binfile / load_from_binfile naming)
         --iters        PageRank iterations (default 50), --damping (0.85)
 
 The graph is read as the reference reads it (gsw_read_mtx: comments and the
@@ -35,6 +38,7 @@ necessary conditions, not a second computation.
 """
 import argparse
 import json
+import os
 import sys
 
 import numpy as np
@@ -133,13 +137,21 @@ def main(argv=None):
     ap.add_argument("-V", "--validation", action="store_true")
     ap.add_argument("-D", "--device", type=int, default=0)
     ap.add_argument("-j", "--json", default=None)
+    ap.add_argument("--cache", action="store_true")
     ap.add_argument("--iters", type=int, default=50)
     ap.add_argument("--damping", type=float, default=0.85)
     a = ap.parse_args(argv)
     if a.device != 0:
         print("note: the library runs on GPU 0", file=sys.stderr)
-    h = gsw.read_mtx(a.path, directed=a.directed, weighted=a.with_weight,
-                     header=a.with_header)
+    cache = os.path.join(os.path.dirname(os.path.abspath(a.path)),
+                         ".csr_" + os.path.basename(a.path))
+    if a.cache and os.path.exists(cache):
+        h = gsw.load_graph(cache)
+    else:
+        h = gsw.read_mtx(a.path, directed=a.directed, weighted=a.with_weight,
+                         header=a.with_header)
+        if a.cache:
+            h.save(cache)
     print("graph: n=%d m=%d (directed CSR entries)%s" % (
         h.n, h.m, " directed" if h.directed else ""))
     w = None

@@ -809,6 +809,27 @@ gsw_status gsw_gen_grid2d(int64_t rows, int64_t cols, double p_delete,
   });
 }
 
+gsw_status gsw_host_graph_save(const gsw_host_graph *hg, const char *path) {
+  return guarded([&] {
+    GSW_REQUIRE(hg && path, GSW_E_INVALID, "null argument");
+    gsw_host::save_csr(hg->g, path);
+  });
+}
+
+gsw_status gsw_host_graph_load(const char *path, gsw_host_graph **out) {
+  return guarded([&] {
+    GSW_REQUIRE(out && path, GSW_E_INVALID, "null argument");
+    auto *h = new gsw_host_graph();
+    try {
+      gsw_host::load_csr(path, h->g);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
 gsw_status gsw_record_features(const char *path) {
   return guarded([&] {
     record_sink_t &k = record_sink();

@@ -92,6 +92,10 @@ void gen_er_batch(int64_t first, int count, int64_t nmin, int64_t nmax,
 void read_mtx(const std::string &path, bool directed, bool with_weight,
               bool with_header, bool rinse, HostGraph &g);
 
+// Binary CSR cache: save / load a HostGraph (see mtx.cpp for the layout).
+void save_csr(const HostGraph &g, const std::string &path);
+void load_csr(const std::string &path, HostGraph &g);
+
 int num_threads();
 
 }  // namespace gsw_host

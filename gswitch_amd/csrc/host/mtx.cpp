@@ -138,3 +138,89 @@ void read_mtx(const std::string &path, bool directed, bool with_weight,
 }
 
 }  // namespace gsw_host
+
+namespace gsw_host {
+
+// Binary CSR cache (the reference's store_to_binfile / load_from_binfile,
+// src/utils/fileIO.cuh:334-387, which dump raw int arrays): here an 8-byte
+// tag "GSWCSR01", then int64 n, int64 m, int32 directed, int32 weighted,
+// then row_ptr int64[n+1], col int32[m], w uint32[m] (weighted), and for a
+// directed graph the reverse row_ptr / col / w the same way.
+namespace {
+const char kTag[8] = {'G', 'S', 'W', 'C', 'S', 'R', '0', '1'};
+
+struct file_t {
+  FILE *f;
+  explicit file_t(FILE *x) : f(x) {}
+  ~file_t() {
+    if (f) std::fclose(f);
+  }
+};
+template <typename T> void put(FILE *f, const T *p, size_t k) {
+  if (k && std::fwrite(p, sizeof(T), k, f) != k)
+    throw std::runtime_error("write failed");
+}
+template <typename T> void get(FILE *f, T *p, size_t k) {
+  if (k && std::fread(p, sizeof(T), k, f) != k)
+    throw std::invalid_argument("truncated CSR cache file");
+}
+}  // namespace
+
+void save_csr(const HostGraph &g, const std::string &path) {
+  file_t fp(std::fopen(path.c_str(), "wb"));
+  if (!fp.f) throw std::invalid_argument("can not create file " + path);
+  const int64_t head[2] = {g.n, g.m};
+  const int32_t flags[2] = {g.directed ? 1 : 0, g.weighted ? 1 : 0};
+  put(fp.f, kTag, 8);
+  put(fp.f, head, 2);
+  put(fp.f, flags, 2);
+  put(fp.f, g.row_ptr.data(), (size_t)g.n + 1);
+  put(fp.f, g.col.data(), (size_t)g.m);
+  if (g.weighted) put(fp.f, g.w.data(), (size_t)g.m);
+  if (g.directed) {
+    put(fp.f, g.r_row_ptr.data(), (size_t)g.n + 1);
+    put(fp.f, g.r_col.data(), (size_t)g.m);
+    if (g.weighted) put(fp.f, g.r_w.data(), (size_t)g.m);
+  }
+}
+
+void load_csr(const std::string &path, HostGraph &g) {
+  file_t fp(std::fopen(path.c_str(), "rb"));
+  if (!fp.f) throw std::invalid_argument("can not open file " + path);
+  char tag[8];
+  get(fp.f, tag, 8);
+  if (std::string(tag, 8) != std::string(kTag, 8))
+    throw std::invalid_argument("not a CSR cache file: " + path);
+  int64_t head[2];
+  int32_t flags[2];
+  get(fp.f, head, 2);
+  get(fp.f, flags, 2);
+  if (head[0] < 0 || head[0] > (int64_t)INT32_MAX || head[1] < 0)
+    throw std::invalid_argument("corrupt CSR cache header");
+  g = HostGraph();
+  g.n = head[0];
+  g.m = head[1];
+  g.directed = flags[0] != 0;
+  g.weighted = flags[1] != 0;
+  auto read_side = [&](std::vector<int64_t> &rp, std::vector<int32_t> &cl,
+                       std::vector<uint32_t> &w) {
+    rp.resize((size_t)g.n + 1);
+    cl.resize((size_t)g.m);
+    get(fp.f, rp.data(), rp.size());
+    get(fp.f, cl.data(), cl.size());
+    if (g.weighted) {
+      w.resize((size_t)g.m);
+      get(fp.f, w.data(), w.size());
+    }
+    if (rp[0] != 0 || rp[g.n] != g.m)
+      throw std::invalid_argument("corrupt CSR cache offsets");
+    for (int64_t v = 0; v < g.n; ++v)
+      if (rp[v + 1] < rp[v]) throw std::invalid_argument("corrupt CSR cache offsets");
+    for (int32_t u : cl)
+      if (u < 0 || u >= g.n) throw std::invalid_argument("corrupt CSR cache ids");
+  };
+  read_side(g.row_ptr, g.col, g.w);
+  if (g.directed) read_side(g.r_row_ptr, g.r_col, g.r_w);
+}
+
+}  // namespace gsw_host

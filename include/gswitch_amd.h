@@ -62,6 +62,13 @@ gsw_status gsw_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b,
 gsw_status gsw_gen_grid2d(int64_t rows, int64_t cols, double p_delete,
                           double long_range_frac, uint64_t seed,
                           int32_t weighted, gsw_host_graph **out);
+/* Binary CSR cache, the reference's store_to_binfile / load_from_binfile
+ * (src/utils/fileIO.cuh:334-387): save a host CSR (offsets, adjacency,
+ * weights, reverse CSR of a directed graph) and load it back without
+ * parsing the text file again. The layout is this library's own (tagged,
+ * int64 offsets); a truncated or corrupt file is an error. */
+gsw_status gsw_host_graph_save(const gsw_host_graph *hg, const char *path);
+gsw_status gsw_host_graph_load(const char *path, gsw_host_graph **out);
 /* Training records in the reference's format (feature_t::record,
  * src/abstraction/features.cuh:86-104, enabled there by --json=<path>):
  * from now on every expanded level of a vertex-centric run appends one JSON

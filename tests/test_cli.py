@@ -92,3 +92,17 @@ def test_check_gc(gsw, orc):
     bad = c.copy()
     bad[5] = -1  # uncoloured
     assert not cli.check_gc(h, bad)
+
+
+def test_cli_cache_file(gsw, tmp_path):
+    """--cache writes .csr_<name> beside the input before any GPU call."""
+    p = tmp_path / "g.txt"
+    p.write_text("1 2\n2 3\n3 1\n")
+    try:
+        cli.main(["cc", str(p), "--cache"])
+    except gsw.GswError:
+        pass  # no GPU in this process
+    c = tmp_path / ".csr_g.txt"
+    assert c.exists()
+    h = gsw.load_graph(str(c))
+    assert h.n == 3 and list(h.col) == [1, 2, 0, 2, 0, 1]

@@ -318,3 +318,37 @@ def test_record_file_parsing(gsw, tmp_path):
     assert [r["0"] for r in recs] == [5, 6] and recs[1]["22"] == 1e-05
     gsw.record_features(str(tmp_path / "x.json"))
     gsw.record_features(None)
+
+
+def test_csr_cache_round_trip(gsw, tmp_path):
+    """The binary CSR cache gives back every array: undirected, weighted,
+    and directed (with its reverse CSR)."""
+    cases = [gsw.rmat(10, 8, seed=2), gsw.rmat(10, 8, seed=2, weighted=True)]
+    p = tmp_path / "d.txt"
+    p.write_text("0 1\n1 2\n2 0\n2 3\n")
+    cases.append(gsw.read_mtx(str(p), directed=True))
+    for i, h in enumerate(cases):
+        f = str(tmp_path / ("c%d.bin" % i))
+        h.save(f)
+        b = gsw.load_graph(f)
+        assert (b.n, b.m, b.directed, b.weighted) == (h.n, h.m, h.directed, h.weighted)
+        assert np.array_equal(b.row_ptr, h.row_ptr) and np.array_equal(b.col, h.col)
+        if h.weighted:
+            assert np.array_equal(b.w, h.w)
+    d = gsw.load_graph(str(tmp_path / "c2.bin"))
+    assert d.directed and list(d.col) == [1, 2, 0, 3]
+
+
+def test_csr_cache_rejects_bad_files(gsw, tmp_path):
+    h = gsw.rmat(8, 4, seed=1)
+    f = tmp_path / "c.bin"
+    h.save(str(f))
+    raw = f.read_bytes()
+    (tmp_path / "short.bin").write_bytes(raw[:-5])
+    (tmp_path / "tag.bin").write_bytes(b"XXXXXXXX" + raw[8:])
+    bad = bytearray(raw)
+    bad[8 + 16 + 4 + 4 + 8] = 0xFF  # row_ptr[1] becomes huge
+    (tmp_path / "rows.bin").write_bytes(bytes(bad))
+    for name in ["short.bin", "tag.bin", "rows.bin", "missing.bin"]:
+        with pytest.raises(gsw.GswError):
+            gsw.load_graph(str(tmp_path / name))
