@@ -50,7 +50,12 @@ struct SSSP : Functor<VC, uint32_t, uint32_t, uint32_t> {
     }
     return Inactive;
   }
-  __device__ uint32_t emit(int v, uint32_t *w, GW g) { return *wa_of(v) + *w; }
+  // saturating: a path longer than 2^32 - 1 counts as no path (UINT32_MAX)
+  // instead of wrapping to a short distance
+  __device__ uint32_t emit(int v, uint32_t *w, GW g) {
+    const uint32_t d = *wa_of(v), s = d + *w;
+    return s < d ? UINT32_MAX : s;
+  }
   __device__ bool comp(uint32_t *d, uint32_t msg, GW g) {
     if (*d > msg) {
       *d = msg;

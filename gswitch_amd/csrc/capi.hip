@@ -247,6 +247,29 @@ struct gsw_graph {
 
 namespace {
 
+// Sort every adjacency row ascending (weights move with their columns):
+// the pulls that split rows by source range (TWOD slices) and the early-exit
+// first probe assume sorted rows. Rows that are already sorted cost one scan.
+void sort_rows(gsw_host::HostGraph &hg) {
+  std::vector<std::pair<int32_t, uint32_t>> tmp;
+  for (int64_t v = 0; v < hg.n; ++v) {
+    const int64_t s = hg.row_ptr[v], e = hg.row_ptr[v + 1];
+    auto *c = hg.col.data();
+    if (std::is_sorted(c + s, c + e)) continue;
+    if (hg.w.empty()) {
+      std::sort(c + s, c + e);
+      continue;
+    }
+    tmp.clear();
+    for (int64_t j = s; j < e; ++j) tmp.emplace_back(c[j], hg.w[j]);
+    std::sort(tmp.begin(), tmp.end());
+    for (int64_t j = s; j < e; ++j) {
+      c[j] = tmp[j - s].first;
+      hg.w[j] = tmp[j - s].second;
+    }
+  }
+}
+
 void upload_graph(gsw_graph *G) {
   G->attr.fill(G->hg.row_ptr.data(), G->hg.n);
   G->dg.build(G->hg);
@@ -969,6 +992,7 @@ gsw_status gsw_graph_create_csr(const int64_t *row_ptr, const int32_t *col,
     G->hg.row_ptr.assign(row_ptr, row_ptr + n + 1);
     G->hg.col.assign(col, col + m);
     if (w) G->hg.w.assign(w, w + m);
+    sort_rows(G->hg);
     try {
       upload_graph(G.get());
     } catch (...) {
@@ -1453,6 +1477,10 @@ gsw_status gsw_gc(gsw_graph *G, const gsw_run_opts *opts, int32_t *colors_out,
                   gsw_run_stats *stats) {
   return guarded([&] {
     GSW_REQUIRE(G, GSW_E_INVALID, "null graph");
+    // Jones-Plassmann over one edge direction would let both ends of a
+    // one-way edge take the same colour
+    GSW_REQUIRE(!G->hg.directed, GSW_E_UNSUPPORTED,
+                "graph colouring needs an undirected graph");
     const run_opts_t o = to_opts_rule(opts);
     if (!G->gc) {
       G->gc = std::make_unique<app_ctx_t<Color>>();
@@ -1543,9 +1571,12 @@ gsw_status gsw_msbfs64(gsw_graph *G, const int32_t *sources, int32_t nsrc,
       fill_async<unsigned long long>((unsigned long long *)G->ms_cnt, 0ull,
                                      sizeof(ms_counters_t) / 8 * MS_SLOTS);
       if (pull) {
-        hipLaunchKernelGGL(k_msbfs_pull, dim3(grid), dim3(BLOCK), 0, st, rows,
-                           cols, n, all, visit, (const uint64_t *)front, next,
-                           G->ms_cnt);
+        // bottom-up over the in-edges (the reverse CSR; the same arrays
+        // for an undirected graph)
+        hipLaunchKernelGGL(k_msbfs_pull, dim3(grid), dim3(BLOCK), 0, st,
+                           (const int64_t *)G->dg.dgr_row_ptr,
+                           (const int32_t *)G->dg.dgr_adj_list, rows, n, all,
+                           visit, (const uint64_t *)front, next, G->ms_cnt);
       } else {
         fill_async<uint64_t>(next, 0, n + 1);
         hipLaunchKernelGGL(k_msbfs_push, dim3(grid), dim3(BLOCK), 0, st, rows,

@@ -320,9 +320,15 @@ struct active_set_t {
       queue.host_sizes[queue.cursor] = 0;
       queue.cursor ^= 1;
     } else if (queue.traceback) {
+      GSW_REQUIRE(queue.cursor > 0, gsw::E_INTERNAL_CHECK,
+                  "history queue traced back past its first level");
       queue.cursor--;
       queue.offset_in -= queue.host_sizes[queue.cursor];
     } else {
+      // the next expand writes the size slot after the new cursor; both
+      // size arrays hold max_levels + 2 slots
+      GSW_REQUIRE(queue.cursor < queue.max_levels, gsw::E_OVERFLOW,
+                  "history queue holds no more levels");
       queue.cursor++;
       queue.offset_in = queue.offset_out;
       queue.offset_out += queue.host_sizes[queue.cursor];

@@ -76,10 +76,14 @@ __device__ __forceinline__ void ms_flush(ms_counters_t *c,
   }
 }
 
-// Bottom-up level. Writes next[v] = newly reached bits for every v.
+// Bottom-up level over the in-edges (rows/cols: the reverse CSR of a
+// directed graph). Writes next[v] = newly reached bits for every v and
+// counts the out-degrees (out_rows) of the newly reached vertices, the
+// next level's push work.
 __global__ void __launch_bounds__(BLOCK)
     k_msbfs_pull(const int64_t *__restrict__ rows,
-                 const int32_t *__restrict__ cols, int64_t n, uint64_t all,
+                 const int32_t *__restrict__ cols,
+                 const int64_t *__restrict__ out_rows, int64_t n, uint64_t all,
                  uint64_t *__restrict__ visit,
                  const uint64_t *__restrict__ front,
                  uint64_t *__restrict__ next, ms_counters_t *c) {
@@ -130,7 +134,7 @@ __global__ void __launch_bounds__(BLOCK)
       if (nw) {
         visit[v] = vis | nw;
         ++newly;
-        ndeg += (uint64_t)(rows[v + 1] - rows[v]);
+        ndeg += (uint64_t)(out_rows[v + 1] - out_rows[v]);
       }
     }
     ms_count_sources(nw, cnt);

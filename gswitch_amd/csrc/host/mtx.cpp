@@ -202,6 +202,20 @@ void load_csr(const std::string &path, HostGraph &g) {
   g.m = head[1];
   g.directed = flags[0] != 0;
   g.weighted = flags[1] != 0;
+  {
+    // the header's sizes must match the file before anything is allocated
+    const long here = std::ftell(fp.f);
+    if (here < 0 || std::fseek(fp.f, 0, SEEK_END) != 0)
+      throw std::invalid_argument("can not size CSR cache file " + path);
+    const long end = std::ftell(fp.f);
+    if (end < 0 || std::fseek(fp.f, here, SEEK_SET) != 0)
+      throw std::invalid_argument("can not size CSR cache file " + path);
+    const double side = 8.0 * ((double)g.n + 1) +
+                        (g.weighted ? 8.0 : 4.0) * (double)g.m;
+    const double want = (double)here + side * (g.directed ? 2 : 1);
+    if ((double)end != want)
+      throw std::invalid_argument("CSR cache file size does not match its header");
+  }
   auto read_side = [&](std::vector<int64_t> &rp, std::vector<int32_t> &cl,
                        std::vector<uint32_t> &w) {
     rp.resize((size_t)g.n + 1);
@@ -218,6 +232,11 @@ void load_csr(const std::string &path, HostGraph &g) {
       if (rp[v + 1] < rp[v]) throw std::invalid_argument("corrupt CSR cache offsets");
     for (int32_t u : cl)
       if (u < 0 || u >= g.n) throw std::invalid_argument("corrupt CSR cache ids");
+    // the writer stores sorted rows; kernels that split rows by id range
+    // (the TWOD slices) rely on it
+    for (int64_t v = 0; v < g.n; ++v)
+      if (!std::is_sorted(cl.begin() + rp[v], cl.begin() + rp[v + 1]))
+        throw std::invalid_argument("CSR cache rows are not sorted");
   };
   read_side(g.row_ptr, g.col, g.w);
   if (g.directed) read_side(g.r_row_ptr, g.r_col, g.r_w);

@@ -34,6 +34,14 @@ void orc_bfs(int64_t n, const int64_t *row_ptr, const int32_t *col,
 /* ---- SPFA: application/sssp.cu:55-89 -------------------------------------
  * Ring-buffer queue; in_q[] keeps a vertex at most once in the queue, so a
  * capacity of n suffices. */
+/* Path lengths saturate at UINT32_MAX (the unreached sentinel): a path
+ * longer than 2^32 - 1 is treated as no path, where a wrapping add would
+ * hand out a short distance. */
+static uint32_t sat_add(uint32_t a, uint32_t b) {
+  uint32_t s = a + b;
+  return s < a ? UINT32_MAX : s;
+}
+
 void orc_spfa(int64_t n, const int64_t *row_ptr, const int32_t *col,
               const uint32_t *w, int32_t root, uint32_t *dist) {
   for (int64_t i = 0; i < n; ++i) dist[i] = UINT32_MAX;
@@ -53,7 +61,7 @@ void orc_spfa(int64_t n, const int64_t *row_ptr, const int32_t *col,
     uint32_t dv = dist[v];
     for (int64_t j = row_ptr[v]; j < row_ptr[v + 1]; ++j) {
       int32_t u = col[j];
-      uint32_t nd = dv + w[j];
+      uint32_t nd = sat_add(dv, w[j]);
       if (dist[u] > nd) {
         dist[u] = nd;
         if (!in_q[u]) {
@@ -124,7 +132,7 @@ void orc_dijkstra(int64_t n, const int64_t *row_ptr, const int32_t *col,
     done[v] = 1;
     for (int64_t j = row_ptr[v]; j < row_ptr[v + 1]; ++j) {
       int32_t u = col[j];
-      uint32_t nd = top.d + w[j];
+      uint32_t nd = sat_add(top.d, w[j]);
       if (nd < dist[u]) {
         dist[u] = nd;
         heap_item ni = {nd, u};
@@ -146,7 +154,7 @@ void orc_bellman_ford(int64_t n, const int64_t *row_ptr, const int32_t *col,
     for (int64_t v = 0; v < n; ++v) {
       if (dist[v] == UINT32_MAX) continue;
       for (int64_t j = row_ptr[v]; j < row_ptr[v + 1]; ++j) {
-        uint32_t nd = dist[v] + w[j];
+        uint32_t nd = sat_add(dist[v], w[j]);
         if (nd < dist[col[j]]) {
           dist[col[j]] = nd;
           changed = 1;

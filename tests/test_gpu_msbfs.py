@@ -67,3 +67,31 @@ def test_msbfs_c4a_shape_rmat16(gsw, orc):
     em, es = oracle_masks_and_sums(orc, h, src)
     assert np.array_equal(masks, em)
     assert np.array_equal(sums, es)
+
+
+@pytest.mark.parametrize("direction", [None, "Push-x-x-x", "Pull-x-x-x"])
+def test_msbfs_directed(gsw, orc, direction):
+    """One-way edges: bit s of v is set iff v is reachable from source s along
+    the edges' directions, whether the levels run top-down (out-edges) or
+    bottom-up (in-edges). Reachability against the edges differs on this
+    graph, so a pull over the wrong adjacency fails."""
+    rng = np.random.default_rng(11)
+    n = 3000
+    h = gsw.from_edges(n, rng.integers(0, n, 6000), rng.integers(0, n, 6000),
+                       directed=True)
+    g = gsw.Graph(h)
+    src = top_degree(h, 64)
+    masks, sums = g.msbfs64(src, configs=direction)
+    em, es = oracle_masks_and_sums(orc, h, src)
+    assert np.array_equal(masks, em)
+    assert np.array_equal(sums, es)
+    # reachability against the edge directions, from the transposed CSR
+    e_src = np.repeat(np.arange(n), np.diff(h.row_ptr))
+    order = np.lexsort((e_src, h.col))
+    rrp = np.concatenate([[0], np.cumsum(np.bincount(h.col, minlength=n))])
+    rcol = e_src[order].astype(np.int32)
+    rev = np.zeros(n, np.uint64)
+    for s, r in enumerate(src):
+        rev[orc.bfs(rrp, rcol, int(r)) >= 0] |= np.uint64(1) << np.uint64(s)
+    assert not np.array_equal(rev, em)
+    g.close()

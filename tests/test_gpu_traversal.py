@@ -117,6 +117,21 @@ def test_sssp_default(gsw, orc, wset, gname):
         assert np.array_equal(g.sssp(root), orc.spfa(h.row_ptr, h.col, h.w, root))
 
 
+def test_sssp_saturates_long_paths(gsw, orc):
+    """uint32 path lengths saturate to the unreached sentinel instead of
+    wrapping (a wrapped 0-1-2 path would cost 0 and undercut 0-3-2)."""
+    big = 2**31
+    rp, col, w = csr_from_edges(5, [(0, 1), (1, 2), (2, 3), (0, 3), (3, 4)],
+                                [big, big, 5, 100, 2**32 - 1 - 100])
+    g = gsw.Graph(row_ptr=rp, col=col, w=w)
+    ref = orc.spfa(rp, col, w, 0)
+    assert list(ref) == [0, big, 105, 100, 2**32 - 1]
+    for v in [None, "Push-Queue-WM-Standalone", "Pull-Bitmap-TM-Standalone",
+              "Push-Queue-STRICT-Fused", "Pull-Bitmap-STRICT-Standalone"]:
+        assert np.array_equal(g.sssp(0, configs=v), ref), v
+    g.close()
+
+
 @pytest.mark.parametrize("variant", ALL_VARIANTS)
 def test_sssp_forced_variant(gsw, orc, wset, variant):
     h, g = wset["rmat12w"]
@@ -172,6 +187,31 @@ def test_pagerank_twod_slices(gsw, orc, monkeypatch, slice_kb):
                         pull_kernel=pull_kernel)
         g.close()
         assert max_rel(pr, ref) <= PR_TOL, (slice_kb, pull_kernel, max_rel(pr, ref))
+
+
+def test_unsorted_rows_are_sorted_on_upload(gsw, orc, monkeypatch):
+    """Rows handed over in any order: the TWOD slices split rows by source
+    range (1 KB slices: 16 per iteration on R-MAT 12) and the bottom-up pull
+    takes each row's first entry, so both see the rows sorted on upload."""
+    monkeypatch.setenv("GSW_TWOD_SLICE_KB", "1")
+    h = gsw.rmat(12, 16, seed=1, weighted=True)
+    rng = np.random.default_rng(3)
+    perm = np.arange(h.m)
+    for v in range(h.n):
+        lo, hi = h.row_ptr[v], h.row_ptr[v + 1]
+        perm[lo:hi] = lo + rng.permutation(hi - lo)
+    col, w = h.col[perm], h.w[perm]
+    assert not all(np.all(np.diff(col[h.row_ptr[v]:h.row_ptr[v + 1]]) > 0)
+                   for v in range(h.n))
+    g = gsw.Graph(row_ptr=h.row_ptr, col=col, w=w)
+    pr = g.pagerank(0.85, 50, configs="Pull-Bitmap-TWOD-Standalone")
+    ref = orc.pagerank(h.row_ptr, h.col, 0.85, 50)
+    assert max_rel(pr, ref) <= PR_TOL
+    root = h.max_degree_vertex()
+    assert np.array_equal(g.bfs(root, configs="Pull-Bitmap-TM-Standalone"),
+                          orc.bfs(h.row_ptr, h.col, root))
+    assert np.array_equal(g.sssp(root), orc.spfa(h.row_ptr, h.col, h.w, root))
+    g.close()
 
 
 def test_pagerank_iteration_counts(gsw, orc, gset):
@@ -271,6 +311,14 @@ def test_training_records(gsw, gset, tmp_path):
 
 
 # ---------------------------------------------------------------- colouring
+def test_gc_rejects_directed(gsw):
+    h = gsw.from_edges(4, [0, 1, 2], [1, 2, 3], directed=True)
+    g = gsw.Graph(h)
+    with pytest.raises(gsw.GswError):
+        g.gc()
+    g.close()
+
+
 @pytest.mark.parametrize("variant", [None, "Pull-Queue-TM-Standalone",
                                      "Pull-Bitmap-TWOD-Standalone",
                                      "Pull-Bitmap-STRICT-Standalone",

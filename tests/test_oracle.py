@@ -119,6 +119,23 @@ def test_sssp_unreached_sentinel(orc):
     assert list(orc.spfa(rp, col, w, 0)) == [0, 5, 2**32 - 1, 2**32 - 1]
 
 
+def test_sssp_saturates_long_paths(orc):
+    """Path lengths past 2^32 - 1 saturate to the unreached sentinel rather
+    than wrapping: 0-1-2 costs 2^31 + 2^31 = 2^32 (wraps to 0), 2-3 adds 5
+    (wraps to 5, shorter than the direct 0-3 edge of 100 only if it wraps).
+    scipy's float64 Dijkstra gives the exact lengths to compare with."""
+    big = 2**31
+    rp, col, w = csr_from_edges(5, [(0, 1), (1, 2), (2, 3), (0, 3), (3, 4)],
+                                [big, big, 5, 100, 2**32 - 1 - 100])
+    exact = cg.dijkstra(to_scipy(rp, col, w), indices=0)
+    mx = 2**32 - 1
+    ref = np.where(exact >= mx, mx, exact).astype(np.uint64)
+    for fn in (orc.spfa, orc.dijkstra, orc.bellman_ford):
+        d = fn(rp, col, w, 0)
+        assert list(d) == list(ref), fn
+        assert list(d) == [0, big, 100 + 5, 100, mx], fn
+
+
 # ---------------------------------------------------------------- CC
 @pytest.mark.parametrize("seed", [6, 7])
 def test_cc_min_label_three_ways(orc, seed):
