@@ -495,6 +495,10 @@ int bfs_captured_run(gsw_graph *G, int64_t k, LB lb, int small, int large,
   GSW_REQUIRE(per_launch >= 2 && per_launch % 2 == 0 && grid >= 1 &&
                   grid <= gsw::MAX_GRID && small >= 1 && large > small,
               GSW_E_INVALID, "invalid captured-loop shape");
+  // k_push_wm scans a wavefront's 64 degrees in int32
+  GSW_REQUIRE(lb != WM || G->attr.max_deg * gsw::WAVE <= (double)INT32_MAX,
+              GSW_E_UNSUPPORTED,
+              "degrees too large for the captured WM expand; use TWC");
   auto &c = bfs_ctx(G);
   active_set_t &as = c.as;
   GSW_REQUIRE(as.queue.mode == Normal && as.queue.buf[1], GSW_E_INTERNAL,
@@ -1301,6 +1305,9 @@ gsw_status gsw_cc(gsw_graph *G, const gsw_run_opts *opts, int32_t *labels_out,
       G->coo.build(G->hg);
       G->has_coo = true;
     }
+    GSW_REQUIRE(G->coo.nedges <= INT32_MAX, GSW_E_UNSUPPORTED,
+                "the edge-centric frontier holds int32 edge ids: CC takes at "
+                "most 2^31 - 1 undirected edges");
     if (!G->cc) {
       G->cc = std::make_unique<app_ctx_t<CC>>();
       G->cc->as.build(MAX((int64_t)1, G->coo.nedges), Normal);

@@ -197,6 +197,13 @@ struct executor_t {
   void expand(active_set_t &as, G &g, F &f, stat_t &stat, feature_t &fets,
               config_t &conf) {
     const double s = gsw_now_ms();
+    // WM and CM scan the degrees of a wavefront's (64) or a workgroup's
+    // (256) frontier vertices in int32; where the graph's degrees could
+    // overflow that sum the level runs STRICT (int64 prefixes), forced
+    // choices included
+    if ((conf.conf_lb == WM || conf.conf_lb == CM) &&
+        fets.max_deg * gsw::BLOCK > (double)INT32_MAX)
+      conf.conf_lb = STRICT;
     level_record_t rec{};
     rec.level = g.level;
     rec.dir = conf.conf_dir;

@@ -296,3 +296,20 @@ def test_gc_matches_recurrence_and_is_proper(orc, seed):
     assert np.array_equal(c, gc_recurrence(n, rp, col))
     u = np.repeat(np.arange(n), np.diff(rp))
     assert np.all(c >= 1) and np.all(c[u] != c[col])
+
+
+def test_complete_bipartite_closed_forms(orc):
+    """Closed forms used at full size past 2^31 entries (K_{a,a} with
+    a = 33000, tests/test_gpu_large.py), pinned here on small K_{a,a} built
+    by the same helper."""
+    from test_gpu_large import complete_bipartite, expected_levels
+    for a in (1, 2, 7, 50):
+        rp, col = complete_bipartite(a)
+        ref_rp, ref_col = csr_from_edges(2 * a, [(i, a + j) for i in range(a)
+                                                 for j in range(a)])
+        assert np.array_equal(rp, ref_rp) and np.array_equal(col, ref_col)
+        for root in (0, a, 2 * a - 1):
+            assert np.array_equal(orc.bfs(rp, col, root), expected_levels(a, root))
+        assert not orc.cc(rp, col).any()
+        pr = orc.pagerank(rp, col, 0.85, 10)
+        assert np.allclose(pr, 1 - 0.85 ** 11, rtol=1e-12)
