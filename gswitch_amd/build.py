@@ -1,6 +1,6 @@
 """Build libgswitch_amd.so (HIP, gfx950) and the CPU oracle.
 
-    python gswitch_amd/build.py            # library + oracle
+    python gswitch_amd/build.py            # library, applications, oracle
     python gswitch_amd/build.py --lib      # library only
 
 The library is compiled in-tree with hipcc for --offload-arch=gfx950 so the
@@ -62,6 +62,30 @@ def build_lib(jobs=None):
     return LIB
 
 
+APPS_DIR = os.path.join(REPO, "tests", "apps")
+APPS = ["bfs_app", "sssp_app", "cc_app"]
+
+
+def build_apps():
+    """Applications written against the template API (gswitch.h) the way a
+    GSWITCH application is written; they link the library for file input
+    and find it through their rpath."""
+    out_dir = os.path.join(APPS_DIR, "build")
+    os.makedirs(out_dir, exist_ok=True)
+    procs = []
+    for app in APPS:
+        cmd = [HIPCC, "-x", "hip", "--offload-arch=" + ARCH, "-O3", "-std=c++17",
+               "-Wno-unused-value", "-Wno-unused-result",
+               os.path.join(APPS_DIR, app + ".hip"), "-o", os.path.join(out_dir, app),
+               "-I", os.path.join(CSRC, "gswitch"), "-I", os.path.join(REPO, "include"),
+               "-L", HERE, "-lgswitch_amd", "-Wl,-rpath,$ORIGIN/../../../gswitch_amd"
+               ] + DEVICE_FLAGS
+        print("+", " ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+    if any(p.wait() for p in procs):
+        raise RuntimeError("application build failed")
+
+
 def build_oracle():
     _run(["make", "-s", "-C", os.path.join(REPO, "oracle"), "all"])
     if os.path.isdir("/root/reference/src/model"):
@@ -71,6 +95,8 @@ def build_oracle():
 def main(argv):
     if "--oracle" not in argv:
         build_lib()
+        if "--lib" not in argv:
+            build_apps()
     if "--lib" not in argv:
         build_oracle()
 

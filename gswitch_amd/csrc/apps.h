@@ -267,7 +267,6 @@ struct engine_t {
   feature_t fets;
   config_t conf;
   stat_t stats;
-  engine_t() { executor.bind(inspector); }
   void release() {
     stats.release();
     executor.release();
@@ -290,7 +289,7 @@ struct run_opts_t {
 inline void enable_learned(engine_t &en, const float *w_dev, const float *w_host,
                            const graph_attr_t &attr, double n, double m,
                            bool fromall, bool toall) {
-  gsw_mlp::sel_dev_t &d = en.inspector.mlp;
+  gsw_mlp::sel_dev_t &d = en.stats.mlp;
   d.w = w_dev;
   d.n = (float)n;
   d.m = (float)m;
@@ -298,7 +297,7 @@ inline void enable_learned(engine_t &en, const float *w_dev, const float *w_host
   d.std_deg = (float)attr.std_deg;
   d.gini = (float)attr.GI;
   d.mask = gsw_mlp::valid_mask(fromall, toall);
-  en.inspector.mlp_host_w = w_host;
+  en.stats.mlp_host_w = w_host;
   en.conf.policy_mlp = true;
   en.conf.mlp_class = -1;
   en.conf.conf_fuse_inspect = false;

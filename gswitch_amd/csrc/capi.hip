@@ -374,6 +374,9 @@ template <typename Gv> struct view_t {
   const gsw_host::HostGraph *hg;
   graph_attr_t attr;
   double mean_weight;
+  double root_degree(int64_t v) const {
+    return (double)(hg->row_ptr[v + 1] - hg->row_ptr[v]);
+  }
 };
 
 double elapsed(gsw_graph *G) {
@@ -667,7 +670,7 @@ double sssp_probe_level(gsw_graph *G, app_ctx_t<SSSP> &c, int root, int cls,
   if (!fused) {
     en.inspector.inspect(c.as, g, c.f, en.stats, en.fets, en.conf);
   } else {
-    const double nq = (double)en.inspector.known_active;
+    const double nq = (double)c.as.known_active;
     const double fr = (double)en.executor.log.back().frontier;
     if (nq > 0 && nq * gsw_mlp::INSPECT_FRACTION >= (double)c.as.size &&
         nq >= gsw_mlp::INSPECT_GROWTH * fr) {
@@ -730,7 +733,7 @@ double bfs_probe_level(gsw_graph *G, app_ctx_t<BFS> &c, int root, int cls,
   } else {
     // a next frontier that the learned engine's gate hands to the inspector
     // costs that inspection too (the level after a fused choice pays it)
-    const double nq = (double)en.inspector.known_active;
+    const double nq = (double)c.as.known_active;
     const double fr = (double)en.executor.log.back().frontier;
     if (nq > 0 && nq * gsw_mlp::INSPECT_FRACTION >= (double)c.as.size &&
         nq >= gsw_mlp::INSPECT_GROWTH * fr) {

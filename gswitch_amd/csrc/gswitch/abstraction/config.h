@@ -77,6 +77,8 @@ struct config_t {
   // level-synchronous functor): the inspector then only reads the counters
   bool conf_pre_inspected = false;
   ins_t force;
+  // one line per expand on stdout (the reference's --verbose)
+  bool verbose = false;
 
   void reset() {
     conf_first_round = true;

@@ -44,9 +44,6 @@ struct executor_t {
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, int>> ev_pairs;  // per log entry
   int ev_next = 0;
-  inspector_t *insp = nullptr;  // for the raw frontier counts
-
-  void bind(inspector_t &i) { insp = &i; }
 
   void clear_log() {
     log.clear();
@@ -166,9 +163,9 @@ struct executor_t {
     int32_t *out = (q.mode == Normal) ? q.buf[q.cursor ^ 1] : q.buf[0] + q.offset_out;
     int32_t *out_size = q.sizes + out_slot;
     const long long expect =
-        (target == Active) ? insp->known_active
-        : (target == Inactive) ? insp->known_inactive
-                               : insp->known_active + insp->known_inactive;
+        (target == Active) ? as.known_active
+        : (target == Inactive) ? as.known_inactive
+                               : as.known_active + as.known_inactive;
     GSW_REQUIRE(expect <= (q.mode == Normal ? q.capacity : q.capacity - q.offset_out),
                 gsw::E_OVERFLOW, "frontier does not fit the queue");
     gsw::set_one_async<int32_t>(out_size, 0);
@@ -212,8 +209,8 @@ struct executor_t {
     rec.fused = conf.conf_fuse_inspect;
     rec.sorted = conf.conf_sorted;
     rec.frontier = as.get_size_host();
-    rec.active = insp ? insp->known_active : 0;
-    rec.inactive = insp ? insp->known_inactive : 0;
+    rec.active = as.known_active;
+    rec.inactive = as.known_inactive;
     as.select_work_row((int)log.size());
     {
       // WM push: about 128 edges per wavefront at the frontier's mean degree
@@ -242,15 +239,20 @@ struct executor_t {
       H_ERR(hipEventRecord(next_event(), gsw::stream()));
     }
     log.push_back(rec);
+    if (conf.verbose) {
+      std::printf("level %d frontier %lld ", rec.level, rec.frontier);
+      conf.show_hints(std::cout);
+      std::cout << std::endl;
+    }
     ev_pairs.push_back({e0, e1});
 
     if (conf.conf_fuse_inspect && conf.conf_centric == VC) {
       // the next queue's length: grid size for the next level, convergence
       const int out_slot = (as.queue.mode == Normal) ? (as.queue.cursor ^ 1)
                                                      : (as.queue.cursor + 1);
-      const bool learned = conf.policy_mlp && insp && insp->mlp.w;
+      const bool learned = conf.policy_mlp && stat.mlp.w;
       int nq = as.read_qsize_device(out_slot);
-      if (insp) insp->known_active = nq;
+      as.known_active = nq;
       as.check_overflow_cached();
       if (nq == 0) as.halt_host();
       // Learned selector on fused levels: a small next frontier stays fused
@@ -389,8 +391,8 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf,
             if (as.fmt == Bitmap && !conf.conf_fromall && !conf.conf_toall) {
               // this level's pull also inspects the next one
               gsw_mlp::sel_dev_t d;
-              if (conf.policy_mlp && insp && insp->mlp.w) {
-                d = insp->mlp;
+              if (conf.policy_mlp && stat.mlp.w) {
+                d = stat.mlp;
                 d.level = (float)g.level;
               }
               hipLaunchKernelGGL(k_pull_tm_inspect<Bd>,

@@ -20,16 +20,11 @@ inline double gsw_now_ms() {
 }
 
 struct inspector_t {
-  // raw counts of the current frontier, used to size queues without a sync
-  long long known_active = 0, known_inactive = 0;
-  // learned selector (mlp.w == nullptr: rule trees only); the host keeps a
-  // copy of the weights for levels decided without an inspector pass
-  gsw_mlp::sel_dev_t mlp;
-  const float *mlp_host_w = nullptr;
-
   // Selector inference for a level whose counters are known on the host
   // (the first round): same features and argmin as the device path.
-  void mlp_host_select(const feature_t &fets, config_t &conf, int level) {
+  void mlp_host_select(const stat_t &stat, const feature_t &fets,
+                       config_t &conf, int level) {
+    const gsw_mlp::sel_dev_t &mlp = stat.mlp;
     gsw_mlp::counts_t c;
     c.n = mlp.n;
     c.m = mlp.m;
@@ -44,7 +39,7 @@ struct inspector_t {
     c.max_deg_active = (float)fets.cur_max_deg_active;
     float x[gsw_mlp::NIN], y[gsw_mlp::NOUT];
     gsw_mlp::features(c, x);
-    gsw_mlp::forward_host(mlp_host_w, x, y);
+    gsw_mlp::forward_host(stat.mlp_host_w, x, y);
     conf.mlp_class = gsw_mlp::argmin_valid(y, mlp.mask);
   }
 
@@ -67,27 +62,27 @@ struct inspector_t {
     if (conf.conf_first_round) {
       // the initial frontier is known exactly (root or everything), also
       // when the first level is fused and features are not synthesised
-      known_active = fets.use_root < 0 ? g.nvertexs : (long long)fets.first_active;
-      known_inactive = g.nvertexs - known_active;
+      as.known_active = fets.use_root < 0 ? g.nvertexs : (long long)fets.first_active;
+      as.known_inactive = g.nvertexs - as.known_active;
     }
     if (need && conf.conf_first_round) {
       set_first_fets(as, fets, conf, (double)g.nvertexs, (double)g.nedges);
-      if (conf.policy_mlp && mlp.w) mlp_host_select(fets, conf, g.level);
+      if (conf.policy_mlp && stat.mlp.w) mlp_host_select(stat, fets, conf, g.level);
       need = false;
     }
     if (need && pre) {
       // the previous expand already classified this level (bitmaps written,
       // counters and, under the learned selector, the class in stat)
       stat.collect();
-      if (conf.policy_mlp && mlp.w) conf.mlp_class = (int)stat.h_runtime_info[6];
+      if (conf.policy_mlp && stat.mlp.w) conf.mlp_class = (int)stat.h_runtime_info[6];
       set_fets(as, stat, fets, conf, (double)g.nvertexs, (double)g.nedges);
       need = false;
     }
     if (need) {
       // the partial slots are zero between passes: k_stat_finish clears them
       gsw_mlp::sel_dev_t d;
-      if (conf.policy_mlp && mlp.w) {
-        d = mlp;
+      if (conf.policy_mlp && stat.mlp.w) {
+        d = stat.mlp;
         d.level = (float)g.level;
       }
       hipLaunchKernelGGL((gsw::k_inspect_vc<device_graph_t<CSR, E>, F>),
@@ -113,8 +108,8 @@ struct inspector_t {
       conf.conf_first_round = false;
       fets.active_vertex = (double)g.nedges;
       fets.active_vertex_ratio = 1;
-      known_active = g.nedges;
-      known_inactive = 0;
+      as.known_active = g.nedges;
+      as.known_inactive = 0;
       return;
     }
     hipLaunchKernelGGL((gsw::k_inspect_ec<device_graph_t<COO, E>, F>),
@@ -157,8 +152,8 @@ struct inspector_t {
       fets.push_workload = fets.first_workload;
       fets.pull_workload = m - fets.first_workload;
     }
-    known_active = (long long)fets.active_vertex;
-    known_inactive = (long long)fets.unactive_vertex;
+    as.known_active = (long long)fets.active_vertex;
+    as.known_inactive = (long long)fets.unactive_vertex;
     fets.active_vertex_ratio = DIV(fets.active_vertex, n);
     fets.unactive_vertex_ratio = DIV(fets.unactive_vertex, n);
     fets.push_workload_ratio = DIV(fets.push_workload, m);
@@ -171,8 +166,8 @@ struct inspector_t {
 
   void set_fets(active_set_t &as, stat_t &stat, feature_t &fets,
                 const config_t &conf, double n, double m) {
-    known_active = stat.active_num();
-    known_inactive = stat.unactive_num();
+    as.known_active = stat.active_num();
+    as.known_inactive = stat.unactive_num();
     fets.active_vertex = (double)stat.active_num();
     fets.unactive_vertex = (double)stat.unactive_num();
     fets.active_vertex_ratio = DIV(fets.active_vertex, n);

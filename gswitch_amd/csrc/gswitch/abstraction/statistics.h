@@ -15,6 +15,7 @@
 // one-workgroup finish kernel (k_stat_finish) sums the slots into the
 // reduced counters and clears them, so the slots are zero between passes.
 #pragma once
+#include "../model/mlp.h"
 #include "../utils/runtime.h"
 
 constexpr int STAT_SLOTS = 64;   // partial-counter slots
@@ -32,6 +33,12 @@ struct stat_t {
   double avg_filter_time = 0, avg_expand_time = 0;
   int level = 0;
 
+  // learned selector (mlp.w == nullptr: rule trees only): the network the
+  // counting passes run after their reduction, and a host copy of its
+  // weights for levels decided without a counting pass
+  gsw_mlp::sel_dev_t mlp;
+  const float *mlp_host_w = nullptr;
+
   void build(int _size = 6) {
     size = _size;
     if (!dg_runtime_info) {
@@ -47,11 +54,14 @@ struct stat_t {
     dg_runtime_info = nullptr;
     h_runtime_info = nullptr;
   }
+  // new run: history cleared, rule trees until a network is installed
   void reset() {
     last_time = avg_time = 0;
     last_filter_time = last_expand_time = 0;
     avg_filter_time = avg_expand_time = 0;
     level = 0;
+    mlp = gsw_mlp::sel_dev_t{};
+    mlp_host_w = nullptr;
   }
   // partial slots the counting passes add into
   long long *slots() const { return dg_runtime_info + SLOT_BASE; }

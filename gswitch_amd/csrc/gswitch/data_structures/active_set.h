@@ -149,6 +149,10 @@ struct active_set_t {
   unsigned long long *dg_work_base = nullptr; // [WORK_SLOTS][WORK_ROWS][WORK_N]
   int32_t *dg_block_count = nullptr;          // sorted-filter scratch [1025]
   int32_t *h_ctrl = nullptr;                  // pinned mirror of dg_ctrl
+  // host-side counts of the current frontier (from the inspector's
+  // counters, or the root): the filter sizes its output queue with them
+  // without a device round trip
+  long long known_active = 0, known_inactive = 0;
   bool converged = false;                     // host view of the halt flag
   bool window_moved = false;                  // finish() advanced the window
 

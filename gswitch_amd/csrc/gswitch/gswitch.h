@@ -17,46 +17,29 @@
 #include "data_structures/graph.h"
 #include "kernels/misc.h"
 #include "model/trees_table.h"
-
-// Host graph bundle (graph_t, src/data_structures/graph.cuh:177-181).
-template <GraphFmt Fmt, typename E> struct graph_t {
-  const gsw_host::HostGraph *hg = nullptr;
-  graph_attr_t attr;
-  double mean_weight = 0;
-  device_graph_t<Fmt, E> dg;
-};
-
-// Upload a host CSR and fill the graph-level features (Fake<VC|EC>::func2,
-// src/data_structures/graph.cuh:203-217).
-template <Centric C, typename E = Empty>
-auto build_graph(const gsw_host::HostGraph &hg, feature_t &fets) {
-  constexpr GraphFmt fmt = (C == VC) ? CSR : COO;
-  graph_t<fmt, E> g;
-  g.hg = &hg;
-  g.attr.fill(hg.row_ptr.data(), hg.n);
-  if (hg.weighted && !hg.w.empty()) {
-    double s = 0;
-    for (uint32_t x : hg.w) s += x;
-    g.mean_weight = s / (double)hg.w.size();
-  }
-  g.dg.build(hg);
-  fets.set_graph((double)hg.n, (double)(hg.directed ? hg.m : hg.m / 2),
-                 g.attr, g.mean_weight);
-  return g;
-}
+#include "utils/cmdline.h"
+#include "utils/fileio.h"
 
 // init_conf (src/gswitch.h:19-63): initial fused/standalone choice from the
 // select_fusion tree on (avg, std, range, GI, Her) of the degrees, disabled
 // for from-all & to-all algorithms, forced override, SSSP window size
 // 8 * mean_weight * 32 / avg_degree.
+// A command line parsed by parse_cmd adds its pinned choices (--configs),
+// its verbosity and its training-record file (--json).
 template <typename Gr, typename F>
 void init_conf(stat_t &stats, feature_t &fets, config_t &conf, Gr &g, F &f) {
   stats.build();
   fets.architecture_features();
   conf.conf_inherit = false;
-  if (fets.use_root >= 0 && g.hg && fets.first_active <= 1)
-    fets.first_workload = (double)(g.hg->row_ptr[fets.use_root + 1] -
-                                   g.hg->row_ptr[fets.use_root]);
+  if (cmd_forces_anything(cmd_opt.ins) && !cmd_forces_anything(conf.force))
+    conf.force = cmd_opt.ins;
+  if (cmd_opt.verbose) conf.verbose = true;
+  if (!cmd_opt.json.empty() && record_sink().path != cmd_opt.json) {
+    record_sink().path = cmd_opt.json;
+    record_sink().written = 0;
+  }
+  if (fets.use_root >= 0 && fets.first_active <= 1)
+    fets.first_workload = g.root_degree(fets.use_root);
   if (fets.centric == VC) {
     fets.flatten();
     double fv[5];
@@ -122,9 +105,11 @@ bool active_set_t::finish(G &g, F &f, config_t &conf) {
   }
 }
 
-// Pointer jumping to a fixpoint (p_jump, src/utils/intrinsics.cuh:351-362).
+// Pointer jumping to a fixpoint (p_jump, src/utils/intrinsics.cuh:351-362):
+// the functor by value as in the reference (it only carries device
+// pointers), dg_flag one device int, h_flag one pinned host int.
 template <typename F>
-void p_jump(F &f, int64_t nvertexs, int32_t *dg_flag, int32_t *h_flag) {
+void p_jump(F f, int64_t nvertexs, int32_t *dg_flag, int32_t *h_flag) {
   for (;;) {
     gsw::set_one_async<int32_t>(dg_flag, 0);
     hipLaunchKernelGGL(gsw::k_pointer_jumping, dim3(gsw::grid_for(nvertexs)),

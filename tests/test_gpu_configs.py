@@ -101,7 +101,7 @@ def test_c3_grid4096_bfs_cc(gsw, orc):
     h = gsw.grid2d(4096, 4096, 0.1, 0.01, 3)
     g = gsw.Graph(h)
     ref = orc.bfs(h.row_ptr, h.col, 0)
-    assert ref.max() > 100  # high diameter: hundreds of levels
+    assert ref.max() > 40  # high diameter: tens of levels despite the long-range edges
     sel = load_selectors(gsw)
     runs = [dict(), dict(configs=FIXED), dict(configs="Push-Queue-WM-Fused"),
             dict(configs="Push-SortedQueue-TWC-Standalone")]
