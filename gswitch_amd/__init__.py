@@ -110,6 +110,8 @@ _SIGS = {
                                 C.POINTER(RunStats)]),
     "gsw_bfs_multi": (C.c_int32, [_P, _P, C.c_int32, C.POINTER(RunOpts), _P,
                                   C.POINTER(RunStats)]),
+    "gsw_batch_bfs_cc": (C.c_int32, [_P, _P, C.c_int32, C.POINTER(RunOpts), _P, _P,
+                                     C.POINTER(RunStats)]),
     "gsw_autotune_bfs": (C.c_int32, [_P, _P, C.c_int32, C.c_int32,
                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32),
@@ -513,6 +515,20 @@ class Graph:
                                   C.byref(st)))
         self.last_stats = st
         return lv
+
+    def batch_bfs_cc(self, offsets, configs=None, out=True, **kw):
+        """BFS from every instance's first vertex and CC of a disjoint union
+        (instance offsets[i] .. offsets[i+1]-1): (levels, labels)."""
+        off = np.ascontiguousarray(offsets, np.int64)
+        kw.setdefault("capture", 8)
+        o = self._opts(configs, kw)
+        st = RunStats()
+        lv = np.empty(self.n, np.int32) if out else None
+        lab = np.empty(self.n, np.int32) if out else None
+        _check(_lib.gsw_batch_bfs_cc(self._h, _ptr(off), len(off) - 1, C.byref(o),
+                                     _ptr(lv), _ptr(lab), C.byref(st)))
+        self.last_stats = st
+        return lv, lab
 
     def explore_bfs(self, root, max_levels=64, stride=1, reps=2):
         """Exhaustive per-level timing of the 30 variant classes for BFS from

@@ -7,6 +7,7 @@
 #include <string>
 
 #include "apps.h"
+#include "gswitch/kernels/cc.h"
 #include "gswitch/kernels/msbfs.h"
 #include "host/host_graph.h"
 
@@ -208,6 +209,26 @@ struct gsw_graph {
   hipGraph_t cap_graph = nullptr;
   hipGraphExec_t cap_exec = nullptr;
   int cap_key[5] = {-1, -1, -1, -1, -1};
+  // sampled-hooking CC: labels (parent pointers), the vertex sample for the
+  // skipped component, batch instance offsets and the captured kernel
+  // sequence of the batched form (instantiated per instance count)
+  int32_t *cc_lab = nullptr;
+  int32_t *cc_ids = nullptr;   // sampled vertex ids [CC_SAMPLE]
+  int32_t *cc_roots = nullptr; // their roots [CC_SAMPLE]
+  int32_t *cc_hroots = nullptr;  // pinned copy
+  int64_t *inst_off = nullptr;
+  int64_t inst_cap = 0;
+  hipGraph_t cc_graph = nullptr;
+  hipGraphExec_t cc_exec = nullptr;
+  int64_t cc_key = -1;
+
+  void drop_cc_capture() {
+    if (cc_exec) hipGraphExecDestroy(cc_exec);
+    if (cc_graph) hipGraphDestroy(cc_graph);
+    cc_exec = nullptr;
+    cc_graph = nullptr;
+    cc_key = -1;
+  }
 
   void drop_capture() {
     if (cap_exec) hipGraphExecDestroy(cap_exec);
@@ -219,6 +240,12 @@ struct gsw_graph {
 
   void release() {
     drop_capture();
+    drop_cc_capture();
+    if (cc_lab) hipFree(cc_lab);
+    if (cc_ids) hipFree(cc_ids);
+    if (cc_roots) hipFree(cc_roots);
+    if (cc_hroots) hipHostFree(cc_hroots);
+    if (inst_off) hipFree(inst_off);
     if (d_roots) hipFree(d_roots);
     if (d_level) hipFree(d_level);
     if (bfs) bfs->release();
@@ -755,6 +782,112 @@ double bfs_probe_level(gsw_graph *G, app_ctx_t<BFS> &c, int root, int cls,
   return wall;
 }
 
+// ---- connected components by sampled hooking (kernels/cc.h) --------------
+constexpr int CC_SAMPLE = 1024;
+
+void cc_buffers(gsw_graph *G) {
+  if (G->cc_lab) return;
+  H_ERR(hipMalloc((void **)&G->cc_lab, sizeof(int32_t) * (G->hg.n + 1)));
+  H_ERR(hipMalloc((void **)&G->cc_ids, sizeof(int32_t) * CC_SAMPLE));
+  H_ERR(hipMalloc((void **)&G->cc_roots, sizeof(int32_t) * CC_SAMPLE));
+  H_ERR(hipHostMalloc((void **)&G->cc_hroots, sizeof(int32_t) * CC_SAMPLE));
+  // a fixed seeded sample of vertex ids
+  std::vector<int32_t> ids(CC_SAMPLE);
+  const uint64_t key = gsw_host::stream_key(0xCC5A3D1Eull, 0);
+  for (int i = 0; i < CC_SAMPLE; ++i)
+    ids[i] = (int32_t)gsw_host::below(gsw_host::draw(key, (uint64_t)i),
+                                      (uint64_t)G->hg.n);
+  H_ERR(hipMemcpy(G->cc_ids, ids.data(), sizeof(int32_t) * CC_SAMPLE,
+                  hipMemcpyHostToDevice));
+}
+
+// init, two sampling rounds, compress (phases 1-2)
+void cc_sample_phase(gsw_graph *G) {
+  using namespace gsw;
+  hipStream_t st = stream();
+  const int64_t n = G->hg.n;
+  const int grid = grid_for(n);
+  hipLaunchKernelGGL(k_cc_init, dim3(grid), dim3(BLOCK), 0, st, G->cc_lab, n);
+  for (int r = 0; r < 2; ++r)
+    hipLaunchKernelGGL(k_cc_sample, dim3(grid), dim3(BLOCK), 0, st,
+                       (const int64_t *)G->dg.dg_row_ptr,
+                       (const int32_t *)G->dg.dg_adj_list, n, r, G->cc_lab);
+  hipLaunchKernelGGL(k_cc_compress, dim3(grid), dim3(BLOCK), 0, st, G->cc_lab, n);
+  H_ERR(hipGetLastError());
+}
+
+// the remaining edges outside the skipped set, then compress (phases 3-4)
+void cc_finish_phase(gsw_graph *G, int skip_root, const int64_t *off,
+                     int64_t count) {
+  using namespace gsw;
+  hipStream_t st = stream();
+  const int64_t n = G->hg.n;
+  hipLaunchKernelGGL(k_cc_rest, dim3(grid_for((n + 63) / 64 * 64)), dim3(BLOCK), 0,
+                     st, (const int64_t *)G->dg.dg_row_ptr,
+                     (const int32_t *)G->dg.dg_adj_list, n, 2, skip_root, off,
+                     count, G->cc_lab);
+  hipLaunchKernelGGL(k_cc_compress, dim3(grid_for(n)), dim3(BLOCK), 0, st,
+                     G->cc_lab, n);
+  H_ERR(hipGetLastError());
+}
+
+// CC of one graph: the skipped component is the most frequent root among a
+// fixed vertex sample (directed graphs skip nothing: an out-edge is seen
+// from one end only). One host read (the sample's roots).
+void cc_sampled_run(gsw_graph *G) {
+  cc_buffers(G);
+  hipStream_t st = gsw::stream();
+  cc_sample_phase(G);
+  int skip = -2;
+  if (!G->hg.directed) {
+    hipLaunchKernelGGL(gsw::k_cc_sample_roots, dim3(CC_SAMPLE / gsw::BLOCK),
+                       dim3(gsw::BLOCK), 0, st, (const int *)G->cc_lab,
+                       (const int32_t *)G->cc_ids, CC_SAMPLE, G->cc_roots);
+    H_ERR(hipMemcpyAsync(G->cc_hroots, G->cc_roots, sizeof(int32_t) * CC_SAMPLE,
+                         hipMemcpyDeviceToHost, st));
+    gsw::sync();
+    std::vector<int32_t> r(G->cc_hroots, G->cc_hroots + CC_SAMPLE);
+    std::sort(r.begin(), r.end());
+    int best = 0;
+    for (int i = 0, j; i < CC_SAMPLE; i = j) {
+      for (j = i; j < CC_SAMPLE && r[j] == r[i]; ++j) {
+      }
+      if (j - i > best) {
+        best = j - i;
+        skip = r[i];
+      }
+    }
+  }
+  cc_finish_phase(G, skip, nullptr, 1);
+}
+
+// Batched CC over disjoint instances (offsets on the device): the whole
+// kernel sequence is one captured graph, replayed without host decisions.
+void cc_batched_captured(gsw_graph *G, int64_t count) {
+  cc_buffers(G);
+  hipStream_t st = gsw::stream();
+  // the sequence's arguments change only with the instance count (a new
+  // offsets buffer drops the capture when it is allocated)
+  const int64_t key = count;
+  if (!G->cc_exec || G->cc_key != key) {
+    G->drop_cc_capture();
+    H_ERR(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+      cc_sample_phase(G);
+      cc_finish_phase(G, -1, G->inst_off, count);
+    } catch (...) {
+      hipGraph_t dead = nullptr;
+      (void)hipStreamEndCapture(st, &dead);
+      if (dead) (void)hipGraphDestroy(dead);
+      throw;
+    }
+    H_ERR(hipStreamEndCapture(st, &G->cc_graph));
+    H_ERR(hipGraphInstantiate(&G->cc_exec, G->cc_graph, nullptr, nullptr, 0));
+    G->cc_key = key;
+  }
+  H_ERR(hipGraphLaunch(G->cc_exec, st));
+}
+
 }  // namespace
 
 extern "C" {
@@ -1106,6 +1239,65 @@ gsw_status gsw_bfs_multi(gsw_graph *G, const int32_t *roots, int32_t nroots,
   });
 }
 
+gsw_status gsw_batch_bfs_cc(gsw_graph *G, const int64_t *offsets, int32_t count,
+                            const gsw_run_opts *opts, int32_t *levels_out,
+                            int32_t *labels_out, gsw_run_stats *stats) {
+  return guarded([&] {
+    GSW_REQUIRE(G && offsets && count >= 1, GSW_E_INVALID, "invalid arguments");
+    GSW_REQUIRE(!G->hg.directed, GSW_E_UNSUPPORTED,
+                "batched BFS + CC needs an undirected graph");
+    GSW_REQUIRE(offsets[0] == 0 && offsets[count] == G->hg.n, GSW_E_INVALID,
+                "instance offsets must run from 0 to n");
+    std::vector<int32_t> roots;
+    for (int32_t i = 0; i < count; ++i) {
+      GSW_REQUIRE(offsets[i] <= offsets[i + 1], GSW_E_INVALID,
+                  "instance offsets must not decrease");
+      if (offsets[i] < offsets[i + 1]) roots.push_back((int32_t)offsets[i]);
+    }
+    const run_opts_t o = to_opts(opts);
+    LB lb = WM;
+    if (o.force.has_lb) {
+      GSW_REQUIRE(o.force.lb == WM || o.force.lb == TWC, GSW_E_UNSUPPORTED,
+                  "captured BFS supports the WM and TWC expands");
+      lb = o.force.lb;
+    }
+    const int capture = (opts && opts->capture > 0) ? opts->capture : 8;
+    const int grid = (opts && opts->grid > 0) ? opts->grid : default_capture_grid(G);
+    double workload = 0;
+    upload_roots(G, roots.data(), (int32_t)roots.size(), workload);
+    if (count + 1 > G->inst_cap) {
+      if (G->inst_off) H_ERR(hipFree(G->inst_off));
+      G->inst_off = nullptr;
+      H_ERR(hipMalloc((void **)&G->inst_off, sizeof(int64_t) * (count + 1)));
+      G->inst_cap = count + 1;
+      G->drop_cc_capture();
+    }
+    H_ERR(hipMemcpy(G->inst_off, offsets, sizeof(int64_t) * (count + 1),
+                    hipMemcpyHostToDevice));
+    // BFS: captured fused level loop from every instance's first vertex
+    const int levels = bfs_captured_run(G, (int64_t)roots.size(), lb, o.twc_small,
+                                        o.twc_large, grid, 2 * capture,
+                                        o.max_levels);
+    // CC: one captured launch sequence over the same graph
+    cc_batched_captured(G, count);
+    H_ERR(hipEventRecord(G->ev1, gsw::stream()));
+    const double ms = elapsed(G);
+    G->records.clear();
+    if (levels_out)
+      H_ERR(hipMemcpy(levels_out, G->bfs->f.data.dg_wa, sizeof(int32_t) * G->hg.n,
+                      hipMemcpyDeviceToHost));
+    if (labels_out)
+      H_ERR(hipMemcpy(labels_out, G->cc_lab, sizeof(int32_t) * G->hg.n,
+                      hipMemcpyDeviceToHost));
+    if (stats) {
+      stats->device_ms = ms;
+      stats->levels = levels;
+      stats->n_records = 0;
+      stats->reached = 0;
+    }
+  });
+}
+
 gsw_status gsw_autotune_bfs(gsw_graph *G, const int32_t *roots, int32_t nroots,
                             int32_t capture, int32_t *best_lb,
                             int32_t *best_small, int32_t *best_large,
@@ -1304,6 +1496,25 @@ gsw_status gsw_cc(gsw_graph *G, const gsw_run_opts *opts, int32_t *labels_out,
   return guarded([&] {
     GSW_REQUIRE(G, GSW_E_INVALID, "null graph");
     const run_opts_t o = to_opts_rule(opts);
+    if (!(opts && opts->configs)) {
+      // sampled hooking over the CSR
+      hipStream_t st = gsw::stream();
+      H_ERR(hipEventRecord(G->ev0, st));
+      cc_sampled_run(G);
+      H_ERR(hipEventRecord(G->ev1, st));
+      const double ms = elapsed(G);
+      G->records.clear();
+      if (labels_out)
+        H_ERR(hipMemcpy(labels_out, G->cc_lab, sizeof(int32_t) * G->hg.n,
+                        hipMemcpyDeviceToHost));
+      if (stats) {
+        stats->device_ms = ms;
+        stats->levels = 1;
+        stats->n_records = 0;
+        stats->reached = 0;
+      }
+      return;
+    }
     if (!G->has_coo) {
       G->coo.build(G->hg);
       G->has_coo = true;

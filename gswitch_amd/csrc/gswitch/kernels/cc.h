@@ -1,0 +1,158 @@
+// Connected components by tree hooking over the CSR, with subgraph sampling
+// (Afforest: Sutton, Ben-Nun, Bar-Noy, "Optimizing Parallel Graph
+// Connectivity Computation via Subgraph Sampling", IPDPS 2018).
+//
+// p[v] is a parent pointer; roots have p[r] = r. link(u, v) hooks the larger
+// of the two roots under the smaller one with a compare-and-swap, so every
+// tree's root is the smallest id in it and, once a component is one tree,
+// its root is the component's minimum vertex id: the canonical label of the
+// reference's CC (application/cc.cu:34-66). link never waits on another
+// lane: each failed CAS means the root it tried moved to a smaller value, so
+// the loop ends.
+//
+// Phases (a fixed sequence of launches, no host decisions in between for
+// the batched form, so it can live in a captured graph):
+//  1. p = identity;
+//  2. sampling: every vertex links its first two neighbours, then compress;
+//  3. skip set S: vertices whose root is a chosen root -- the most frequent
+//     root of a vertex sample for one graph, or each instance's first vertex
+//     (its smallest id) for a disjoint batch. Vertices outside S link all
+//     their remaining edges; a vertex in S skips the edges that stay inside
+//     its instance (their other end is either in the same tree already or
+//     outside S, and then links the edge itself) and links only edges that
+//     leave its instance, so a batch whose instances share edges stays
+//     correct;
+//  4. compress: p[v] = root(v).
+// Undirected graphs only use S (an edge must be visible from both ends);
+// a directed graph links every out-edge of every vertex (weak components).
+#pragma once
+#include "../utils/runtime.h"
+#include "../utils/wave.h"
+
+namespace gsw {
+
+constexpr int CC_WIDE = 128;  // degree above which a wavefront shares a row
+
+__device__ __forceinline__ int cc_get(const int *p, int64_t i) {
+  return __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// join the trees of u and v
+__device__ __forceinline__ void cc_link(int u, int v, int *p) {
+  int a = cc_get(p, u), b = cc_get(p, v);
+  while (a != b) {
+    const int hi = a > b ? a : b, lo = a > b ? b : a;
+    const int ph = cc_get(p, hi);
+    if (ph == lo) return;
+    if (ph == hi && atomicCAS(p + hi, hi, lo) == hi) return;
+    a = cc_get(p, cc_get(p, hi));
+    b = cc_get(p, lo);
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_cc_init(int *p, int64_t n) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    p[v] = (int)v;
+}
+
+// link every vertex with its r-th neighbour
+__global__ void __launch_bounds__(BLOCK)
+    k_cc_sample(const int64_t *__restrict__ rows, const int32_t *__restrict__ cols,
+                int64_t n, int r, int *p) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = rows[v];
+    if (rows[v + 1] - s > r) cc_link((int)v, cols[s + r], p);
+  }
+}
+
+// p[v] <- root of v (no links run meanwhile, so p is a forest)
+__global__ void __launch_bounds__(BLOCK) k_cc_compress(int *p, int64_t n) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    int x = cc_get(p, v);
+    for (int y = cc_get(p, x); y != x; y = cc_get(p, x)) x = y;
+    p[v] = x;
+  }
+}
+
+// roots of `k` sampled vertices (the host takes the most frequent one)
+__global__ void __launch_bounds__(BLOCK)
+    k_cc_sample_roots(const int *p, const int32_t *ids, int k, int *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < k) out[i] = p[ids[i]];
+}
+
+// Instance of v: the largest i with offsets[i] <= v (offsets ascending,
+// offsets[0] = 0, count + 1 entries).
+__device__ __forceinline__ int64_t cc_instance(const int64_t *off, int64_t count,
+                                               int64_t v) {
+  int64_t lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (off[mid] <= v)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Phase 3. skip_root >= 0: S = {v : p[v] == skip_root} (one graph);
+// skip_root == -1: S = {v : p[v] == first vertex of v's instance} with
+// `off` the count + 1 instance offsets; skip_root == -2: no skipping.
+// Edges [from, deg) of each row are visited (the sampled ones are done).
+// One wavefront per 64 vertices; rows longer than CC_WIDE are walked by
+// the whole wavefront, one at a time.
+__global__ void __launch_bounds__(BLOCK)
+    k_cc_rest(const int64_t *__restrict__ rows, const int32_t *__restrict__ cols,
+              int64_t n, int from, int skip_root, const int64_t *__restrict__ off,
+              int64_t count, int *p) {
+  const int lane = lane_id();
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> LOG_WAVE;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> LOG_WAVE;
+  for (int64_t base = wave << 6; base < n; base += nwaves << 6) {
+    const int64_t v = base + lane;
+    int64_t s = 0, e = 0, lo = 0, hi = 0;
+    bool in_s = false;
+    if (v < n) {
+      s = rows[v] + from;
+      e = rows[v + 1];
+      if (s > e) s = e;
+      const int root = cc_get(p, v);
+      if (skip_root >= 0) {
+        in_s = root == skip_root;
+        hi = n;
+      } else if (skip_root == -1) {
+        const int64_t i = cc_instance(off, count, v);
+        lo = off[i];
+        hi = off[i + 1];
+        in_s = root == (int)lo;
+      }
+      // one graph: a vertex of S has nothing to link
+      if (in_s && skip_root >= 0) e = s;
+    }
+    const bool wide = (e - s) > CC_WIDE;
+    if (!wide)
+      for (int64_t j = s; j < e; ++j) {
+        const int u = cols[j];
+        if (!in_s || u < lo || u >= hi) cc_link((int)v, u, p);
+      }
+    uint64_t wmask = ballot(wide);
+    while (wmask) {
+      const int w = __builtin_ctzll(wmask);
+      wmask &= wmask - 1;
+      const int64_t ws = __shfl(s, w, 64), we = __shfl(e, w, 64);
+      const int64_t wl = __shfl(lo, w, 64), wh = __shfl(hi, w, 64);
+      const bool wi = __shfl((int)in_s, w, 64) != 0;
+      const int wv = (int)(base + w);
+      for (int64_t j = ws + lane; j < we; j += WAVE) {
+        const int u = cols[j];
+        if (!wi || u < wl || u >= wh) cc_link(wv, u, p);
+      }
+    }
+  }
+}
+
+}  // namespace gsw

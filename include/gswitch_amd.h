@@ -165,9 +165,13 @@ gsw_status gsw_bfs(gsw_graph *g, int32_t root, const gsw_run_opts *opts,
  * window src/data_structures/window.cuh:44-123. */
 gsw_status gsw_sssp(gsw_graph *g, int32_t root, const gsw_run_opts *opts,
                     uint32_t *dist_out, gsw_run_stats *stats);
-/* Connected components, label = smallest vertex id in the component.
- * Replaces run_cc (application/cc.cu:82-108): edge-centric hooking
- * (cc.cu:15-32) + pointer jumping (src/utils/intrinsics.cuh:328-362). */
+/* Connected components, label = smallest vertex id in the component
+ * (run_cc, application/cc.cu:82-108). By default: tree hooking over the CSR
+ * with subgraph sampling (two sampled neighbours per vertex, then the
+ * remaining edges of every vertex outside the most frequent sampled
+ * component). With opts->configs set: the reference's edge-centric hooking
+ * (cc.cu:15-32) + pointer jumping (src/utils/intrinsics.cuh:328-362) through
+ * the level loop, honouring the pinned choices. */
 gsw_status gsw_cc(gsw_graph *g, const gsw_run_opts *opts, int32_t *labels_out,
                   gsw_run_stats *stats);
 /* PageRank, un-normalised: pr_0 = 1-d, pr_{k+1}(v) = 1-d + d*sum pr_k(u)/deg(u),
@@ -216,6 +220,20 @@ gsw_status gsw_msbfs64(gsw_graph *g, const int32_t *sources, int32_t nsrc,
 gsw_status gsw_bfs_multi(gsw_graph *g, const int32_t *roots, int32_t nroots,
                          const gsw_run_opts *opts, int32_t *levels_out,
                          gsw_run_stats *stats);
+/* Batched BFS + CC over a disjoint union of `count` instances (instance i =
+ * vertices offsets[i] .. offsets[i+1]-1, offsets[0] = 0, offsets[count] = n,
+ * as gsw_gen_er_batch lays them out): BFS levels from every instance's
+ * first vertex through the captured fused level loop (opts->configs LB WM or
+ * TWC, opts->capture, opts->grid, TWC thresholds), then connected
+ * components by sampled hooking as one captured launch sequence, skipping
+ * in each instance the edges inside the component of its first vertex
+ * (edges that leave an instance are always linked). Each instance's levels
+ * equal its own run_bfs from its first vertex (application/bfs.cu:78-102)
+ * and its labels its own run_cc (application/cc.cu:82-108). device_ms covers
+ * both. Undirected graphs only. No counterpart in the reference. */
+gsw_status gsw_batch_bfs_cc(gsw_graph *g, const int64_t *offsets, int32_t count,
+                            const gsw_run_opts *opts, int32_t *levels_out,
+                            int32_t *labels_out, gsw_run_stats *stats);
 /* Autotuner for the captured batched BFS: times {WM, TWC} x TWC bin
  * thresholds x workgroups per launch on this graph and roots (best of 3)
  * and returns the fastest (lb uses the gsw_level_record codes; feed the

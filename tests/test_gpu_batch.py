@@ -68,6 +68,46 @@ def test_batched_cc(gsw, orc, batch):
         assert np.array_equal(lab[off[i]:off[i + 1]] - off[i], orc.cc(rp, col)), i
 
 
+@pytest.mark.parametrize("configs,capture,grid", [
+    ("Push-x-WM-x", 1, 0), ("Push-x-WM-x", 8, 7), ("Push-x-TWC-x", 4, 0)])
+def test_batch_bfs_cc(gsw, orc, batch, configs, capture, grid):
+    """BFS from every instance's first vertex and CC in one call (captured
+    BFS loop, captured CC sequence): every instance equals its own oracle
+    runs, also on repeated calls (the captured CC is replayed)."""
+    h, off, g = batch
+    for rep in range(2):
+        lv, lab = g.batch_bfs_cc(off, configs=configs, capture=capture, grid=grid)
+        check_batched_levels(orc, h, off, lv)
+        for i in range(len(off) - 1):
+            rp, col = instance(h, off, i)
+            assert np.array_equal(lab[off[i]:off[i + 1]] - off[i], orc.cc(rp, col)), \
+                (rep, i)
+    assert g.last_stats.levels == lv.max() + 1
+
+
+def test_batch_bfs_cc_instances_sharing_edges(gsw, orc):
+    """Offsets that do not cut the graph into separate pieces: edges between
+    "instances" are linked from both ends, so the labels are still the
+    whole graph's components; the levels are the multi-root BFS from the
+    instances' first vertices."""
+    h = gsw.grid2d(30, 40, 0.3, 0.01, 11)
+    g = gsw.Graph(h)
+    off = np.array([0, 1, 300, 301, 640, 1199, 1200], np.int64)
+    lv, lab = g.batch_bfs_cc(off, configs="Push-x-WM-x", capture=2)
+    assert np.array_equal(lab, orc.cc(h.row_ptr, h.col))
+    big = np.iinfo(np.int32).max
+    exp = np.full(h.n, big)
+    for r in off[:-1]:
+        x = orc.bfs(h.row_ptr, h.col, int(r))
+        exp = np.minimum(exp, np.where(x < 0, big, x))
+    assert np.array_equal(lv, np.where(exp == big, -1, exp).astype(np.int32))
+    with pytest.raises(gsw.GswError):
+        g.batch_bfs_cc(np.array([0, 5], np.int64))  # does not end at n
+    with pytest.raises(gsw.GswError):
+        g.batch_bfs_cc(np.array([0, 700, 600, h.n], np.int64))  # decreasing
+    g.close()
+
+
 def test_captured_replays_repeat(gsw, orc, batch):
     """The captured graph is reused: repeated runs and alternating shapes
     give the same levels (state is re-initialised outside the graph)."""

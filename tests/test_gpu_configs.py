@@ -11,9 +11,9 @@ with the CPU oracle on it:
     highest-degree vertices: masks and per-source level sums bit-exact
     against 64 oracle BFS runs;
   * one whole 1000-instance piece of the 20,000-graph Erdos-Renyi batch
-    (seed 5) through the autotuned captured BFS and CC on the union, every
-    instance against its own oracle run, plus seeded samples of instances
-    from the rest of the batch;
+    (seed 5) through the autotuned captured BFS and the captured CC of
+    gsw_batch_bfs_cc, every instance against its own oracle runs, plus
+    seeded samples of instances from the rest of the batch;
   * held-out selector evaluation graphs (R-MAT scale 20, Erdos-Renyi, grid)
     under the rule trees and under both committed learned selectors.
 The oracle runs of one test are spread over threads (the ctypes calls
@@ -161,11 +161,10 @@ def run_batch(g, off, tune=None):
     roots = off[:-1].astype(np.int32)
     if tune is None:
         tune = g.autotune_bfs(roots, capture=8)
-    lv = g.bfs_multi(roots, configs="Push-x-%s-x" % tune["lb"], capture=8,
-                     grid=tune["grid"], twc_small=tune["twc_small"],
-                     twc_large=tune["twc_large"])
+    lv, lab = g.batch_bfs_cc(off, configs="Push-x-%s-x" % tune["lb"], capture=8,
+                             grid=tune["grid"], twc_small=tune["twc_small"],
+                             twc_large=tune["twc_large"])
     assert g.last_stats.levels == lv.max() + 1
-    lab = g.cc()
     return lv, lab, tune
 
 

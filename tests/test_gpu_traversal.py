@@ -154,6 +154,29 @@ def test_cc(gsw, orc, gset, gname, variant):
     assert np.array_equal(g.cc(configs=variant), orc.cc(h.row_ptr, h.col))
 
 
+def test_cc_sampled_hooking_shapes(gsw, orc):
+    """The default CC (sampled hooking): many small components, isolated
+    vertices, a star (one hub holding every edge, walked by whole
+    wavefronts) and a directed graph (weak components, no skipped set)."""
+    rng = np.random.default_rng(4)
+    h = gsw.grid2d(50, 50, 0.45, 0.0, 9)
+    g = gsw.Graph(h)
+    assert np.array_equal(g.cc(), orc.cc(h.row_ptr, h.col))
+    g.close()
+    rp, col = csr_from_edges(3000, [(0, v) for v in range(1, 3000, 2)] +
+                             [(v, v + 1) for v in range(2, 2990, 7)])
+    g = gsw.Graph(row_ptr=rp, col=col)
+    assert np.array_equal(g.cc(), orc.cc(rp, col))
+    g.close()
+    n = 2000
+    src, dst = rng.integers(0, n, 1500), rng.integers(0, n, 1500)
+    hd = gsw.from_edges(n, src, dst, directed=True)
+    g = gsw.Graph(hd)
+    srp, scol = csr_from_edges(n, np.stack([src, dst], 1))
+    assert np.array_equal(g.cc(), orc.cc(srp, scol))
+    g.close()
+
+
 # ---------------------------------------------------------------- PageRank
 def max_rel(a, b):
     return float(np.max(np.abs(a.astype(np.float64) - b) / np.abs(b)))
