@@ -90,6 +90,9 @@ struct CC : Functor<EC, int, Empty, Empty> {
 };
 
 struct PR : Functor<VC, double, float, Empty> {
+  // a message is a stored FP32 rank widened to FP64: it is stored exactly
+  // as FP32 where messages are staged per edge (propagation blocking)
+  using msg_t = float;
   float damp;
   float eps;
   __device__ Status filter(int v, GU g) {
@@ -280,6 +283,7 @@ struct run_opts_t {
   bool sorted = false;
   int twc_small = 32, twc_large = 2048;
   bool mp_pull = true;
+  bool pb_pull = false;
   int selector = 0;  // 0 rule trees, 1 learned MLP
 };
 
@@ -310,6 +314,7 @@ inline void apply_opts(config_t &conf, const run_opts_t &o) {
   conf.twc_small = o.twc_small;
   conf.twc_large = o.twc_large;
   conf.mp_pull = o.mp_pull;
+  conf.pb_pull = o.pb_pull;
 }
 
 // Generic vertex-centric level loop (application/bfs.cu:87-98 shape).

@@ -213,9 +213,7 @@ struct gsw_graph {
   // skipped component, batch instance offsets and the captured kernel
   // sequence of the batched form (instantiated per instance count)
   int32_t *cc_lab = nullptr;
-  int32_t *cc_ids = nullptr;   // sampled vertex ids [CC_SAMPLE]
-  int32_t *cc_roots = nullptr; // their roots [CC_SAMPLE]
-  int32_t *cc_hroots = nullptr;  // pinned copy
+  int32_t *cc_ids = nullptr;   // sampled vertex ids [CC_SAMPLE] + chosen root
   int64_t *inst_off = nullptr;
   int64_t inst_cap = 0;
   hipGraph_t cc_graph = nullptr;
@@ -243,8 +241,6 @@ struct gsw_graph {
     drop_cc_capture();
     if (cc_lab) hipFree(cc_lab);
     if (cc_ids) hipFree(cc_ids);
-    if (cc_roots) hipFree(cc_roots);
-    if (cc_hroots) hipHostFree(cc_hroots);
     if (inst_off) hipFree(inst_off);
     if (d_roots) hipFree(d_roots);
     if (d_level) hipFree(d_level);
@@ -339,6 +335,7 @@ run_opts_t to_opts(const gsw_run_opts *o) {
   if (o->twc_small > 0) r.twc_small = o->twc_small;
   if (o->twc_large > 0) r.twc_large = o->twc_large;
   r.mp_pull = o->pull_kernel == 0;
+  r.pb_pull = o->pull_kernel == 2;
   r.selector = o->selector;
   return r;
 }
@@ -788,77 +785,63 @@ constexpr int CC_SAMPLE = 1024;
 void cc_buffers(gsw_graph *G) {
   if (G->cc_lab) return;
   H_ERR(hipMalloc((void **)&G->cc_lab, sizeof(int32_t) * (G->hg.n + 1)));
-  H_ERR(hipMalloc((void **)&G->cc_ids, sizeof(int32_t) * CC_SAMPLE));
-  H_ERR(hipMalloc((void **)&G->cc_roots, sizeof(int32_t) * CC_SAMPLE));
-  H_ERR(hipHostMalloc((void **)&G->cc_hroots, sizeof(int32_t) * CC_SAMPLE));
-  // a fixed seeded sample of vertex ids
-  std::vector<int32_t> ids(CC_SAMPLE);
-  const uint64_t key = gsw_host::stream_key(0xCC5A3D1Eull, 0);
-  for (int i = 0; i < CC_SAMPLE; ++i)
-    ids[i] = (int32_t)gsw_host::below(gsw_host::draw(key, (uint64_t)i),
-                                      (uint64_t)G->hg.n);
-  H_ERR(hipMemcpy(G->cc_ids, ids.data(), sizeof(int32_t) * CC_SAMPLE,
-                  hipMemcpyHostToDevice));
+  H_ERR(hipMalloc((void **)&G->cc_ids, sizeof(int32_t) * (CC_SAMPLE + 64)));
 }
 
-// init, two sampling rounds, compress (phases 1-2)
-void cc_sample_phase(gsw_graph *G) {
+// init, two sampling rounds, compress (phases 1-2) into lab[n]
+void cc_sample_phase(const gsw_graph *G, int32_t *lab, hipStream_t st) {
   using namespace gsw;
-  hipStream_t st = stream();
   const int64_t n = G->hg.n;
   const int grid = grid_for(n);
-  hipLaunchKernelGGL(k_cc_init, dim3(grid), dim3(BLOCK), 0, st, G->cc_lab, n);
+  hipLaunchKernelGGL(k_cc_init, dim3(grid), dim3(BLOCK), 0, st, lab, n);
   for (int r = 0; r < 2; ++r)
     hipLaunchKernelGGL(k_cc_sample, dim3(grid), dim3(BLOCK), 0, st,
                        (const int64_t *)G->dg.dg_row_ptr,
-                       (const int32_t *)G->dg.dg_adj_list, n, r, G->cc_lab);
-  hipLaunchKernelGGL(k_cc_compress, dim3(grid), dim3(BLOCK), 0, st, G->cc_lab, n);
+                       (const int32_t *)G->dg.dg_adj_list, n, r, lab);
+  hipLaunchKernelGGL(k_cc_compress, dim3(grid), dim3(BLOCK), 0, st, lab, n);
   H_ERR(hipGetLastError());
 }
 
 // the remaining edges outside the skipped set, then compress (phases 3-4)
-void cc_finish_phase(gsw_graph *G, int skip_root, const int64_t *off,
-                     int64_t count) {
+void cc_finish_phase(const gsw_graph *G, int32_t *lab, int skip_root,
+                     const int *skip_dev, const int64_t *off, int64_t count,
+                     hipStream_t st) {
   using namespace gsw;
-  hipStream_t st = stream();
   const int64_t n = G->hg.n;
   hipLaunchKernelGGL(k_cc_rest, dim3(grid_for((n + 63) / 64 * 64)), dim3(BLOCK), 0,
                      st, (const int64_t *)G->dg.dg_row_ptr,
-                     (const int32_t *)G->dg.dg_adj_list, n, 2, skip_root, off,
-                     count, G->cc_lab);
-  hipLaunchKernelGGL(k_cc_compress, dim3(grid_for(n)), dim3(BLOCK), 0, st,
-                     G->cc_lab, n);
+                     (const int32_t *)G->dg.dg_adj_list, n, 2, skip_root, skip_dev,
+                     off, count, lab);
+  hipLaunchKernelGGL(k_cc_compress, dim3(grid_for(n)), dim3(BLOCK), 0, st, lab, n);
   H_ERR(hipGetLastError());
 }
 
-// CC of one graph: the skipped component is the most frequent root among a
-// fixed vertex sample (directed graphs skip nothing: an out-edge is seen
-// from one end only). One host read (the sample's roots).
+// CC of one graph into lab[n]: the skipped component is the most frequent
+// root among `samples` sampled vertices, found on the device (directed
+// graphs skip nothing: an out-edge is seen from one end only). `ids` holds
+// samples + 1 ints of scratch (the sample, then the chosen root). No host
+// read and no allocation: every launch is queued on `st`.
+void cc_sampled_launch(const gsw_graph *G, int32_t *lab, int32_t *ids, int samples,
+                       hipStream_t st) {
+  using namespace gsw;
+  if (G->hg.n == 0) return;
+  cc_sample_phase(G, lab, st);
+  if (G->hg.directed) {
+    cc_finish_phase(G, lab, -2, nullptr, nullptr, 1, st);
+    return;
+  }
+  int32_t *skip = ids + samples;
+  hipLaunchKernelGGL(k_cc_sample_ids, dim3((samples + BLOCK - 1) / BLOCK), dim3(BLOCK),
+                     0, st, ids, samples, G->hg.n);
+  hipLaunchKernelGGL(k_cc_mode, dim3(1), dim3(CC_MODE_BLOCK), 0, st,
+                     (const int *)lab, (const int32_t *)ids, samples, skip);
+  H_ERR(hipGetLastError());
+  cc_finish_phase(G, lab, -2, skip, nullptr, 1, st);
+}
+
 void cc_sampled_run(gsw_graph *G) {
   cc_buffers(G);
-  hipStream_t st = gsw::stream();
-  cc_sample_phase(G);
-  int skip = -2;
-  if (!G->hg.directed) {
-    hipLaunchKernelGGL(gsw::k_cc_sample_roots, dim3(CC_SAMPLE / gsw::BLOCK),
-                       dim3(gsw::BLOCK), 0, st, (const int *)G->cc_lab,
-                       (const int32_t *)G->cc_ids, CC_SAMPLE, G->cc_roots);
-    H_ERR(hipMemcpyAsync(G->cc_hroots, G->cc_roots, sizeof(int32_t) * CC_SAMPLE,
-                         hipMemcpyDeviceToHost, st));
-    gsw::sync();
-    std::vector<int32_t> r(G->cc_hroots, G->cc_hroots + CC_SAMPLE);
-    std::sort(r.begin(), r.end());
-    int best = 0;
-    for (int i = 0, j; i < CC_SAMPLE; i = j) {
-      for (j = i; j < CC_SAMPLE && r[j] == r[i]; ++j) {
-      }
-      if (j - i > best) {
-        best = j - i;
-        skip = r[i];
-      }
-    }
-  }
-  cc_finish_phase(G, skip, nullptr, 1);
+  cc_sampled_launch(G, G->cc_lab, G->cc_ids, CC_SAMPLE, gsw::stream());
 }
 
 // Batched CC over disjoint instances (offsets on the device): the whole
@@ -873,8 +856,8 @@ void cc_batched_captured(gsw_graph *G, int64_t count) {
     G->drop_cc_capture();
     H_ERR(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     try {
-      cc_sample_phase(G);
-      cc_finish_phase(G, -1, G->inst_off, count);
+      cc_sample_phase(G, G->cc_lab, st);
+      cc_finish_phase(G, G->cc_lab, -1, nullptr, G->inst_off, count, st);
     } catch (...) {
       hipGraph_t dead = nullptr;
       (void)hipStreamEndCapture(st, &dead);
@@ -2365,3 +2348,4 @@ gsw_status gsw_scan_exclusive_i64(const int64_t *in, int64_t n, int64_t *out) {
 }
 
 }  // extern "C"
+

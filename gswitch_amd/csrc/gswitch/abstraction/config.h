@@ -65,6 +65,9 @@ struct config_t {
   // from-all bitmap pulls use the merge-path kernel (k_pull_mp*) instead of
   // the segmented-reduction STRICT/TWOD kernels
   bool mp_pull = true;
+  // from-all pulls of unweighted graphs through the propagation-blocked
+  // layout (kernels/pb.h) instead of the TWOD slices
+  bool pb_pull = false;
   // learned selector: every level runs the variant class the MLP picks
   // (mlp_class, written by the inspector / the fused-level selection)
   bool policy_mlp = false;

@@ -21,6 +21,7 @@
 #include "../kernels/expand_push.h"
 #include "../kernels/filter.h"
 #include "../kernels/misc.h"
+#include "../kernels/pb.h"
 #include "../kernels/twod.h"
 #include "config.h"
 #include "features.h"
@@ -324,7 +325,13 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf,
     if constexpr (!G::has_edata)
       twod_ok = !push && conf.conf_fromall && as.fmt == Bitmap && g.twod;
     if (lb == ELB || (lb == TWOD && !twod_ok)) lb = STRICT;
-    if (lb == TWOD) {
+    using MSG = typename pb_msg<F>::type;
+    using WAt = typename F::wa_t;
+    const bool use_pb = lb == TWOD && conf.pb_pull && g.pb;
+    if (use_pb) {
+      pb_ensure(g, PB_LDS_BYTES / (int)sizeof(MSG), PB_LDS_BYTES / (int)sizeof(WAt));
+      pb_vals(*g.pb, sizeof(MSG));
+    } else if (lb == TWOD) {
       // slice width from the bytes of source state one message reads
       constexpr int64_t src_bytes = sizeof(typename F::ra_t) > 4 ? sizeof(typename F::ra_t) : 4;
       twod_ensure(g, src_bytes);
@@ -422,6 +429,16 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf,
                              dim3(BLOCK), 0, st, body);
           break;
         case TWOD: {
+          if (use_pb) {
+            // propagation blocking: messages to slots, then slots to targets
+            const pb_t &pb = *g.pb;
+            if (pb.n1 > 0)
+              hipLaunchKernelGGL((k_pb_scatter<Bd, MSG>), dim3(pb.n1), dim3(PB_BLOCK),
+                                 0, st, body, pb);
+            hipLaunchKernelGGL((k_pb_gather<Bd, MSG>), dim3(pb.n2), dim3(PB_BLOCK), 0,
+                               st, body, pb);
+            break;
+          }
           // the slices one after another (each launch owns its targets)
           const twod_t &tw = *g.twod;
           for (int p = 0; p < tw.P; ++p) {

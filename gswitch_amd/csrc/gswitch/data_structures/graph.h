@@ -100,6 +100,48 @@ struct twod_t {
   twod_slice_t s[TWOD_MAX];
 };
 
+// Propagation-blocked layout for from-all pulls (kernels/pb.h builds it on
+// first use). Sources are cut into chunks of S consecutive ids, targets
+// into bins of at most D consecutive ids and about ecap in-edges (a target
+// with more in-edges is a bin of its own). Tile (b, c) holds the edges from
+// chunk c into bin b; the edges are stored bin-major ([b][c] tiles, each
+// contiguous), one 16-bit source offset (within its chunk) and one 16-bit
+// target offset (within its bin) per edge. One expand is two launches:
+// phase 1, one workgroup per piece of a chunk, stages the chunk's messages
+// in LDS and writes every edge's message into `vals` at the edge's slot
+// (coalesced stores, tile by tile); phase 2, one workgroup per bin (or per
+// piece of a one-target bin), streams the bin's slots and combines them
+// into LDS accumulators. Every random access is an LDS access.
+struct pb_t {
+  int S = 0, D = 0;              // chunk width, largest bin width
+  int64_t C = 0, B = 0;          // chunks, bins
+  int64_t m = 0, ecap = 0;       // edges, edges per piece
+  int32_t *bin_lo = nullptr;     // [B + 1] first target of each bin
+  uint16_t *src = nullptr;       // [m] source - chunk start (bin-major)
+  uint16_t *dst = nullptr;       // [m] target - bin start
+  int64_t *tile_off = nullptr;   // [B * C + 1] bin-major tile offsets
+  int64_t *cm_start = nullptr;   // [C * B] tile (b, c) start, chunk-major
+  int32_t *cm_len = nullptr;     // [C * B] its length
+  int64_t n1 = 0;                // phase-1 pieces
+  int32_t *p1 = nullptr;         // [n1][4]: chunk, first bin, end bin, 0
+  int64_t n2 = 0;                // phase-2 pieces
+  int64_t *p2 = nullptr;         // [n2][4]: bin, first slot, end slot, whole bin
+  void *vals = nullptr;          // [m] messages (msg type of the functor)
+  size_t vals_bytes = 0;
+  void release() {
+    hipFree(bin_lo);
+    hipFree(src);
+    hipFree(dst);
+    hipFree(tile_off);
+    hipFree(cm_start);
+    hipFree(cm_len);
+    hipFree(p1);
+    hipFree(p2);
+    hipFree(vals);
+    *this = pb_t();
+  }
+};
+
 template <GraphFmt fmt, typename E> struct device_graph_t {};
 
 template <typename E> struct device_graph_t<CSR, E> {
@@ -126,6 +168,9 @@ template <typename E> struct device_graph_t<CSR, E> {
   // host-side handle of the TWOD slices (kernels/twod.h builds them on first
   // use); shared by every copy of this struct, owned by the one that built it
   twod_t *twod = nullptr;
+  // host-side handle of the propagation-blocked layout (kernels/pb.h), owned
+  // the same way
+  pb_t *pb = nullptr;
 
   // Upload a host CSR. `w` is converted to E when E is not Empty.
   int64_t build(const gsw_host::HostGraph &hg) {
@@ -134,6 +179,7 @@ template <typename E> struct device_graph_t<CSR, E> {
     directed = hg.directed;
     level = 0;
     twod = new twod_t();
+    pb = new pb_t();
     int64_t bytes = 0;
     H_ERR(hipMalloc((void **)&dg_row_ptr, sizeof(int64_t) * (nvertexs + 1)));
     H_ERR(hipMemcpy(dg_row_ptr, hg.row_ptr.data(),
@@ -189,6 +235,11 @@ template <typename E> struct device_graph_t<CSR, E> {
       }
       delete twod;
       twod = nullptr;
+    }
+    if (pb) {
+      pb->release();
+      delete pb;
+      pb = nullptr;
     }
     if (directed) {
       hipFree(dgr_row_ptr);

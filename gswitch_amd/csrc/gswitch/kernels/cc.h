@@ -77,11 +77,66 @@ __global__ void __launch_bounds__(BLOCK) k_cc_compress(int *p, int64_t n) {
   }
 }
 
-// roots of `k` sampled vertices (the host takes the most frequent one)
+// ---- choosing the skipped component on the device ------------------------
+// A fixed sample of vertex ids (SplitMix64 output mix over key + (i+1)*phi,
+// scaled to [0, n) by a multiply-high: the host generators' counter RNG),
+// then the most frequent root among the sampled vertices. No host read, so
+// the whole CC is one asynchronous launch sequence.
+constexpr int CC_MAX_SAMPLES = 4096;
+constexpr int CC_MODE_BLOCK = 1024;
+constexpr uint64_t CC_SAMPLE_KEY = 0xCC5A3D1Eull;
+
+__device__ __forceinline__ uint64_t cc_mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
 __global__ void __launch_bounds__(BLOCK)
-    k_cc_sample_roots(const int *p, const int32_t *ids, int k, int *out) {
+    k_cc_sample_ids(int32_t *ids, int k, int64_t n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < k) out[i] = p[ids[i]];
+  if (i >= k) return;
+  const uint64_t key = cc_mix64(cc_mix64(CC_SAMPLE_KEY));
+  const uint64_t r = cc_mix64(key + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull);
+  ids[i] = (int32_t)__umul64hi(r, (uint64_t)n);
+}
+
+// *skip <- the most frequent value of p[ids[0..k)] (ties: the smallest).
+// One workgroup of CC_MODE_BLOCK threads, 1 <= k <= CC_MAX_SAMPLES: bitonic
+// sort in LDS, then each run start measures its run and a 64-bit max of
+// (run length, INT_MAX - value) picks the winner.
+__global__ void __launch_bounds__(CC_MODE_BLOCK)
+    k_cc_mode(const int *p, const int32_t *ids, int k, int *skip) {
+  __shared__ int s[CC_MAX_SAMPLES];
+  __shared__ unsigned long long best;
+  int k2 = 1;
+  while (k2 < k) k2 <<= 1;
+  for (int i = threadIdx.x; i < k2; i += blockDim.x)
+    s[i] = i < k ? cc_get(p, ids[i]) : INT_MAX;
+  if (threadIdx.x == 0) best = 0;
+  for (int size = 2; size <= k2; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      __syncthreads();
+      for (int t = threadIdx.x; t < (k2 >> 1); t += blockDim.x) {
+        const int i = 2 * t - (t & (stride - 1)), j = i + stride;
+        const int a = s[i], b = s[j];
+        if ((a > b) == ((i & size) == 0)) {
+          s[i] = b;
+          s[j] = a;
+        }
+      }
+    }
+  __syncthreads();
+  for (int i = threadIdx.x; i < k; i += blockDim.x) {
+    if (i > 0 && s[i] == s[i - 1]) continue;
+    int j = i + 1;
+    while (j < k && s[j] == s[i]) ++j;
+    const unsigned long long key =
+        ((unsigned long long)(j - i) << 32) | (unsigned)(INT_MAX - s[i]);
+    atomicMax(&best, key);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *skip = INT_MAX - (int)(unsigned)(best & 0xFFFFFFFFull);
 }
 
 // Instance of v: the largest i with offsets[i] <= v (offsets ascending,
@@ -101,14 +156,16 @@ __device__ __forceinline__ int64_t cc_instance(const int64_t *off, int64_t count
 
 // Phase 3. skip_root >= 0: S = {v : p[v] == skip_root} (one graph);
 // skip_root == -1: S = {v : p[v] == first vertex of v's instance} with
-// `off` the count + 1 instance offsets; skip_root == -2: no skipping.
+// `off` the count + 1 instance offsets; skip_root == -2: no skipping. A
+// non-null skip_dev replaces skip_root by *skip_dev (written by k_cc_mode).
 // Edges [from, deg) of each row are visited (the sampled ones are done).
 // One wavefront per 64 vertices; rows longer than CC_WIDE are walked by
 // the whole wavefront, one at a time.
 __global__ void __launch_bounds__(BLOCK)
     k_cc_rest(const int64_t *__restrict__ rows, const int32_t *__restrict__ cols,
-              int64_t n, int from, int skip_root, const int64_t *__restrict__ off,
-              int64_t count, int *p) {
+              int64_t n, int from, int skip_arg, const int *skip_dev,
+              const int64_t *__restrict__ off, int64_t count, int *p) {
+  const int skip_root = skip_dev ? *skip_dev : skip_arg;
   const int lane = lane_id();
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> LOG_WAVE;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> LOG_WAVE;

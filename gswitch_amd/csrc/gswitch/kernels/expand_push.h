@@ -36,10 +36,11 @@ namespace gsw {
 // apart, so each has a cache line of its own: atomics from every workgroup
 // onto one line serialise on MI355X). Every thread of the workgroup calls
 // it; a workgroup that leaves early leaves as a whole and counts nothing.
+template <int NT = BLOCK>
 __device__ __forceinline__ void add_work(unsigned long long *work,
                                          unsigned long long edges,
                                          unsigned long long verts) {
-  __shared__ unsigned long long sh[2][BLOCK / WAVE];
+  __shared__ unsigned long long sh[2][NT / WAVE];
   edges = wave_sum_any(edges);
   verts = wave_sum_any(verts);
   if (lane_id() == 0) {

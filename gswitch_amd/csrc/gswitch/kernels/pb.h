@@ -1,0 +1,357 @@
+// Propagation-blocked from-all pull (layout: pb_t in data_structures/graph.h).
+//
+// A from-all pull (every vertex sends emit(u) along each out-edge, every
+// target combines what arrives) gathers one message per edge at a random
+// source. On MI355X that gather is bound by the L2 request rate (about one
+// request per edge, whether the line then hits L2 or not). Propagation
+// blocking (Beamer, Asanovic, Patterson, "Reducing PageRank Communication
+// via Propagation Blocking", IPDPS 2017) turns both random sides into LDS
+// accesses at the price of one write and one read of a per-edge message:
+//   phase 1  per source chunk (S consecutive ids): the chunk's messages are
+//            computed once into LDS; for every tile (b, c) of the chunk the
+//            edges' 16-bit source offsets are streamed and each edge's
+//            message is stored at the edge's slot (consecutive lanes,
+//            consecutive slots);
+//   phase 2  per target bin: the bin's slots are one contiguous range
+//            (bin-major layout); messages and 16-bit target offsets are
+//            streamed and combined with compAtomic into LDS accumulators
+//            that start from the targets' WA, which are then written back
+//            (the workgroup owns its targets). A target with more in-edges
+//            than a piece holds is a bin of its own, split over several
+//            workgroups that each reduce their part in registers and add it
+//            with one compAtomic on the target.
+// HBM bytes per expand: 2 (source offset) + 2 * sizeof(msg) (store, load) +
+// 2 (target offset) per edge, plus the messages and WA per vertex; no
+// random global access. Messages are stored as F::msg_t when the functor
+// declares one (its emit() result must convert to it exactly), else as WA.
+//
+// The layout is built once per graph on the device: bins by a host pass
+// over the in-degree offsets, tile counts and slots by two passes over the
+// in-CSR (one wavefront per row, one atomic per run of equal tiles), the
+// pieces on the host from the tile offsets.
+#pragma once
+#include <cstdlib>
+#include <type_traits>
+#include <vector>
+
+#include "../data_structures/graph.h"
+#include "expand_pull.h"
+#include "scan.h"
+
+namespace gsw {
+
+constexpr int PB_BLOCK = 512;
+constexpr int PB_LDS_BYTES = 64 << 10;  // per phase: messages / accumulators
+constexpr int PB_UNROLL = 4;            // slots a lane keeps in flight
+
+template <typename F, typename = void> struct pb_msg {
+  using type = typename F::wa_t;
+};
+template <typename F> struct pb_msg<F, std::void_t<typename F::msg_t>> {
+  using type = typename F::msg_t;
+};
+
+// bin of target v: the last b with bin_lo[b] <= v
+__device__ __forceinline__ int64_t pb_bin_of(const int32_t *bin_lo, int64_t B,
+                                             int64_t v) {
+  int64_t lo = 0, hi = B;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (bin_lo[mid] <= v)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// One wavefront per in-CSR row v (edges sorted by source, so tiles are
+// non-decreasing along the row): count (cursor == nullptr) adds each run of
+// equal tiles to cnt[tile]; fill (cursor != nullptr) claims slots for a run
+// with one atomic and writes the edges' offsets.
+__global__ void __launch_bounds__(BLOCK)
+    k_pb_tiles(const int64_t *__restrict__ rows, const int32_t *__restrict__ cols,
+               int64_t n, const int32_t *__restrict__ bin_lo, int64_t B, int S,
+               int64_t C, unsigned long long *cnt, int32_t *cursor,
+               const int64_t *__restrict__ tile_off, uint16_t *src, uint16_t *dst) {
+  const int lane = lane_id();
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> LOG_WAVE;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> LOG_WAVE;
+  for (int64_t v = wave; v < n; v += nwaves) {
+    const int64_t s = rows[v], e = rows[v + 1];
+    if (s == e) continue;
+    const int64_t b = pb_bin_of(bin_lo, B, v);
+    const int32_t vlo = bin_lo[b];
+    for (int64_t k0 = s; k0 < e; k0 += WAVE) {
+      const int64_t k = k0 + lane;
+      const bool ok = k < e;
+      const int u = ok ? cols[k] : 0;
+      const int64_t tile = ok ? b * C + u / S : -1;
+      const int64_t prev = __shfl_up(tile, 1, 64);
+      const bool head = ok && (lane == 0 || prev != tile);
+      const uint64_t hm = ballot(head);
+      const int nvalid = (int)MIN((int64_t)WAVE, e - k0);
+      const uint64_t later = hm & ~((2ull << lane) - 1);
+      const int run = head ? (later ? __builtin_ctzll(later) : nvalid) - lane : 0;
+      if (!cursor) {
+        if (head) atomicAdd(cnt + tile, (unsigned long long)run);
+      } else {
+        int base = 0;
+        if (head) base = atomicAdd(cursor + tile, run);
+        const int hl = 63 - __clzll((long long)(hm & ((2ull << lane) - 1)));
+        base = __shfl(base, hl, 64);
+        if (ok) {
+          const int64_t pos = tile_off[tile] + base + (lane - hl);
+          src[pos] = (uint16_t)(u - (u / S) * S);
+          dst[pos] = (uint16_t)(v - vlo);
+        }
+      }
+    }
+  }
+}
+
+// Phase 1: one workgroup per piece (chunk c, bins [b0, b1)).
+template <class Bd, typename MSG>
+__global__ void __launch_bounds__(PB_BLOCK) k_pb_scatter(Bd body, pb_t pb) {
+  constexpr int CAP = PB_LDS_BYTES / (int)sizeof(MSG);
+  __shared__ MSG s_msg[CAP];
+  const int32_t *pc = pb.p1 + 4 * (int64_t)blockIdx.x;
+  const int c = pc[0], b0 = pc[1], b1 = pc[2];
+  const int64_t lo = (int64_t)c * pb.S;
+  const int w = (int)MIN((int64_t)pb.S, body.g.nvertexs - lo);
+  for (int i = threadIdx.x; i < w; i += PB_BLOCK)
+    s_msg[i] = (MSG)body.msg((int)(lo + i), 0);
+  __syncthreads();
+  MSG *vals = (MSG *)pb.vals;
+  const int lane = lane_id(), wv = threadIdx.x >> LOG_WAVE;
+  for (int b = b0 + wv; b < b1; b += PB_BLOCK / WAVE) {
+    const int64_t t = (int64_t)c * pb.B + b;
+    const int64_t start = pb.cm_start[t];
+    const int len = pb.cm_len[t];
+    for (int k0 = 0; k0 < len; k0 += WAVE * PB_UNROLL) {
+      uint16_t si[PB_UNROLL];
+#pragma unroll
+      for (int j = 0; j < PB_UNROLL; ++j) {
+        const int k = k0 + j * WAVE + lane;
+        si[j] = k < len ? pb.src[start + k] : (uint16_t)0;
+      }
+#pragma unroll
+      for (int j = 0; j < PB_UNROLL; ++j) {
+        const int k = k0 + j * WAVE + lane;
+        if (k < len) vals[start + k] = s_msg[si[j]];
+      }
+    }
+  }
+}
+
+// Phase 2: one workgroup per piece (bin b, slots [e0, e1), whole bin or
+// part of a one-target bin).
+template <class Bd, typename MSG>
+__global__ void __launch_bounds__(PB_BLOCK) k_pb_gather(Bd body, pb_t pb) {
+  using WA = typename Bd::WA;
+  constexpr int CAP = PB_LDS_BYTES / (int)sizeof(WA);
+  __shared__ WA s_acc[CAP];
+  __shared__ int s_has[PB_BLOCK / WAVE];
+  const int64_t *q = pb.p2 + 4 * (int64_t)blockIdx.x;
+  const int64_t b = q[0], e0 = q[1], e1 = q[2];
+  const bool whole = q[3] != 0;
+  const int32_t lo = pb.bin_lo[b];
+  const int w = pb.bin_lo[b + 1] - lo;
+  const Status want = body.conf.want();
+  const MSG *vals = (const MSG *)pb.vals;
+  if (whole) {
+    for (int i = threadIdx.x; i < w; i += PB_BLOCK) s_acc[i] = *body.f.wa_of(lo + i);
+    __syncthreads();
+    for (int64_t k0 = e0; k0 < e1; k0 += (int64_t)PB_BLOCK * PB_UNROLL) {
+      MSG mv[PB_UNROLL];
+      uint16_t di[PB_UNROLL];
+#pragma unroll
+      for (int j = 0; j < PB_UNROLL; ++j) {
+        const int64_t k = k0 + (int64_t)j * PB_BLOCK + threadIdx.x;
+        const int64_t kk = k < e1 ? k : e1 - 1;
+        mv[j] = vals[kk];
+        di[j] = pb.dst[kk];
+      }
+#pragma unroll
+      for (int j = 0; j < PB_UNROLL; ++j)
+        if (k0 + (int64_t)j * PB_BLOCK + threadIdx.x < e1)
+          body.f.compAtomic(&s_acc[di[j]], (WA)mv[j], body.g);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < w; i += PB_BLOCK) {
+      const int v = lo + i;
+      if (body.as.fetch(v, want) >= 0) *body.f.wa_of(v) = s_acc[i];
+    }
+  } else {
+    // one target: reduce the part in registers, then across the workgroup
+    WA acc{};
+    bool has = false;
+    for (int64_t k = e0 + threadIdx.x; k < e1; k += PB_BLOCK) {
+      const WA m = (WA)vals[k];
+      if (has)
+        body.f.comp(&acc, m, body.g);
+      else
+        acc = m;
+      has = true;
+    }
+    bin_combine(body, acc, has, WAVE);
+    const int wv = threadIdx.x >> LOG_WAVE;
+    if (lane_id() == 0) {
+      s_acc[wv] = acc;
+      s_has[wv] = has;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      bool any = false;
+      WA tot{};
+      for (int i = 0; i < PB_BLOCK / WAVE; ++i) {
+        if (!s_has[i]) continue;
+        if (any)
+          body.f.comp(&tot, s_acc[i], body.g);
+        else
+          tot = s_acc[i];
+        any = true;
+      }
+      if (any && body.as.fetch(lo, want) >= 0)
+        body.f.compAtomic(body.f.wa_of(lo), tot, body.g);
+    }
+  }
+  add_work<PB_BLOCK>(body.as.dg_work, threadIdx.x == 0 ? (unsigned long long)(e1 - e0) : 0ull,
+                     0ull);
+}
+
+// Build the layout of `g` for chunks of S sources and bins of at most D
+// targets (no-op if built with the same widths).
+template <typename G> void pb_ensure(G &g, int S, int D) {
+  GSW_REQUIRE(g.pb, E_INTERNAL_CHECK, "graph has no PB handle");
+  GSW_REQUIRE(S >= 1 && S <= 65536 && D >= 1 && D <= 65536, E_INTERNAL_CHECK,
+              "PB widths must fit 16-bit offsets");
+  pb_t &pb = *g.pb;
+  if (pb.src && pb.S == S && pb.D == D) return;
+  pb.release();
+  const int64_t n = g.nvertexs, m = g.nedges;
+  GSW_REQUIRE(n >= 1, E_INVALID, "empty graph");
+  // in-degree offsets on the host: bins of <= D targets and <= ecap in-edges
+  std::vector<int64_t> rp((size_t)n + 1);
+  H_ERR(hipMemcpy(rp.data(), g.dgr_row_ptr, sizeof(int64_t) * (n + 1),
+                  hipMemcpyDeviceToHost));
+  int64_t ecap = MAX((int64_t)16384, m / 1024);
+  if (const char *e = std::getenv("GSW_PB_ECAP")) {
+    const long long x = std::atoll(e);
+    if (x > 0) ecap = x;
+  }
+  std::vector<int32_t> blo(1, 0);
+  for (int64_t v = 0; v < n;) {
+    int64_t w = v + 1;
+    while (w < n && w - v < D && rp[w + 1] - rp[v] <= ecap) ++w;
+    blo.push_back((int32_t)w);
+    v = w;
+  }
+  pb.S = S;
+  pb.D = D;
+  pb.m = m;
+  pb.ecap = ecap;
+  pb.B = (int64_t)blo.size() - 1;
+  pb.C = (n + S - 1) / S;
+  const int64_t T = pb.B * pb.C;
+  H_ERR(hipMalloc((void **)&pb.bin_lo, sizeof(int32_t) * (pb.B + 1)));
+  H_ERR(hipMemcpy(pb.bin_lo, blo.data(), sizeof(int32_t) * (pb.B + 1),
+                  hipMemcpyHostToDevice));
+  H_ERR(hipMalloc((void **)&pb.tile_off, sizeof(int64_t) * (T + 1)));
+  H_ERR(hipMalloc((void **)&pb.src, sizeof(uint16_t) * (m + 1)));
+  H_ERR(hipMalloc((void **)&pb.dst, sizeof(uint16_t) * (m + 1)));
+  int32_t *cursor = nullptr;
+  int64_t *bsum = nullptr;
+  H_ERR(hipMalloc((void **)&cursor, sizeof(int32_t) * (T + 1)));
+  H_ERR(hipMalloc((void **)&bsum, sizeof(int64_t) * ((T + 1 + SCAN_TILE - 1) / SCAN_TILE + 2)));
+  fill_async<int64_t>(pb.tile_off, 0, T + 1);
+  fill_async<int32_t>(cursor, 0, T + 1);
+  const int grid = grid_for(n * WAVE);
+  hipLaunchKernelGGL(k_pb_tiles, dim3(grid), dim3(BLOCK), 0, stream(),
+                     (const int64_t *)g.dgr_row_ptr, (const int32_t *)g.dgr_adj_list,
+                     n, (const int32_t *)pb.bin_lo, pb.B, S, pb.C,
+                     (unsigned long long *)pb.tile_off, nullptr, nullptr, nullptr,
+                     nullptr);
+  H_ERR(hipGetLastError());
+  scan_exclusive_inplace(pb.tile_off, T, bsum);
+  hipLaunchKernelGGL(k_pb_tiles, dim3(grid), dim3(BLOCK), 0, stream(),
+                     (const int64_t *)g.dgr_row_ptr, (const int32_t *)g.dgr_adj_list,
+                     n, (const int32_t *)pb.bin_lo, pb.B, S, pb.C, nullptr, cursor,
+                     (const int64_t *)pb.tile_off, pb.src, pb.dst);
+  H_ERR(hipGetLastError());
+  std::vector<int64_t> off((size_t)T + 1);
+  H_ERR(hipMemcpyAsync(off.data(), pb.tile_off, sizeof(int64_t) * (T + 1),
+                       hipMemcpyDeviceToHost, stream()));
+  sync();
+  hipFree(cursor);
+  hipFree(bsum);
+  GSW_REQUIRE(off[T] == m, E_INTERNAL_CHECK, "PB tiles do not cover the edges");
+  // chunk-major tile table and phase-1 pieces (runs of bins of <= ecap edges)
+  std::vector<int64_t> cs((size_t)T);
+  std::vector<int32_t> cl((size_t)T);
+  std::vector<int32_t> p1;
+  for (int64_t c = 0; c < pb.C; ++c) {
+    int64_t acc = 0, b0 = 0;
+    for (int64_t b = 0; b < pb.B; ++b) {
+      const int64_t t = b * pb.C + c;
+      const int64_t len = off[t + 1] - off[t];
+      cs[c * pb.B + b] = off[t];
+      cl[c * pb.B + b] = (int32_t)len;
+      if (acc > 0 && acc + len > ecap) {
+        p1.insert(p1.end(), {(int32_t)c, (int32_t)b0, (int32_t)b, 0});
+        b0 = b;
+        acc = 0;
+      }
+      acc += len;
+    }
+    if (acc > 0) p1.insert(p1.end(), {(int32_t)c, (int32_t)b0, (int32_t)pb.B, 0});
+  }
+  // phase-2 pieces: whole bins, or ecap-slot parts of a one-target bin
+  std::vector<int64_t> p2;
+  for (int64_t b = 0; b < pb.B; ++b) {
+    const int64_t e0 = off[b * pb.C], e1 = off[(b + 1) * pb.C];
+    if (blo[b + 1] - blo[b] == 1 && e1 - e0 > ecap) {
+      for (int64_t k = e0; k < e1; k += ecap)
+        p2.insert(p2.end(), {b, k, MIN(e1, k + ecap), (int64_t)0});
+    } else {
+      p2.insert(p2.end(), {b, e0, e1, (int64_t)1});
+    }
+  }
+  pb.n1 = (int64_t)p1.size() / 4;
+  pb.n2 = (int64_t)p2.size() / 4;
+  // the pieces cover every edge once: phase 1 by chunk, phase 2 in order
+  int64_t covered = 0;
+  for (int64_t i = 0; i < pb.n1; ++i)
+    for (int64_t b = p1[4 * i + 1]; b < p1[4 * i + 2]; ++b)
+      covered += cl[(int64_t)p1[4 * i] * pb.B + b];
+  GSW_REQUIRE(covered == m, E_INTERNAL_CHECK, "PB phase-1 pieces miss edges");
+  for (int64_t i = 0, next = 0; i < pb.n2; ++i) {
+    GSW_REQUIRE(p2[4 * i + 1] == next && p2[4 * i + 2] >= next &&
+                    blo[p2[4 * i] + 1] - blo[p2[4 * i]] <= D,
+                E_INTERNAL_CHECK, "PB phase-2 pieces out of order");
+    next = p2[4 * i + 2];
+    if (i + 1 == pb.n2)
+      GSW_REQUIRE(next == m, E_INTERNAL_CHECK, "PB phase-2 pieces miss edges");
+  }
+  H_ERR(hipMalloc((void **)&pb.cm_start, sizeof(int64_t) * (T + 1)));
+  H_ERR(hipMalloc((void **)&pb.cm_len, sizeof(int32_t) * (T + 1)));
+  H_ERR(hipMalloc((void **)&pb.p1, sizeof(int32_t) * (p1.size() + 4)));
+  H_ERR(hipMalloc((void **)&pb.p2, sizeof(int64_t) * (p2.size() + 4)));
+  H_ERR(hipMemcpy(pb.cm_start, cs.data(), sizeof(int64_t) * T, hipMemcpyHostToDevice));
+  H_ERR(hipMemcpy(pb.cm_len, cl.data(), sizeof(int32_t) * T, hipMemcpyHostToDevice));
+  if (!p1.empty())
+    H_ERR(hipMemcpy(pb.p1, p1.data(), sizeof(int32_t) * p1.size(), hipMemcpyHostToDevice));
+  H_ERR(hipMemcpy(pb.p2, p2.data(), sizeof(int64_t) * p2.size(), hipMemcpyHostToDevice));
+}
+
+// Message storage of `bytes` per edge (grown on demand).
+inline void pb_vals(pb_t &pb, size_t bytes) {
+  const size_t need = bytes * (size_t)(pb.m + 1);
+  if (pb.vals_bytes >= need) return;
+  if (pb.vals) H_ERR(hipFree(pb.vals));
+  pb.vals = nullptr;
+  H_ERR(hipMalloc(&pb.vals, need));
+  pb.vals_bytes = need;
+}
+
+}  // namespace gsw

@@ -120,7 +120,8 @@ typedef struct {
   int32_t twc_small;  /* TWC bins: degree < twc_small -> thread */
   int32_t twc_large;  /*           degree < twc_large -> wavefront */
   int32_t profile;    /* time every expand with hipEvents */
-  int32_t pull_kernel; /* from-all pulls: 0 merge-path walk, 1 segmented reduction */
+  int32_t pull_kernel; /* from-all pulls: 0 merge-path walk, 1 segmented
+                          reduction, 2 propagation blocking */
   int32_t capture;     /* gsw_bfs_multi: >0 = hipGraph-captured fused level loop,
                           2*capture levels per replay; 0 = per-level host loop */
   int32_t grid;        /* gsw_bfs_multi captured loop: workgroups per launch

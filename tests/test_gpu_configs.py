@@ -86,7 +86,7 @@ def test_c2_pagerank50_rmat22(gsw, orc, rmat22w):
     h, g = rmat22w
     say("C2: FP64 PageRank oracle, 50 iterations")
     ref = orc.pagerank(h.row_ptr, h.col, 0.85, 50)
-    for kw in [dict(), dict(pull_kernel=1),
+    for kw in [dict(), dict(pull_kernel=1), dict(pull_kernel=2),
                dict(configs="Pull-Bitmap-STRICT-Standalone"),
                dict(configs="Pull-Bitmap-TM-Standalone")]:
         say("C2 PageRank", kw)

@@ -184,7 +184,7 @@ def max_rel(a, b):
 
 @pytest.mark.parametrize("variant", [None] + ["%s-Bitmap-%s-Standalone" % (d, l)
                                               for d in DIRS for l in LBS + ["TWOD"]])
-@pytest.mark.parametrize("pull_kernel", [0, 1])
+@pytest.mark.parametrize("pull_kernel", [0, 1, 2])
 def test_pagerank(gsw, orc, gset, variant, pull_kernel):
     for gname in ["rmat12", "ba", "grid"]:
         h, g = gset[gname]
@@ -210,6 +210,30 @@ def test_pagerank_twod_slices(gsw, orc, monkeypatch, slice_kb):
                         pull_kernel=pull_kernel)
         g.close()
         assert max_rel(pr, ref) <= PR_TOL, (slice_kb, pull_kernel, max_rel(pr, ref))
+
+
+@pytest.mark.parametrize("ecap", [0, 300])
+def test_pagerank_propagation_blocking(gsw, orc, monkeypatch, ecap):
+    """Propagation-blocked pulls (pull_kernel 2). R-MAT 15 has 32 768
+    vertices, so its sources fall in two 16 384-id chunks and its targets
+    in bins of at most 8 192; with pieces of 300 slots (GSW_PB_ECAP) the
+    hubs (degrees in the thousands) become one-target bins split over
+    several workgroups that add their parts atomically, and chunks are cut
+    into several phase-1 pieces. Also a grid (no hubs) and a graph with
+    isolated vertices."""
+    if ecap:
+        monkeypatch.setenv("GSW_PB_ECAP", str(ecap))
+    h = gsw.rmat(15, 16, seed=7)
+    deg = np.diff(h.row_ptr)
+    assert deg.max() > 3 * 300 and (deg == 0).any()
+    for hh in (h, gsw.grid2d(70, 90, 0.2, 0.01, 5)):
+        ref = orc.pagerank(hh.row_ptr, hh.col, 0.85, 30)
+        g = gsw.Graph(hh)  # the layout is built on the graph's first such pull
+        for rep in range(2):
+            pr = g.pagerank(0.85, 30, configs="Pull-Bitmap-TWOD-Standalone",
+                            pull_kernel=2)
+            assert max_rel(pr, ref) <= PR_TOL, (ecap, rep, max_rel(pr, ref))
+        g.close()
 
 
 def test_unsorted_rows_are_sorted_on_upload(gsw, orc, monkeypatch):
@@ -261,8 +285,9 @@ def test_graphs_without_edges(gsw, orc, n):
     assert np.array_equal(g.cc(), np.arange(n, dtype=np.int32))
     pr = g.pagerank(0.85, 10)
     assert max_rel(pr, orc.pagerank(rp, col, 0.85, 10)) <= PR_TOL
-    pr2 = g.pagerank(0.85, 10, configs="Pull-Bitmap-TWOD-Standalone", pull_kernel=1)
-    assert max_rel(pr2, orc.pagerank(rp, col, 0.85, 10)) <= PR_TOL
+    for pk in (1, 2):
+        pr2 = g.pagerank(0.85, 10, configs="Pull-Bitmap-TWOD-Standalone", pull_kernel=pk)
+        assert max_rel(pr2, orc.pagerank(rp, col, 0.85, 10)) <= PR_TOL, pk
     lv, nsp, dep = g.bc(0)
     rl, rn, rd = orc.bc(rp, col, 0)
     assert np.array_equal(lv, rl) and np.allclose(nsp, rn) and np.allclose(dep, rd)
