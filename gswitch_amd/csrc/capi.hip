@@ -308,6 +308,8 @@ void upload_graph(gsw_graph *G) {
     G->dgw.dgr_row_ptr = G->dg.dgr_row_ptr;
     G->dgw.dgr_adj_list = G->dg.dgr_adj_list;
     G->dgw.dgr_first = G->dg.dgr_first;
+    G->dgw.dg_odegree = G->dg.dg_odegree;
+    G->dgw.dgr_idegree = G->dg.dgr_idegree;
     G->dgw.twod = nullptr;  // TWOD slices are built for unweighted views only
     GSW_REQUIRE(!G->hg.directed, GSW_E_UNSUPPORTED,
                 "weighted directed graphs are not supported");

@@ -165,6 +165,12 @@ template <typename E> struct device_graph_t<CSR, E> {
   // consecutive targets is one coalesced load instead of one scattered
   // adjacency load per lane
   int32_t *dgr_first = nullptr;
+  // out- and in-degrees as int32 (the reference's dg_odegree,
+  // graph.cuh:44-175): per-vertex passes that need only a degree read 4
+  // bytes instead of an 8-byte offset pair; the in-degrees alias the
+  // out-degrees of an undirected graph
+  int32_t *dg_odegree = nullptr;
+  int32_t *dgr_idegree = nullptr;
   // host-side handle of the TWOD slices (kernels/twod.h builds them on first
   // use); shared by every copy of this struct, owned by the one that built it
   twod_t *twod = nullptr;
@@ -219,6 +225,11 @@ template <typename E> struct device_graph_t<CSR, E> {
                       hipMemcpyHostToDevice));
       bytes += sizeof(int32_t) * (nvertexs + 1);
     }
+    bytes += upload_degrees(hg.row_ptr, &dg_odegree);
+    if (directed)
+      bytes += upload_degrees(hg.r_row_ptr, &dgr_idegree);
+    else
+      dgr_idegree = dg_odegree;
     return bytes;
   }
 
@@ -249,8 +260,11 @@ template <typename E> struct device_graph_t<CSR, E> {
     hipFree(dg_row_ptr);
     hipFree(dg_adj_list);
     hipFree(dgr_first);
+    if (directed) hipFree(dgr_idegree);
+    hipFree(dg_odegree);
     if (has_edata) hipFree(dg_edgedata);
     dgr_first = nullptr;
+    dg_odegree = dgr_idegree = nullptr;
     dg_row_ptr = dgr_row_ptr = nullptr;
     dg_adj_list = dgr_adj_list = nullptr;
     dg_edgedata = dgr_edgedata = nullptr;
@@ -260,10 +274,10 @@ template <typename E> struct device_graph_t<CSR, E> {
     return dg_row_ptr[v];
   }
   __device__ __forceinline__ int get_out_degree(int v) const {
-    return (int)(dg_row_ptr[v + 1] - dg_row_ptr[v]);
+    return dg_odegree[v];
   }
   __device__ __forceinline__ int get_in_degree(int v) const {
-    return (int)(dgr_row_ptr[v + 1] - dgr_row_ptr[v]);
+    return dgr_idegree[v];
   }
   __device__ __forceinline__ int64_t get_in_start_pos(int v) const {
     return dgr_row_ptr[v];
@@ -286,6 +300,18 @@ template <typename E> struct device_graph_t<CSR, E> {
   int *level_ptr = nullptr;
 
  private:
+  static int64_t upload_degrees(const std::vector<int64_t> &rp, int32_t **dst) {
+    const size_t n = rp.empty() ? 0 : rp.size() - 1;
+    std::vector<int32_t> d(n + 1, 0);
+    for (size_t v = 0; v < n; ++v) {
+      const int64_t x = rp[v + 1] - rp[v];
+      GSW_REQUIRE(x <= INT32_MAX, gsw::E_INVALID, "a vertex degree exceeds int32");
+      d[v] = (int32_t)x;
+    }
+    H_ERR(hipMalloc((void **)dst, sizeof(int32_t) * (n + 1)));
+    H_ERR(hipMemcpy(*dst, d.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice));
+    return (int64_t)(sizeof(int32_t) * n);
+  }
   int64_t upload_w(const std::vector<uint32_t> &w, E **dst) {
     std::vector<E> tmp(w.size());
     for (size_t i = 0; i < w.size(); ++i) tmp[i] = (E)w[i];

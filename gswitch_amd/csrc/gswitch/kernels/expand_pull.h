@@ -167,21 +167,31 @@ __global__ void __launch_bounds__(BLOCK)
     bool found = false;
     if (v < n && !((word >> lane) & 1ull)) {
       ++nv;
-      const int64_t s = body.rows()[v], e = body.rows()[v + 1];
+      // the row's offset is read only when an edge past the first copy is
+      // walked (or the functor reads edge data): a target whose first
+      // in-neighbour is in the frontier costs its degree and that copy
+      const int64_t id = body.g.get_in_degree((int)v);
+      int64_t s = -1;
+      auto row = [&]() {
+        if (s < 0) s = body.rows()[v];
+        return s;
+      };
+      constexpr bool edata = decltype(body.g)::has_edata;
       WA acc;
-      int64_t j = s, end = e;
-      if (body.conf.pull_first && j < e) {
+      int64_t k = 0, end = id;  // positions within the row
+      if (body.conf.pull_first && k < end) {
         const int u = body.first_col((int)v);
         if (body.source_ok(u)) {
-          const WA m = body.msg(u, j);
+          const WA m = body.msg(u, edata ? row() : 0);
           acc = *body.f.wa_of(v);
           found = true;
           body.f.comp(&acc, m, body.g);
-          if (prune) end = j + 1;
+          if (prune) end = k + 1;
         }
-        ++j;
+        ++k;
       }
-      for (; j < end; ++j) {
+      for (; k < end; ++k) {
+        const int64_t j = row() + k;
         const int u = body.cols()[j];
         if (!body.source_ok(u)) continue;
         const WA m = body.msg(u, j);
@@ -191,11 +201,11 @@ __global__ void __launch_bounds__(BLOCK)
         }
         body.f.comp(&acc, m, body.g);
         if (prune) {
-          ++j;
+          ++k;
           break;
         }
       }
-      ne += j - s;
+      ne += k;
       if (found) {
         *body.f.wa_of(v) = acc;
         const long long od = body.g.get_out_degree((int)v);
@@ -203,7 +213,6 @@ __global__ void __launch_bounds__(BLOCK)
         pushw += od;
         maxa = od > maxa ? od : maxa;
       } else {
-        const long long id = e - s;
         ++inact;
         pullw += id;
         maxi = id > maxi ? id : maxi;
