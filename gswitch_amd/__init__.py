@@ -98,6 +98,7 @@ _SIGS = {
     "gsw_sssp": (C.c_int32, [_P, C.c_int32, C.POINTER(RunOpts), _P,
                              C.POINTER(RunStats)]),
     "gsw_cc": (C.c_int32, [_P, C.POINTER(RunOpts), _P, C.POINTER(RunStats)]),
+    "gsw_cc_noalloc": (C.c_int32, [_P, C.c_int32, _P, _P]),
     "gsw_pagerank": (C.c_int32, [_P, C.c_float, C.c_int32, C.c_float,
                                  C.POINTER(RunOpts), _P, C.POINTER(RunStats)]),
     "gsw_gc": (C.c_int32, [_P, C.POINTER(RunOpts), _P, C.POINTER(RunStats)]),
@@ -449,6 +450,13 @@ class Graph:
         _check(_lib.gsw_cc(self._h, C.byref(o), _ptr(lab), C.byref(st)))
         self.last_stats = st
         return lab
+
+    def cc_noalloc(self, labels_ptr, samples=1024, stream=None):
+        """Allocation-free CC into device memory at `labels_ptr` (n int32),
+        queued on `stream` (a hipStream_t as an int; None: the null
+        stream); returns before the kernels finish."""
+        _check(_lib.gsw_cc_noalloc(self._h, int(samples), C.c_void_p(int(labels_ptr)),
+                                   C.c_void_p(stream or 0)))
 
     def pagerank(self, damping=0.85, iters=50, eps=0.0, configs=None, out=True, **kw):
         o = self._opts(configs, kw)

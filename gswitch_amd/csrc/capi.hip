@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 
 #include "apps.h"
@@ -2351,3 +2352,57 @@ gsw_status gsw_scan_exclusive_i64(const int64_t *in, int64_t n, int64_t *out) {
 
 }  // extern "C"
 
+// ---- allocation-free connected components -------------------------------
+namespace {
+
+// Scratch of gsw_cc_noalloc: the sampled vertex ids, then the chosen root.
+// Every GPU holds its own copy for the life of the process. In host code
+// the name is only a stand-in for the symbol: kernels get the device address
+// that hipGetSymbolAddress returns for the current device.
+constexpr int NOALLOC_SCRATCH_INTS = gsw::CC_MAX_SAMPLES + 64;
+static __device__ __align__(256) int32_t scratch[NOALLOC_SCRATCH_INTS];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+int32_t *g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// address of `scratch` on the current device, looked up once per device
+int32_t *noalloc_scratch() {
+  int dev = 0;
+  H_ERR(hipGetDevice(&dev));
+  GSW_REQUIRE(dev >= 0 && dev < NOALLOC_MAX_DEVICES, GSW_E_UNSUPPORTED,
+              "device id beyond the scratch address table");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void *p = nullptr;
+    H_ERR(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    GSW_REQUIRE(p != nullptr, GSW_E_HIP, "no device address for the CC scratch");
+    g_noalloc_ptr[dev] = (int32_t *)p;
+  }
+  return g_noalloc_ptr[dev];
+}
+
+}  // namespace
+
+extern "C" {
+
+gsw_status gsw_cc_noalloc(const gsw_graph *G, int32_t samples, int32_t *labels,
+                          void *stream) {
+  return guarded([&] {
+    GSW_REQUIRE(G && labels, GSW_E_INVALID, "invalid arguments");
+    GSW_REQUIRE(samples >= 1 && samples <= gsw::CC_MAX_SAMPLES, GSW_E_INVALID,
+                "samples must be 1.." + std::to_string(gsw::CC_MAX_SAMPLES) +
+                    " (the size of the static scratch)");
+    hipPointerAttribute_t attr;
+    const hipError_t pe = hipPointerGetAttributes(&attr, labels);
+    if (pe != hipSuccess) (void)hipGetLastError();  // clear the failed query
+    GSW_REQUIRE(pe == hipSuccess &&
+                    (attr.type == hipMemoryTypeDevice ||
+                     attr.type == hipMemoryTypeManaged),
+                GSW_E_INVALID, "labels must be device memory");
+    int32_t *s = noalloc_scratch();
+    cc_sampled_launch(G, labels, s, samples, (hipStream_t)stream);
+  });
+}
+
+}  // extern "C"

@@ -175,6 +175,21 @@ gsw_status gsw_sssp(gsw_graph *g, int32_t root, const gsw_run_opts *opts,
  * the level loop, honouring the pinned choices. */
 gsw_status gsw_cc(gsw_graph *g, const gsw_run_opts *opts, int32_t *labels_out,
                   gsw_run_stats *stats);
+/* Allocation-free form of gsw_cc's default path (sampled hooking): no
+ * counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops). `labels` is DEVICE
+ * memory of n int32 owned by the caller and receives the labels;
+ * `stream` is a hipStream_t (NULL: the null stream). The call only queues
+ * kernels and returns; it allocates nothing, reads nothing back and
+ * synchronises nothing. `samples` (1..4096; gsw_cc uses 1024) vertices
+ * are sampled to pick the component whose inner edges are skipped. The
+ * sample and the chosen root live in one static device array per GPU
+ * (GSW_CC_NOALLOC_SCRATCH_BYTES, held until the process ends) that every
+ * call on that GPU shares: calls that overlap on one GPU race. */
+#define GSW_CC_NOALLOC_MAX_SAMPLES 4096
+#define GSW_CC_NOALLOC_SCRATCH_BYTES (4 * (4096 + 64))
+gsw_status gsw_cc_noalloc(const gsw_graph *g, int32_t samples, int32_t *labels,
+                          void *stream);
 /* PageRank, un-normalised: pr_0 = 1-d, pr_{k+1}(v) = 1-d + d*sum pr_k(u)/deg(u),
  * `iters` iterations; eps > 0 additionally deactivates vertices whose
  * relative change is below eps as the reference does. Replaces
