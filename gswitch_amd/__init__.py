@@ -119,6 +119,7 @@ _SIGS = {
                                      C.POINTER(C.c_double)]),
     "gsw_selector_load": (C.c_int32, [_P, C.c_int32]),
     "gsw_selector_load_app": (C.c_int32, [C.c_int32, _P, C.c_int32]),
+    "gsw_selector_load_tree": (C.c_int32, [C.c_int32, C.c_int32, _P, _P, _P, _P, _P]),
     "gsw_mlp_features": (C.c_int32, [_P, _P]),
     "gsw_mlp_forward": (C.c_int32, [_P, _P, C.c_int64, _P]),
     "gsw_mlp_select": (C.c_int32, [_P, _P, C.c_int64, C.c_uint32, _P]),
@@ -322,6 +323,24 @@ def selector_load(w, app="bfs"):
         raise ValueError("no learned selector for %r" % (app,))
     w = _f32(w, MLP_NPARAM)
     _check(_lib.gsw_selector_load_app(SELECTOR_APPS[app], _ptr(w), MLP_NPARAM))
+
+
+def selector_load_tree(tree, app="bfs"):
+    """Install an application's class tree, used by its runs with
+    selector=2. `tree` is a dict of equal-length node arrays feat, thr,
+    left, right, cls (gswitch_amd.learn.fit_tree / load_tree)."""
+    if app not in SELECTOR_APPS:
+        raise ValueError("no class-tree selector for %r" % (app,))
+    feat = np.ascontiguousarray(tree["feat"], np.int32)
+    n = feat.size
+    thr = np.ascontiguousarray(tree["thr"], np.float32)
+    left = np.ascontiguousarray(tree["left"], np.int32)
+    right = np.ascontiguousarray(tree["right"], np.int32)
+    cls = np.ascontiguousarray(tree["cls"], np.int32)
+    if not (thr.size == left.size == right.size == cls.size == n):
+        raise ValueError("tree arrays differ in length")
+    _check(_lib.gsw_selector_load_tree(SELECTOR_APPS[app], n, _ptr(feat), _ptr(thr),
+                                       _ptr(left), _ptr(right), _ptr(cls)))
 
 
 def mlp_features(counts):

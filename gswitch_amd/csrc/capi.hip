@@ -418,12 +418,23 @@ double elapsed(gsw_graph *G) {
 // application's own per-level timings
 float *g_mlp_dev[GSW_APPS] = {};
 std::vector<float> g_mlp_host[GSW_APPS];
+// class trees per application (selector 2)
+gsw_tree::table_t g_tree[GSW_APPS];
 
 template <typename F>
 void maybe_learned(app_ctx_t<F> &c, const run_opts_t &o, const gsw_graph *G,
                    bool fromall, bool toall, int app) {
   if (o.selector == 0) return;
-  GSW_REQUIRE(o.selector == 1, GSW_E_INVALID, "unknown selector");
+  GSW_REQUIRE(o.selector == 1 || o.selector == 2, GSW_E_INVALID, "unknown selector");
+  if (o.selector == 2) {
+    GSW_REQUIRE(!g_tree[app].empty(), GSW_E_INVALID,
+                "class-tree selector requested but no tree is loaded for this "
+                "application");
+    enable_learned(c.en, nullptr, nullptr, G->attr, (double)G->hg.n,
+                   (double)G->hg.m, fromall, toall);
+    c.en.stats.tree = &g_tree[app];
+    return;
+  }
   GSW_REQUIRE(g_mlp_dev[app], GSW_E_INVALID,
               "learned selector requested but no weights are loaded for this "
               "application");
@@ -1845,6 +1856,33 @@ gsw_status gsw_selector_load_app(int32_t app, const float *w, int32_t nparam) {
     H_ERR(hipMemcpy(g_mlp_dev[app], w, sizeof(float) * gsw_mlp::NPARAM,
                     hipMemcpyHostToDevice));
     g_mlp_host[app].assign(w, w + nparam);
+  });
+}
+
+gsw_status gsw_selector_load_tree(int32_t app, int32_t nodes, const int32_t *feat,
+                                  const float *thr, const int32_t *left,
+                                  const int32_t *right, const int32_t *cls) {
+  return guarded([&] {
+    GSW_REQUIRE(app >= 0 && app < GSW_APPS, GSW_E_INVALID, "unknown application");
+    GSW_REQUIRE(nodes >= 1 && nodes <= 4096 && feat && thr && left && right && cls,
+                GSW_E_INVALID, "invalid tree");
+    for (int32_t i = 0; i < nodes; ++i) {
+      if (feat[i] >= 0) {
+        // children after their parent: every path ends at a leaf
+        GSW_REQUIRE(feat[i] < gsw_mlp::NIN && std::isfinite(thr[i]) && left[i] > i &&
+                        left[i] < nodes && right[i] > i && right[i] < nodes,
+                    GSW_E_INVALID, "invalid tree node " + std::to_string(i));
+      } else {
+        GSW_REQUIRE(cls[i] >= 0 && cls[i] < gsw_mlp::NOUT, GSW_E_INVALID,
+                    "invalid leaf class at node " + std::to_string(i));
+      }
+    }
+    gsw_tree::table_t &t = g_tree[app];
+    t.feat.assign(feat, feat + nodes);
+    t.thr.assign(thr, thr + nodes);
+    t.left.assign(left, left + nodes);
+    t.right.assign(right, right + nodes);
+    t.cls.assign(cls, cls + nodes);
   });
 }
 

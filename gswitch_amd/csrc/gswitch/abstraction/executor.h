@@ -251,7 +251,7 @@ struct executor_t {
       // the next queue's length: grid size for the next level, convergence
       const int out_slot = (as.queue.mode == Normal) ? (as.queue.cursor ^ 1)
                                                      : (as.queue.cursor + 1);
-      const bool learned = conf.policy_mlp && stat.mlp.w;
+      const bool learned = conf.policy_mlp && stat.learned();
       int nq = as.read_qsize_device(out_slot);
       as.known_active = nq;
       as.check_overflow_cached();

@@ -20,8 +20,9 @@ inline double gsw_now_ms() {
 }
 
 struct inspector_t {
-  // Selector inference for a level whose counters are known on the host
-  // (the first round): same features and argmin as the device path.
+  // Selector inference on the host for a level whose counters are known
+  // there: the first round (network or class tree), and every inspected
+  // level under the class tree. Same features as the device path.
   void mlp_host_select(const stat_t &stat, const feature_t &fets,
                        config_t &conf, int level) {
     const gsw_mlp::sel_dev_t &mlp = stat.mlp;
@@ -39,6 +40,12 @@ struct inspector_t {
     c.max_deg_active = (float)fets.cur_max_deg_active;
     float x[gsw_mlp::NIN], y[gsw_mlp::NOUT];
     gsw_mlp::features(c, x);
+    if (stat.tree) {
+      const int k = stat.tree->eval(x);
+      // a leaf class the application cannot run falls back to the rules
+      conf.mlp_class = (k >= 0 && k < gsw_mlp::NOUT && ((mlp.mask >> k) & 1u)) ? k : -1;
+      return;
+    }
     gsw_mlp::forward_host(stat.mlp_host_w, x, y);
     conf.mlp_class = gsw_mlp::argmin_valid(y, mlp.mask);
   }
@@ -67,7 +74,7 @@ struct inspector_t {
     }
     if (need && conf.conf_first_round) {
       set_first_fets(as, fets, conf, (double)g.nvertexs, (double)g.nedges);
-      if (conf.policy_mlp && stat.mlp.w) mlp_host_select(stat, fets, conf, g.level);
+      if (conf.policy_mlp && stat.learned()) mlp_host_select(stat, fets, conf, g.level);
       need = false;
     }
     if (need && pre) {
@@ -76,6 +83,7 @@ struct inspector_t {
       stat.collect();
       if (conf.policy_mlp && stat.mlp.w) conf.mlp_class = (int)stat.h_runtime_info[6];
       set_fets(as, stat, fets, conf, (double)g.nvertexs, (double)g.nedges);
+      if (conf.policy_mlp && stat.tree) mlp_host_select(stat, fets, conf, g.level);
       need = false;
     }
     if (need) {
@@ -95,6 +103,7 @@ struct inspector_t {
       stat.collect();
       if (d.w) conf.mlp_class = (int)stat.h_runtime_info[6];
       set_fets(as, stat, fets, conf, (double)g.nvertexs, (double)g.nedges);
+      if (conf.policy_mlp && stat.tree) mlp_host_select(stat, fets, conf, g.level);
     }
     fets.flatten();
   }

@@ -15,6 +15,7 @@
 // one-workgroup finish kernel (k_stat_finish) sums the slots into the
 // reduced counters and clears them, so the slots are zero between passes.
 #pragma once
+#include "../model/class_tree.h"
 #include "../model/mlp.h"
 #include "../utils/runtime.h"
 
@@ -38,6 +39,10 @@ struct stat_t {
   // weights for levels decided without a counting pass
   gsw_mlp::sel_dev_t mlp;
   const float *mlp_host_w = nullptr;
+  // class-tree selector (selector 2): evaluated on the host from the same
+  // 12 features; mlp.w stays null, so no counting pass runs the network
+  const gsw_tree::table_t *tree = nullptr;
+  bool learned() const { return mlp.w != nullptr || tree != nullptr; }
 
   void build(int _size = 6) {
     size = _size;
@@ -62,6 +67,7 @@ struct stat_t {
     level = 0;
     mlp = gsw_mlp::sel_dev_t{};
     mlp_host_w = nullptr;
+    tree = nullptr;
   }
   // partial slots the counting passes add into
   long long *slots() const { return dg_runtime_info + SLOT_BASE; }

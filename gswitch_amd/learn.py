@@ -12,11 +12,18 @@ network each).
    the rule trees' choices against the exhaustive optimum, and end-to-end
    time under the rule trees, the network, and fixed strategies.
 
+4. `fit-trees` (CPU): a class tree per application -- the reference's
+   offline-trained decision trees (src/model/*.h, README.md:13) retrained
+   for this engine -- fitted to the same per-level timings: every split
+   and every leaf class is chosen to minimise the summed time of the
+   levels it covers (selector 2 of the engine).
+
 Run as a module on a GPU machine (--app bfs | sssp, default bfs):
     python -m gswitch_amd.learn collect --app A --set train --out DIR
     python -m gswitch_amd.learn train --app A --data DIR/train.npz --out DIR
     python -m gswitch_amd.learn collect --app A --set eval --out DIR
     python -m gswitch_amd.learn evaluate --app A --data DIR/eval.npz --model M --out DIR
+    python -m gswitch_amd.learn fit-trees --app A --data DIR/train.npz --out DIR
 """
 import argparse
 import json
@@ -34,6 +41,10 @@ APPS = ("bfs", "sssp")
 
 def model_path(app="bfs"):
     return os.path.join(HERE, "model", "selector_%s.npz" % app)
+
+
+def tree_path(app="bfs"):
+    return os.path.join(HERE, "model", "trees_gfx950_%s.json" % app)
 
 
 MODEL_PATH = model_path("bfs")
@@ -209,6 +220,94 @@ def per_level_regret(ms, choice):
     return float(ms[idx, choice].sum() / ms.min(axis=1).sum())
 
 
+# ------------------------------------------------------------ class trees
+TREE_DEPTH = 3
+TREE_MIN_LEAF = 16
+
+
+def _leaf_class(ms, valid):
+    tot = np.where(valid, ms.sum(axis=0), np.inf)
+    c = int(np.argmin(tot))
+    return c, float(tot[c])
+
+
+def fit_tree(X, ms, valid=None, depth=TREE_DEPTH, min_leaf=TREE_MIN_LEAF):
+    """Greedy cost tree: a leaf takes the class with the least summed time
+    over its levels; a split (feature, threshold halfway between adjacent
+    distinct values, >= min_leaf levels a side) is kept when it lowers the
+    summed time of its two leaves. Returns the node table (fit order:
+    children after their parent)."""
+    X = np.asarray(X, np.float32)
+    ms = np.asarray(ms, np.float64)
+    if valid is None:
+        valid = np.isfinite(ms).all(axis=0)
+    nodes = []
+
+    def grow(idx, d):
+        i = len(nodes)
+        c, cost = _leaf_class(ms[idx], valid)
+        nodes.append([-1, 0.0, -1, -1, c])
+        if d == 0 or len(idx) < 2 * min_leaf:
+            return i
+        best = None
+        for f in range(X.shape[1]):
+            o = idx[np.argsort(X[idx, f], kind="stable")]
+            xs = X[o, f]
+            pre = np.cumsum(ms[o], axis=0)
+            k = np.arange(min_leaf, len(o) - min_leaf + 1)
+            k = k[xs[k - 1] < xs[k]]
+            if len(k) == 0:
+                continue
+            lc = np.where(valid, pre[k - 1], np.inf).min(axis=1)
+            rc = np.where(valid, pre[-1] - pre[k - 1], np.inf).min(axis=1)
+            j = int(np.argmin(lc + rc))
+            if best is None or lc[j] + rc[j] < best[0]:
+                thr = np.float32(0.5 * (np.float64(xs[k[j] - 1]) + np.float64(xs[k[j]])))
+                # the threshold must separate the two values in float32
+                if not (xs[k[j] - 1] <= thr < xs[k[j]]):
+                    thr = xs[k[j] - 1]
+                best = (float(lc[j] + rc[j]), f, float(thr))
+        if best is None or best[0] >= cost * (1 - 1e-9):
+            return i
+        _, f, thr = best
+        go_left = X[idx, f] <= np.float32(thr)
+        nodes[i][0], nodes[i][1] = f, thr
+        nodes[i][2] = grow(idx[go_left], d - 1)
+        nodes[i][3] = grow(idx[~go_left], d - 1)
+        return i
+
+    grow(np.arange(len(ms)), depth)
+    return {"feat": [int(r[0]) for r in nodes], "thr": [float(r[1]) for r in nodes],
+            "left": [int(r[2]) for r in nodes], "right": [int(r[3]) for r in nodes],
+            "cls": [int(r[4]) for r in nodes]}
+
+
+def tree_select(tree, X):
+    """Class of every row of X under a node table (float32 comparisons, as
+    the engine evaluates it)."""
+    X = np.asarray(X, np.float32)
+    feat, thr = tree["feat"], np.asarray(tree["thr"], np.float32)
+    out = np.empty(len(X), np.int32)
+    for r, x in enumerate(X):
+        i = 0
+        while feat[i] >= 0:
+            i = tree["left"][i] if x[feat[i]] <= thr[i] else tree["right"][i]
+        out[r] = tree["cls"][i]
+    return out
+
+
+def save_tree(path, tree, meta):
+    os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(dict(tree, meta=meta), f, indent=1)
+
+
+def load_tree(path):
+    with open(path) as f:
+        t = json.load(f)
+    return {k: t[k] for k in ("feat", "thr", "left", "right", "cls")}
+
+
 def save_model(path, w, meta):
     os.makedirs(os.path.dirname(path), exist_ok=True)
     np.savez(path, w=np.asarray(w, np.float32), meta=json.dumps(meta))
@@ -255,7 +354,7 @@ def end_to_end(specs, w, reps=3, log=print, app="bfs"):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=["collect", "train", "evaluate"])
+    ap.add_argument("cmd", choices=["collect", "train", "evaluate", "fit-trees"])
     ap.add_argument("--app", default="bfs", choices=APPS)
     ap.add_argument("--set", default="train", choices=["train", "eval"])
     ap.add_argument("--count", type=int, default=300)
@@ -270,6 +369,19 @@ def main(argv=None):
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
     log = lambda s: print(s, flush=True)
+    if a.cmd == "fit-trees":
+        path = a.data or os.path.join(HERE, "model", "selector_%s_train.npz" % a.app)
+        with np.load(path, allow_pickle=False) as z:
+            d = {k: z[k] for k in z.files}
+        tree = fit_tree(d["features"], d["ms"])
+        ch = tree_select(tree, d["features"])
+        log("train regret: tree %.3f rule %.3f (%d nodes)" % (
+            per_level_regret(d["ms"], ch), per_level_regret(d["ms"], d["rule"]),
+            len(tree["feat"])))
+        save_tree(os.path.join(a.out, "trees_gfx950_%s.json" % a.app), tree,
+                  dict(app=a.app, depth=TREE_DEPTH, min_leaf=TREE_MIN_LEAF,
+                       samples=int(len(d["ms"])), data=os.path.basename(path)))
+        return 0
     if a.cmd == "collect":
         specs = training_specs() if a.set == "train" else eval_specs()
         specs = specs[a.start:a.start + a.count]

@@ -127,7 +127,8 @@ typedef struct {
   int32_t grid;        /* gsw_bfs_multi captured loop: workgroups per launch
                           (0 = sized from the graph) */
   int32_t selector;    /* 0 rule-based trees (src/model/select_*.h);
-                          1 learned MLP (weights from gsw_selector_load) */
+                          1 learned MLP (weights from gsw_selector_load);
+                          2 class tree (gsw_selector_load_tree) */
 } gsw_run_opts;
 void gsw_run_opts_init(gsw_run_opts *o);
 
@@ -304,6 +305,17 @@ gsw_status gsw_tree_eval(const char *name, const double *fv, int32_t nfv,
 gsw_status gsw_selector_load_app(int32_t app, const float *w, int32_t nparam);
 /* = gsw_selector_load_app(GSW_APP_BFS, w, nparam). */
 gsw_status gsw_selector_load(const float *w, int32_t nparam);
+/* Install the class tree that `app`'s runs use when gsw_run_opts.selector
+ * == 2: the reference's offline-trained decision trees (src/model/*.h,
+ * trained from feature_t::record dumps, README.md:13) retrained for this
+ * engine, mapping the 12 inputs of gsw_mlp_features straight to a variant
+ * class. Node i is internal if feat[i] >= 0 (x[feat[i]] <= thr[i] -> left[i],
+ * else right[i]; children have larger indices) and a leaf of class cls[i]
+ * otherwise; node 0 is the root. A leaf class the application cannot run
+ * falls back to the rule trees for that level. */
+gsw_status gsw_selector_load_tree(int32_t app, int32_t nodes, const int32_t *feat,
+                                  const float *thr, const int32_t *left,
+                                  const int32_t *right, const int32_t *cls);
 /* The 12 inputs from counts[11] = {n, m, avg deg, std deg, Gini, level,
  * active, inactive, push workload, pull workload, max active degree}:
  * log2(1+n)/24, log2(1+m)/32, log2(1+avg)/8, log2(1+std/avg)/4, Gini,

@@ -103,3 +103,28 @@ def test_committed_training_sets(orc, app):
     r_mlp = learn.per_level_regret(d["ms"], choice)
     r_rule = learn.per_level_regret(d["ms"], d["rule"])
     assert r_mlp < r_rule and r_mlp < 1.2, (app, r_mlp, r_rule)
+
+
+def test_fit_tree_closed_forms():
+    """Cost trees on hand-made levels: a feature that separates two regimes
+    (class 1 cheap below 0.5, class 2 cheap above) gives one split at the
+    midpoint with those leaves; a single cheapest class gives one leaf;
+    tree_select follows the table with float32 comparisons."""
+    from gswitch_amd import learn
+    rng = np.random.default_rng(1)
+    X = rng.uniform(0, 1, (200, 12)).astype(np.float32)
+    ms = np.full((200, 30), 10.0)
+    lo = X[:, 4] <= 0.5
+    ms[lo, 1] = 1.0
+    ms[~lo, 2] = 1.0
+    t = learn.fit_tree(X, ms, depth=1, min_leaf=4)
+    assert t["feat"][0] == 4 and len(t["feat"]) == 3
+    xs = np.sort(X[:, 4])
+    k = np.searchsorted(xs, 0.5, side="right")
+    assert xs[k - 1] <= t["thr"][0] < xs[k]
+    assert t["cls"][t["left"][0]] == 1 and t["cls"][t["right"][0]] == 2
+    assert np.array_equal(learn.tree_select(t, X), np.where(lo, 1, 2))
+    ms2 = np.full((50, 30), 5.0)
+    ms2[:, 7] = 1.0
+    t2 = learn.fit_tree(X[:50], ms2)
+    assert t2["feat"] == [-1] and t2["cls"] == [7]

@@ -47,3 +47,33 @@ def test_learned_selector_beats_rule_on_held_out_levels(orc, evald):
         evald["app"], t_mlp / best, t_rule / best)
     assert t_mlp < t_rule, msg
     assert t_mlp / best < 1.5, msg
+
+
+def test_class_trees_beat_rule_on_held_out_levels(evald):
+    """The committed gfx950 class trees (selector 2), evaluated with the
+    same float32 comparisons as the engine: cheaper than the reference's
+    rule trees over the held-out levels."""
+    from gswitch_amd import learn
+    tree = learn.load_tree(learn.tree_path(evald["app"]))
+    ms = evald["ms"].astype(np.float64)
+    idx = np.arange(len(ms))
+    best = ms.min(axis=1).sum()
+    t_tree = ms[idx, learn.tree_select(tree, evald["features"])].sum()
+    t_rule = ms[idx, evald["rule"]].sum()
+    msg = "%s per-level cost / optimum: class tree %.3f, rule %.3f" % (
+        evald["app"], t_tree / best, t_rule / best)
+    assert t_tree < t_rule, msg
+
+
+def test_class_trees_refit_from_committed_data(evald):
+    """The committed trees are what fit_tree gives on the committed training
+    timings (the pipeline is deterministic)."""
+    from gswitch_amd import learn
+    path = os.path.join(os.path.dirname(learn.model_path()), "selector_%s_train.npz"
+                        % evald["app"])
+    with np.load(path, allow_pickle=False) as z:
+        tree = learn.fit_tree(z["features"], z["ms"])
+    committed = learn.load_tree(learn.tree_path(evald["app"]))
+    for k in ("feat", "left", "right", "cls"):
+        assert tree[k] == committed[k], k
+    np.testing.assert_allclose(tree["thr"], committed["thr"], rtol=0, atol=0)
