@@ -67,7 +67,7 @@ struct queue_t {
   // Append u to the output queue, staged per wavefront in LDS (no-op during
   // traceback): all 64 lanes call it; the entries reach the queue when the
   // stage fills and at flush_stage().
-  static constexpr int STAGE = 256;  // entries per wavefront
+  static constexpr int STAGE = 512;  // entries per wavefront
   using stage_t = gsw::wave_stage_t<STAGE>;
   __device__ __forceinline__ void push(bool want, int u, stage_t &st) {
     if (mode == Cached && traceback) return;
@@ -76,6 +76,11 @@ struct queue_t {
   __device__ __forceinline__ void flush_stage(stage_t &st) {
     if (mode == Cached && traceback) return;
     st.flush(output_size(), output_base(), output_capacity(), overflow);
+  }
+  // the workgroup's final flush (all threads call it; s_cnt: waves + 1 ints)
+  __device__ __forceinline__ void flush_stage_block(stage_t &st, int *s_cnt) {
+    if (mode == Cached && traceback) return;
+    st.flush_block(output_size(), output_base(), output_capacity(), overflow, s_cnt);
   }
 
   // ---- host side (a plain array: queue_t is copied byte-wise to kernels) --

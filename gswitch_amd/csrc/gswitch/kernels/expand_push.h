@@ -95,9 +95,12 @@ template <typename G, typename F, bool FUSED> struct push_body {
   }
 
   // fused pushes stage the next queue per wavefront in LDS (STAGE_LDS ints
-  // per workgroup); emit_q and emit_flush are called by all 64 lanes
+  // per workgroup: the wavefronts' buffers, then the workgroup's flush
+  // counts); emit_q is called by all 64 lanes of a wavefront, emit_flush by
+  // every thread of the workgroup at the end of the kernel
   using stage_t = queue_t::stage_t;
-  static constexpr int STAGE_LDS = FUSED ? (BLOCK / WAVE) * queue_t::STAGE : 1;
+  static constexpr int STAGE_WAVES = BLOCK / WAVE;
+  static constexpr int STAGE_LDS = FUSED ? STAGE_WAVES * queue_t::STAGE + STAGE_WAVES + 1 : 1;
   __device__ __forceinline__ stage_t stage(int *lds) const {
     stage_t st;
     st.buf = FUSED ? lds + (threadIdx.x >> LOG_WAVE) * queue_t::STAGE : lds;
@@ -106,8 +109,9 @@ template <typename G, typename F, bool FUSED> struct push_body {
   __device__ __forceinline__ void emit_q(bool q, int u, stage_t &st) {
     if constexpr (FUSED) as.queue.push(q, u, st);
   }
-  __device__ __forceinline__ void emit_flush(stage_t &st) {
-    if constexpr (FUSED) as.queue.flush_stage(st);
+  __device__ __forceinline__ void emit_flush(stage_t &st, int *lds) {
+    if constexpr (FUSED)
+      as.queue.flush_stage_block(st, lds + STAGE_WAVES * queue_t::STAGE);
   }
 };
 
@@ -144,7 +148,7 @@ __global__ void __launch_bounds__(BLOCK) k_push_tm(B body) {
       body.emit_q(q, u, st);
     }
   }
-  body.emit_flush(st);
+  body.emit_flush(st, s_stage);
   add_work(body.as.dg_work, ne, nv);
 }
 
@@ -197,7 +201,7 @@ __global__ void __launch_bounds__(BLOCK) k_push_wm(B body, int vpw) {
     }
     wave_sync();
   }
-  body.emit_flush(st);
+  body.emit_flush(st, s_stage);
   add_work(body.as.dg_work, ne, nv);
 }
 
@@ -245,7 +249,7 @@ __global__ void __launch_bounds__(BLOCK) k_push_cm(B body) {
     }
     __syncthreads();
   }
-  body.emit_flush(st);
+  body.emit_flush(st, s_stage);
   add_work(body.as.dg_work, ne, nv);
 }
 
@@ -376,7 +380,7 @@ __global__ void __launch_bounds__(BLOCK)
     __syncthreads();
     k0 = nk;
   }
-  body.emit_flush(st);
+  body.emit_flush(st, s_stage);
   add_work(body.as.dg_work, ne, 0);
 }
 
@@ -489,7 +493,7 @@ __global__ void __launch_bounds__(BLOCK) k_push_twc(B body) {
       body.emit_q(q, u, st);
     }
   }
-  body.emit_flush(st);
+  body.emit_flush(st, s_stage);
   add_work(body.as.dg_work, ne, nv);
 }
 

@@ -155,6 +155,36 @@ struct wave_stage_t {
     wave_sync();  // buffer reads done before it is refilled
     cnt = 0;
   }
+  // The workgroup's last flush: every wavefront's remaining entries go out
+  // behind one atomic for the whole workgroup (a launch ends with one
+  // atomic per workgroup instead of one per wavefront). Every thread of the
+  // workgroup calls it; `s_cnt` is LDS of (waves + 1) ints.
+  __device__ __forceinline__ void flush_block(int *counter, int *out, int64_t cap,
+                                              int *overflow, int *s_cnt) {
+    const int w = threadIdx.x >> LOG_WAVE, nw = blockDim.x >> LOG_WAVE;
+    wave_sync();  // every lane's buffer writes
+    if (lane_id() == 0) s_cnt[w] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int tot = 0;
+      for (int i = 0; i < nw; ++i) {
+        const int c = s_cnt[i];
+        s_cnt[i] = tot;
+        tot += c;
+      }
+      s_cnt[nw] = tot ? atomicAdd(counter, tot) : 0;
+    }
+    __syncthreads();
+    const int64_t base = (int64_t)s_cnt[nw] + s_cnt[w];
+    for (int i = lane_id(); i < cnt; i += 64) {
+      const int64_t slot = base + i;
+      if (slot < cap)
+        out[slot] = buf[i];
+      else
+        *overflow = 1;
+    }
+    cnt = 0;
+  }
   __device__ __forceinline__ void put(bool want, int u, int *counter, int *out,
                                       int64_t cap, int *overflow) {
     const uint64_t m = ballot(want);
