@@ -2,6 +2,7 @@
 #include "../../include/gswitch_amd.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -9,6 +10,7 @@
 
 #include "apps.h"
 #include "gswitch/kernels/cc.h"
+#include "gswitch/kernels/dobfs.h"
 #include "gswitch/kernels/msbfs.h"
 #include "host/host_graph.h"
 
@@ -220,6 +222,24 @@ struct gsw_graph {
   hipGraph_t cc_graph = nullptr;
   hipGraphExec_t cc_exec = nullptr;
   int64_t cc_key = -1;
+  // device-resident single-query BFS (kernels/dobfs.h): rotating frontier
+  // and visited words, heavy lists, control block and its captured loop
+  uint64_t *do_bits = nullptr;  // visited [nw] | F [3 nw]
+  int32_t *do_heavy = nullptr;
+  int64_t *do_hpre = nullptr;
+  long long *do_ctrl = nullptr;
+  long long *do_hctrl = nullptr;  // pinned copy of the first control words
+  hipGraph_t do_graph = nullptr;
+  hipGraphExec_t do_exec = nullptr;
+  double do_key[5] = {-1, -1, -1, -1, -1};
+
+  void drop_do_capture() {
+    if (do_exec) hipGraphExecDestroy(do_exec);
+    if (do_graph) hipGraphDestroy(do_graph);
+    do_exec = nullptr;
+    do_graph = nullptr;
+    for (double &x : do_key) x = -1;
+  }
 
   void drop_cc_capture() {
     if (cc_exec) hipGraphExecDestroy(cc_exec);
@@ -240,6 +260,12 @@ struct gsw_graph {
   void release() {
     drop_capture();
     drop_cc_capture();
+    drop_do_capture();
+    if (do_bits) hipFree(do_bits);
+    if (do_heavy) hipFree(do_heavy);
+    if (do_hpre) hipFree(do_hpre);
+    if (do_ctrl) hipFree(do_ctrl);
+    if (do_hctrl) hipHostFree(do_hctrl);
     if (cc_lab) hipFree(cc_lab);
     if (cc_ids) hipFree(cc_ids);
     if (inst_off) hipFree(inst_off);
@@ -793,6 +819,111 @@ double bfs_probe_level(gsw_graph *G, app_ctx_t<BFS> &c, int root, int cls,
   return wall;
 }
 
+// ---- device-resident single-query BFS (kernels/dobfs.h) --------------------
+double env_num(const char *name, double dflt) {
+  const char *e = std::getenv(name);
+  if (!e || !*e) return dflt;
+  char *end = nullptr;
+  const double x = std::strtod(e, &end);
+  return (end && *end == 0) ? x : dflt;
+}
+
+// The loop's view of graph G (its buffers are allocated on first use and
+// kept for the graph's lifetime, so a captured loop stays valid).
+gsw::dobfs_t do_view(gsw_graph *G) {
+  using namespace gsw;
+  const int64_t n = G->hg.n, nw = (n + 63) / 64;
+  if (!G->do_bits) {
+    H_ERR(hipMalloc((void **)&G->do_bits, sizeof(uint64_t) * 4 * (nw + 1)));
+    H_ERR(hipMalloc((void **)&G->do_heavy, sizeof(int32_t) * 2 * DO_HCAP));
+    H_ERR(hipMalloc((void **)&G->do_hpre, sizeof(int64_t) * (DO_HCAP + 1)));
+    H_ERR(hipMalloc((void **)&G->do_ctrl, sizeof(long long) * DO_TOTAL));
+    H_ERR(hipHostMalloc((void **)&G->do_hctrl, sizeof(long long) * DO_CTRL_WORDS));
+  }
+  dobfs_t d;
+  d.n = n;
+  d.nw = nw;
+  d.m = G->hg.m;
+  d.rows = G->dg.dg_row_ptr;
+  d.cols = G->dg.dg_adj_list;
+  d.rrows = G->dg.dgr_row_ptr;
+  d.rcols = G->dg.dgr_adj_list;
+  d.first = G->dg.dgr_first;
+  d.odeg = G->dg.dg_odegree;
+  d.ideg = G->dg.dgr_idegree;
+  d.lv = bfs_ctx(G).f.data.dg_wa;
+  d.vis = G->do_bits;
+  d.F = G->do_bits + nw;
+  d.heavy = G->do_heavy;
+  d.hpre = G->do_hpre;
+  d.ctrl = G->do_ctrl;
+  // switching thresholds and the heavy-vertex degree (defaults measured on
+  // MI355X; the variables exist for sweeps)
+  d.alpha = (float)env_num("GSW_DO_ALPHA", 14.0);
+  d.beta = (float)env_num("GSW_DO_BETA", 24.0);
+  d.heavy_deg = (int)env_num("GSW_DO_HEAVY", 4096.0);
+  // a wavefront scans 64 degrees of at most heavy_deg each in int32
+  GSW_REQUIRE(d.alpha > 0 && d.beta > 0 && d.heavy_deg >= 64 &&
+                  d.heavy_deg <= INT32_MAX / WAVE,
+              GSW_E_INVALID, "invalid direction-switch parameters");
+  return d;
+}
+
+// One BFS from `root` with the direction chosen on the device: a reset
+// launch, then `per_launch` captured levels (expand + decide) per hipGraph
+// replay and one read of the control block per replay. Returns the number
+// of non-empty frontiers expanded.
+int bfs_device_run(gsw_graph *G, int32_t root, int per_launch, int grid, int max_levels) {
+  using namespace gsw;
+  GSW_REQUIRE(per_launch >= 1 && per_launch <= 256 && grid >= 1 && grid <= MAX_GRID,
+              GSW_E_INVALID, "invalid device-loop shape");
+  const dobfs_t d = do_view(G);
+  hipStream_t st = stream();
+  H_ERR(hipEventRecord(G->ev0, st));
+  hipLaunchKernelGGL(k_do_reset, dim3(grid_for(MAX(d.n, d.nw))), dim3(BLOCK), 0, st, d,
+                     root);
+  H_ERR(hipGetLastError());
+  const double key[5] = {(double)per_launch, (double)grid, (double)d.heavy_deg,
+                         (double)d.alpha, (double)d.beta};
+  if (!G->do_exec || std::memcmp(key, G->do_key, sizeof(key)) != 0) {
+    G->drop_do_capture();
+    H_ERR(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int j = 0; j < per_launch; ++j) {
+        hipLaunchKernelGGL(k_do_expand, dim3(grid), dim3(BLOCK), 0, st, d);
+        hipLaunchKernelGGL(k_do_decide, dim3(1), dim3(DO_DECIDE_BLOCK), 0, st, d);
+        H_ERR(hipGetLastError());
+      }
+    } catch (...) {
+      hipGraph_t dead = nullptr;
+      (void)hipStreamEndCapture(st, &dead);
+      if (dead) (void)hipGraphDestroy(dead);
+      throw;
+    }
+    H_ERR(hipStreamEndCapture(st, &G->do_graph));
+    H_ERR(hipGraphInstantiate(&G->do_exec, G->do_graph, nullptr, nullptr, 0));
+    std::memcpy(G->do_key, key, sizeof(key));
+  }
+  const long long *h = G->do_hctrl;
+  bool halted = false;
+  for (int done = 0; done < max_levels && !halted; done += per_launch) {
+    H_ERR(hipGraphLaunch(G->do_exec, st));
+    H_ERR(hipMemcpyAsync(G->do_hctrl, G->do_ctrl, sizeof(long long) * DO_CTRL_WORDS,
+                         hipMemcpyDeviceToHost, st));
+    sync();
+    halted = h[DO_HALT] != 0;
+  }
+  H_ERR(hipEventRecord(G->ev1, st));
+  return halted ? (int)h[DO_LEVELS] : (int)h[DO_LEVEL];
+}
+
+// default workgroups of the device loop: one frontier word per wavefront
+// up to the full grid
+int default_do_grid(const gsw_graph *G) {
+  const int64_t nw = (G->hg.n + 63) / 64;
+  return gsw::grid_for(nw * gsw::WAVE);
+}
+
 // ---- connected components by sampled hooking (kernels/cc.h) --------------
 constexpr int CC_SAMPLE = 1024;
 
@@ -808,11 +939,12 @@ void cc_sample_phase(const gsw_graph *G, int32_t *lab, hipStream_t st) {
   const int64_t n = G->hg.n;
   const int grid = grid_for(n);
   hipLaunchKernelGGL(k_cc_init, dim3(grid), dim3(BLOCK), 0, st, lab, n);
-  for (int r = 0; r < 2; ++r)
+  for (int r = 0; r < 2; ++r) {
     hipLaunchKernelGGL(k_cc_sample, dim3(grid), dim3(BLOCK), 0, st,
                        (const int64_t *)G->dg.dg_row_ptr,
                        (const int32_t *)G->dg.dg_adj_list, n, r, lab);
-  hipLaunchKernelGGL(k_cc_compress, dim3(grid), dim3(BLOCK), 0, st, lab, n);
+    hipLaunchKernelGGL(k_cc_compress, dim3(grid), dim3(BLOCK), 0, st, lab, n);
+  }
   H_ERR(hipGetLastError());
 }
 
@@ -1181,10 +1313,21 @@ gsw_status gsw_bfs(gsw_graph *G, int32_t root, const gsw_run_opts *opts,
   return guarded([&] {
     GSW_REQUIRE(G, GSW_E_INVALID, "null graph");
     GSW_REQUIRE(root >= 0 && root < G->hg.n, GSW_E_INVALID, "root out of range");
-    const int levels = bfs_engine_run(G, opts, root, nullptr, 1, 0.0);
+    int levels = 0;
+    if (opts && opts->capture > 0) {
+      GSW_REQUIRE(!opts->configs && opts->selector == 0, GSW_E_UNSUPPORTED,
+                  "the device-resident BFS picks its own directions "
+                  "(no configs or selector)");
+      const run_opts_t o = to_opts(opts);
+      const int grid = opts->grid > 0 ? opts->grid : default_do_grid(G);
+      levels = bfs_device_run(G, root, opts->capture, grid, o.max_levels);
+      G->records.clear();
+    } else {
+      levels = bfs_engine_run(G, opts, root, nullptr, 1, 0.0);
+      collect_records(G, G->bfs->en.executor, G->bfs->as);
+    }
     const double ms = elapsed(G);
     auto &c = *G->bfs;
-    collect_records(G, c.en.executor, c.as);
     if (levels_out)
       H_ERR(hipMemcpy(levels_out, c.f.data.dg_wa, sizeof(int32_t) * G->hg.n,
                       hipMemcpyDeviceToHost));

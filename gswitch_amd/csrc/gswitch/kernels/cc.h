@@ -2,18 +2,19 @@
 // (Afforest: Sutton, Ben-Nun, Bar-Noy, "Optimizing Parallel Graph
 // Connectivity Computation via Subgraph Sampling", IPDPS 2018).
 //
-// p[v] is a parent pointer; roots have p[r] = r. link(u, v) hooks the larger
-// of the two roots under the smaller one with a compare-and-swap, so every
-// tree's root is the smallest id in it and, once a component is one tree,
-// its root is the component's minimum vertex id: the canonical label of the
-// reference's CC (application/cc.cu:34-66). link never waits on another
-// lane: each failed CAS means the root it tried moved to a smaller value, so
-// the loop ends.
+// p[v] is a parent pointer with p[v] <= v; roots have p[r] = r. link(u, v)
+// hooks the larger of the two roots under the smaller one with a
+// compare-and-swap, so every tree's root is the smallest id in it and, once a
+// component is one tree, its root is the component's minimum vertex id: the
+// canonical label of the reference's CC (application/cc.cu:34-66). link
+// never waits on another lane: each failed CAS means the root it tried moved
+// to a smaller value, so the loop ends.
 //
 // Phases (a fixed sequence of launches, no host decisions in between for
 // the batched form, so it can live in a captured graph):
 //  1. p = identity;
-//  2. sampling: every vertex links its first two neighbours, then compress;
+//  2. sampling: every vertex links its first neighbour, compress, links its
+//     second neighbour, compress;
 //  3. skip set S: vertices whose root is a chosen root -- the most frequent
 //     root of a vertex sample for one graph, or each instance's first vertex
 //     (its smallest id) for a disjoint batch. Vertices outside S link all
@@ -37,16 +38,38 @@ __device__ __forceinline__ int cc_get(const int *p, int64_t i) {
   return __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// join the trees of u and v
+// Root of v's tree with path halving on the way up (Jaiganesh and
+// Burtscher, "A High-Performance Connected Components Implementation for
+// GPUs", HPDC 2018): every vertex passed is pointed at its grandparent with a
+// plain store. Only non-roots are rewritten, always to an ancestor, and a
+// non-root never becomes a root again, so concurrent halving and hooking
+// (which changes roots only, by CAS) keep p a forest whose roots are the
+// smallest ids of their trees; trees stay shallow while links run.
+__device__ __forceinline__ int cc_find(int v, int *p) {
+  int cur = cc_get(p, v);
+  if (cur == v) return v;
+  int prev = v;
+  for (int next = cc_get(p, cur); next != cur; next = cc_get(p, cur)) {
+    __hip_atomic_store(p + prev, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    prev = cur;
+    cur = next;
+  }
+  return cur;
+}
+
+// join the trees of u and v: hook the larger root under the smaller one; a
+// failed CAS returns the root's new parent (smaller), and the hook is
+// retried from there
 __device__ __forceinline__ void cc_link(int u, int v, int *p) {
-  int a = cc_get(p, u), b = cc_get(p, v);
+  int a = cc_find(u, p), b = cc_find(v, p);
   while (a != b) {
     const int hi = a > b ? a : b, lo = a > b ? b : a;
-    const int ph = cc_get(p, hi);
-    if (ph == lo) return;
-    if (ph == hi && atomicCAS(p + hi, hi, lo) == hi) return;
-    a = cc_get(p, cc_get(p, hi));
-    b = cc_get(p, lo);
+    const int r = atomicCAS(p + hi, hi, lo);
+    if (r == hi) return;
+    // hi was hooked meanwhile (under a smaller id): continue from the
+    // current roots of both sides
+    a = cc_find(r, p);
+    b = cc_find(lo, p);
   }
 }
 
@@ -67,13 +90,22 @@ __global__ void __launch_bounds__(BLOCK)
   }
 }
 
-// p[v] <- root of v (no links run meanwhile, so p is a forest)
+// p[v] <- root of v (no links run meanwhile, so p is a forest whose roots
+// stay put). Every vertex moves its own pointer to its grandparent until
+// the parent is a root; the pointers it reads are being shortened by their
+// own vertices at the same time, so a chain of length d collapses in about
+// log d steps per vertex instead of each vertex walking it alone (sampled
+// links on a grid make chains of thousands).
 __global__ void __launch_bounds__(BLOCK) k_cc_compress(int *p, int64_t n) {
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += (int64_t)gridDim.x * blockDim.x) {
     int x = cc_get(p, v);
-    for (int y = cc_get(p, x); y != x; y = cc_get(p, x)) x = y;
-    p[v] = x;
+    int y = cc_get(p, x);
+    while (y != x) {
+      __hip_atomic_store(p + v, y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      x = y;
+      y = cc_get(p, x);
+    }
   }
 }
 

@@ -1,0 +1,375 @@
+// Device-resident direction-optimising BFS for one query.
+//
+// The engine's level loop (inspect -> select -> filter -> expand) reads the
+// inspector's counters back to the host every level, and the host picks the
+// next kernel. Here the whole choice stays on the device, so a query is a
+// captured sequence of two kernels per level replayed as one hipGraph,
+// with one host read per replay:
+//   k_do_expand   one level: top-down (push) over the frontier bitmap, or
+//                 bottom-up (pull) over the unvisited vertices, as the
+//                 control block says; writes the next frontier bitmap and
+//                 the next level's counters (vertices, out-edges)
+//   k_do_decide   one workgroup: reduces the counters, stops the loop on an
+//                 empty frontier and picks the next direction with the
+//                 switching rule of Beamer, Asanovic and Patterson
+//                 ("Direction-Optimizing Breadth-First Search", SC 2012):
+//                 top-down -> bottom-up when the frontier's edges exceed
+//                 the unexplored vertices' edges / alpha and the frontier
+//                 grew, bottom-up -> top-down when the frontier holds fewer
+//                 than n / beta vertices and shrank.
+// Frontiers are bitmaps (one uint64 per 64 vertices) in three rotating
+// buffers: level L reads F[L % 3], writes F[(L + 1) % 3] and clears
+// F[(L + 2) % 3] for the level after. The level, the direction and the
+// counters live in the control block, so every captured level is the same
+// pair of launches.
+//
+// Top-down: wavefronts take 64 frontier words per step; for every non-zero
+// word its vertices' edges are walked 64 at a time (degree scan across the
+// lanes, owner found in LDS). A vertex of more than `heavy_deg` out-edges
+// is not walked by one wavefront: every such vertex found in a level is
+// appended to a short list, and the next top-down level splits the listed
+// vertices' edges evenly over the whole grid (prefix of their degrees
+// computed by k_do_decide). If the list overflows, its vertices are walked
+// by their own wavefront, 64 edges per step.
+// Bottom-up: one wavefront per frontier word, one lane per target; the
+// first probe is the per-vertex copy of the first in-neighbour (coalesced),
+// then the in-row until a frontier source is found (early exit). The next
+// frontier word is the wavefront's ballot (plain store, every word written).
+//
+// The result is the BFS level array, the same whatever the directions.
+#pragma once
+#include "../utils/runtime.h"
+#include "../utils/wave.h"
+
+namespace gsw {
+
+constexpr int DO_HCAP = 16384;   // listed heavy vertices per level
+constexpr int DO_SLOTS = 64;     // partial-counter slots, one line each
+constexpr int DO_STRIDE = 16;    // long longs per slot
+constexpr int DO_DECIDE_BLOCK = 1024;
+
+// control block (long long words)
+enum : int {
+  DO_LEVEL = 0,   // level being expanded (its frontier holds level L)
+  DO_HALT,        // 1 once a level found nothing
+  DO_DIR,         // 0 top-down, 1 bottom-up (for level DO_LEVEL)
+  DO_NF,          // frontier vertices of level DO_LEVEL
+  DO_MF,          // their out-edges
+  DO_VISE,        // out-edges of every visited vertex
+  DO_HN,          // listed heavy vertices of the frontier
+  DO_HOVF,        // 1: the frontier's heavy list overflowed
+  DO_HNEXT,       // appends to the next level's list
+  DO_HE,          // out-edges of the listed heavy vertices
+  DO_LEVELS,      // non-empty frontiers expanded (set on halt)
+  DO_CTRL_WORDS = 16,
+  DO_SLOT_BASE = DO_CTRL_WORDS,
+  DO_TOTAL = DO_SLOT_BASE + DO_SLOTS * DO_STRIDE
+};
+
+struct dobfs_t {
+  int64_t n = 0, nw = 0, m = 0;
+  const int64_t *rows = nullptr;   // out-CSR
+  const int32_t *cols = nullptr;
+  const int64_t *rrows = nullptr;  // in-CSR
+  const int32_t *rcols = nullptr;
+  const int32_t *first = nullptr;  // first in-neighbour, -1 without in-edges
+  const int32_t *odeg = nullptr;
+  const int32_t *ideg = nullptr;
+  int32_t *lv = nullptr;           // levels, -1 unreached
+  uint64_t *vis = nullptr;         // [nw] visited
+  uint64_t *F = nullptr;           // [3 * nw] rotating frontiers
+  int32_t *heavy = nullptr;        // [2 * DO_HCAP] heavy lists (level parity)
+  int64_t *hpre = nullptr;         // [DO_HCAP + 1] degree prefix of the list
+  long long *ctrl = nullptr;       // [DO_TOTAL]
+  int heavy_deg = 4096;
+  float alpha = 14.f, beta = 24.f;
+};
+
+// Query reset: levels -1 (root 0), visited and frontier words zero except
+// the root's bit, control block for level 0 (top-down from the root; the
+// root is the heavy list of level 0 when its degree is above heavy_deg).
+__global__ void __launch_bounds__(BLOCK) k_do_reset(dobfs_t d, int root) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t rw = root >> 6;
+  const uint64_t rb = 1ull << (root & 63);
+  for (int64_t v = tid; v < d.n; v += stride) d.lv[v] = v == root ? 0 : -1;
+  for (int64_t w = tid; w < d.nw; w += stride) {
+    const uint64_t x = w == rw ? rb : 0ull;
+    d.vis[w] = x;
+    d.F[w] = x;
+    d.F[d.nw + w] = 0ull;
+    d.F[2 * d.nw + w] = 0ull;
+  }
+  if (blockIdx.x == 0) {
+    for (int i = threadIdx.x; i < DO_TOTAL; i += blockDim.x) d.ctrl[i] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const long long deg = d.odeg[root];
+      d.ctrl[DO_NF] = 1;
+      d.ctrl[DO_MF] = deg;
+      d.ctrl[DO_VISE] = deg;
+      if (deg > d.heavy_deg) {
+        d.heavy[0] = root;
+        d.hpre[0] = 0;
+        d.hpre[1] = deg;
+        d.ctrl[DO_HN] = 1;
+        d.ctrl[DO_HE] = deg;
+      }
+    }
+  }
+}
+
+struct do_level_t {
+  int64_t L;
+  const uint64_t *Fc;
+  uint64_t *Fn;
+  int32_t *hnext;
+};
+
+// A vertex found for level L + 1: level, counters, heavy list.
+__device__ __forceinline__ void do_found(const dobfs_t &d, const do_level_t &t, int u,
+                                         long long &nf, long long &mf) {
+  d.lv[u] = (int)(t.L + 1);
+  const int du = d.odeg[u];
+  ++nf;
+  mf += du;
+  if (du > d.heavy_deg) {
+    const long long i = atomicAdd((unsigned long long *)&d.ctrl[DO_HNEXT], 1ull);
+    if (i < DO_HCAP) t.hnext[i] = u;
+  }
+}
+
+// Top-down visit of u: the first visitor (atomic on the visited word) owns it.
+__device__ __forceinline__ void do_visit(const dobfs_t &d, const do_level_t &t, int u,
+                                         long long &nf, long long &mf) {
+  const int64_t w = u >> 6;
+  const uint64_t b = 1ull << (u & 63);
+  if (d.vis[w] & b) return;
+  const uint64_t old = atomicOr((unsigned long long *)&d.vis[w], (unsigned long long)b);
+  if (old & b) return;
+  atomicOr((unsigned long long *)&t.Fn[w], (unsigned long long)b);
+  do_found(d, t, u, nf, mf);
+}
+
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t x, int src) {
+  const int lo = __shfl((int)(uint32_t)x, src, 64);
+  const int hi = __shfl((int)(uint32_t)(x >> 32), src, 64);
+  return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_do_expand(dobfs_t d) {
+  __shared__ int s_scan[BLOCK];
+  __shared__ int64_t s_start[BLOCK];
+  __shared__ long long sh[BLOCK / WAVE];
+  const long long *ctrl = d.ctrl;
+  if (ctrl[DO_HALT]) return;
+  do_level_t t;
+  t.L = ctrl[DO_LEVEL];
+  const bool pull = ctrl[DO_DIR] != 0;
+  t.Fc = d.F + (t.L % 3) * d.nw;
+  t.Fn = d.F + ((t.L + 1) % 3) * d.nw;
+  uint64_t *Fz = d.F + ((t.L + 2) % 3) * d.nw;
+  t.hnext = d.heavy + ((t.L + 1) & 1) * DO_HCAP;
+  const int lane = lane_id();
+  const int off = threadIdx.x & ~(WAVE - 1);
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t wave = tid >> LOG_WAVE;
+  const int64_t nwaves = stride >> LOG_WAVE;
+  long long nf = 0, mf = 0;
+  for (int64_t i = tid; i < d.nw; i += stride) Fz[i] = 0ull;
+
+  if (!pull) {
+    const bool hovf = ctrl[DO_HOVF] != 0;
+    // listed heavy vertices: their edges split evenly over the grid
+    const int64_t HE = hovf ? 0 : ctrl[DO_HE];
+    if (HE > 0) {
+      const int hn = (int)ctrl[DO_HN];
+      const int32_t *hcur = d.heavy + (t.L & 1) * DO_HCAP;
+      for (int64_t k = tid; k < HE; k += stride) {
+        const int o = owner_search(d.hpre, hn, k);
+        const int v = hcur[o];
+        const int u = d.cols[d.rows[v] + (k - d.hpre[o])];
+        do_visit(d, t, u, nf, mf);
+      }
+    }
+    // the other frontier vertices: 64 words per wavefront step
+    for (int64_t w0 = wave * WAVE; w0 < d.nw; w0 += nwaves * WAVE) {
+      const int64_t wi = w0 + lane;
+      const uint64_t word = wi < d.nw ? t.Fc[wi] : 0ull;
+      uint64_t nz = ballot(word != 0ull);
+      while (nz) {
+        const int j = __builtin_ctzll(nz);
+        nz &= nz - 1;
+        const uint64_t bits = shfl_u64(word, j);
+        const int v = (int)(((w0 + j) << 6) + lane);
+        int deg = 0;
+        int64_t start = 0;
+        bool alone = false;
+        if ((bits >> lane) & 1ull) {
+          deg = d.odeg[v];
+          if (deg > d.heavy_deg) {
+            alone = hovf;  // walked below by this wavefront
+            deg = 0;       // otherwise walked by the heavy pass
+          } else {
+            start = d.rows[v];
+          }
+        }
+        int tot;
+        const int ex = wave_exclusive_scan(deg, tot);
+        s_scan[off + lane] = ex;
+        s_start[off + lane] = start;
+        wave_sync();
+        for (int j0 = 0; j0 < tot; j0 += WAVE) {
+          const int k = j0 + lane;
+          if (k < tot) {
+            const int o = owner_search(&s_scan[off], WAVE, k);
+            const int u = d.cols[s_start[off + o] + (k - s_scan[off + o])];
+            do_visit(d, t, u, nf, mf);
+          }
+        }
+        wave_sync();
+        // heavy vertices of an overflowed list: one wavefront each
+        uint64_t al = ballot(alone);
+        while (al) {
+          const int a = __builtin_ctzll(al);
+          al &= al - 1;
+          const int va = __shfl(v, a, 64);
+          const int64_t s = d.rows[va], e = d.rows[va + 1];
+          for (int64_t k = s + lane; k < e; k += WAVE) do_visit(d, t, d.cols[k], nf, mf);
+        }
+      }
+    }
+  } else {
+    // bottom-up: one frontier word per wavefront step
+    for (int64_t w = wave; w < d.nw; w += nwaves) {
+      const int64_t v = (w << 6) + lane;
+      const uint64_t vw = d.vis[w];
+      bool found = false;
+      if (v < d.n && !((vw >> lane) & 1ull)) {
+        const int id = d.ideg[v];
+        if (id > 0) {
+          int u = d.first[v];
+          found = (t.Fc[u >> 6] >> (u & 63)) & 1ull;
+          if (!found && id > 1) {
+            const int64_t s = d.rrows[v];
+            for (int k = 1; k < id; ++k) {
+              u = d.rcols[s + k];
+              if ((t.Fc[u >> 6] >> (u & 63)) & 1ull) {
+                found = true;
+                break;
+              }
+            }
+          }
+        }
+      }
+      const uint64_t fm = ballot(found);
+      if (lane == 0) {
+        t.Fn[w] = fm;
+        if (fm) d.vis[w] = vw | fm;
+      }
+      if (found) do_found(d, t, (int)v, nf, mf);
+    }
+  }
+  // counters of level L + 1 into this workgroup's slot
+  long long *slot = d.ctrl + DO_SLOT_BASE + (int64_t)(blockIdx.x & (DO_SLOTS - 1)) * DO_STRIDE;
+  const long long a = block_sum(nf, sh);
+  if (threadIdx.x == 0 && a) atomicAdd((unsigned long long *)&slot[0], (unsigned long long)a);
+  const long long b = block_sum(mf, sh);
+  if (threadIdx.x == 0 && b) atomicAdd((unsigned long long *)&slot[1], (unsigned long long)b);
+}
+
+// End of level L: counters of level L + 1, halt on an empty frontier, the
+// direction of level L + 1, its heavy list and the list's degree prefix.
+__global__ void __launch_bounds__(DO_DECIDE_BLOCK) k_do_decide(dobfs_t d) {
+  __shared__ long long s_red[2][DO_DECIDE_BLOCK / WAVE];
+  __shared__ int s_go;
+  __shared__ long long s_w[DO_DECIDE_BLOCK / WAVE];
+  long long *ctrl = d.ctrl;
+  if (ctrl[DO_HALT]) return;
+  long long nf = 0, mf = 0;
+  if (threadIdx.x < DO_SLOTS) {
+    long long *slot = ctrl + DO_SLOT_BASE + threadIdx.x * DO_STRIDE;
+    nf = slot[0];
+    mf = slot[1];
+    slot[0] = 0;
+    slot[1] = 0;
+  }
+  nf = block_sum(nf, s_red[0]);
+  mf = block_sum(mf, s_red[1]);
+  const int nth = DO_DECIDE_BLOCK;
+  if (threadIdx.x == 0) {
+    const long long L = ctrl[DO_LEVEL];
+    s_go = 0;
+    if (nf == 0) {
+      ctrl[DO_HALT] = 1;
+      ctrl[DO_LEVELS] = L + 1;
+    } else {
+      const long long vise = ctrl[DO_VISE] + mf;
+      const long long mu = d.m - vise;
+      const long long prev_nf = ctrl[DO_NF];
+      const bool was_pull = ctrl[DO_DIR] != 0;
+      bool pull;
+      if (!was_pull)
+        pull = (double)mf > (double)mu / d.alpha && nf > prev_nf;
+      else
+        pull = !((double)nf < (double)d.n / d.beta && nf < prev_nf);
+      const long long hn = ctrl[DO_HNEXT];
+      ctrl[DO_VISE] = vise;
+      ctrl[DO_NF] = nf;
+      ctrl[DO_MF] = mf;
+      ctrl[DO_DIR] = pull ? 1 : 0;
+      ctrl[DO_HN] = hn < DO_HCAP ? hn : DO_HCAP;
+      ctrl[DO_HOVF] = hn > DO_HCAP ? 1 : 0;
+      ctrl[DO_HNEXT] = 0;
+      ctrl[DO_HE] = 0;
+      ctrl[DO_LEVEL] = L + 1;
+      s_go = (!pull && hn > 0 && hn <= DO_HCAP) ? 1 : 0;
+    }
+  }
+  __syncthreads();
+  if (!s_go) return;
+  // degree prefix of level L + 1's heavy list (the list written in level L)
+  const long long L1 = ctrl[DO_LEVEL];
+  const int hn = (int)ctrl[DO_HN];
+  const int32_t *h = d.heavy + (L1 & 1) * DO_HCAP;
+  constexpr int PER = DO_HCAP / DO_DECIDE_BLOCK;
+  long long loc[PER];
+  long long sum = 0;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int k = threadIdx.x * PER + i;
+    loc[i] = k < hn ? (long long)d.odeg[h[k]] : 0;
+    sum += loc[i];
+  }
+  // exclusive scan of the per-thread sums in 64 bits: shuffles inside the
+  // wavefront, wavefront totals through LDS
+  long long incl = sum;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const long long y = __shfl_up(incl, o, 64);
+    if (lane_id() >= o) incl += y;
+  }
+  long long ex = incl - sum;
+  const int w = threadIdx.x >> LOG_WAVE;
+  if (lane_id() == WAVE - 1) s_w[w] = incl;
+  __syncthreads();
+  long long tot = 0;
+  for (int i = 0; i < nth / WAVE; ++i) {
+    ex += i < w ? s_w[i] : 0;
+    tot += s_w[i];
+  }
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int k = threadIdx.x * PER + i;
+    if (k < hn) d.hpre[k] = ex;
+    ex += loc[i];
+  }
+  if (threadIdx.x == 0) {
+    d.hpre[hn] = tot;
+    ctrl[DO_HE] = tot;
+  }
+}
+
+}  // namespace gsw

@@ -38,6 +38,8 @@ def main(argv=None):
     ap.add_argument("--er-batch", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--selector", type=int, default=0)
+    ap.add_argument("--capture", type=int, default=0,
+                    help="bfs: device-resident loop, levels per replay")
     ap.add_argument("--no-profile", action="store_true",
                     help="no per-expand hipEvents (records keep no times)")
     a = ap.parse_args(argv)
@@ -63,6 +65,8 @@ def main(argv=None):
     kw = dict(configs=a.configs, out=False, profile=not a.no_profile)
     if a.selector and a.app in ("bfs", "sssp"):
         kw["selector"] = 1
+    if a.capture and a.app == "bfs":
+        kw["capture"] = a.capture
     print("graph built in %.1fs: n=%d m=%d" % (time.perf_counter() - t, h.n, h.m),
           file=sys.stderr, flush=True)
     for _ in range(a.repeat):
