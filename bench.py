@@ -189,6 +189,8 @@ def bfs_single(gsw, h, root, reps, g=None, **kw):
 
 
 PQWM = "Push-Queue-WM-Standalone"
+DEVICE_LEVELS = 8        # device-resident BFS: levels per hipGraph replay
+DEVICE_LEVELS_GRID = 16
 
 
 def fixed_sweep(run, ref, reps=1):
@@ -275,6 +277,15 @@ def extended(gsw, args):
         single["learned_ms"] = round(ms_l, 3)
         single["learned_bit_exact"] = bool(np.array_equal(lv2, lv))
         learned = ms_l
+    # the device-resident loop (direction chosen on the device, levels
+    # replayed from a captured hipGraph)
+    lvd, ms_d, recs_d, nbytes_d = bfs_single(gsw, h, root, 7, g=g, capture=DEVICE_LEVELS)
+    single["device_loop"] = {
+        "ms": round(ms_d, 3), "gteps": round(e / (ms_d * 1e-3) / 1e9, 2),
+        "algorithmic_GBps": round(nbytes_d / (ms_d * 1e-3) / 1e9, 1),
+        "hbm_frac": round(nbytes_d / (ms_d * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "levels": len(recs_d), "bit_exact": bool(np.array_equal(lvd, lv)),
+        "directions": "".join("P" if r["dir"] == "Pull" else "p" for r in recs_d)}
     g.close()
     out["bfs_single"] = single
     del h
@@ -290,6 +301,9 @@ def extended(gsw, args):
             "bfs_ms": round(ms_g, 3),
             "bfs_gteps": round(graph500_edges(h, lvg) / (ms_g * 1e-3) / 1e9, 2),
             "bfs_fixed_strategies": sweepg}
+    lvg_d, ms_gd, recs_gd, _ = bfs_single(gsw, h, 0, 7, g=g, capture=DEVICE_LEVELS_GRID)
+    grid["bfs_device_loop_ms"] = round(ms_gd, 3)
+    grid["bfs_device_loop_bit_exact"] = bool(np.array_equal(lvg, lvg_d))
     ms_g_learned = ms_g
     if learned is not None:
         lvg_l, ms_g_learned, _, _ = bfs_single(gsw, h, 0, 7, g=g, selector=1)

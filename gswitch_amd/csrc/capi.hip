@@ -228,6 +228,7 @@ struct gsw_graph {
   int32_t *do_heavy = nullptr;
   int64_t *do_hpre = nullptr;
   long long *do_ctrl = nullptr;
+  long long *do_hist = nullptr;
   long long *do_hctrl = nullptr;  // pinned copy of the first control words
   hipGraph_t do_graph = nullptr;
   hipGraphExec_t do_exec = nullptr;
@@ -265,6 +266,7 @@ struct gsw_graph {
     if (do_heavy) hipFree(do_heavy);
     if (do_hpre) hipFree(do_hpre);
     if (do_ctrl) hipFree(do_ctrl);
+    if (do_hist) hipFree(do_hist);
     if (do_hctrl) hipHostFree(do_hctrl);
     if (cc_lab) hipFree(cc_lab);
     if (cc_ids) hipFree(cc_ids);
@@ -838,6 +840,7 @@ gsw::dobfs_t do_view(gsw_graph *G) {
     H_ERR(hipMalloc((void **)&G->do_heavy, sizeof(int32_t) * 2 * DO_HCAP));
     H_ERR(hipMalloc((void **)&G->do_hpre, sizeof(int64_t) * (DO_HCAP + 1)));
     H_ERR(hipMalloc((void **)&G->do_ctrl, sizeof(long long) * DO_TOTAL));
+    H_ERR(hipMalloc((void **)&G->do_hist, sizeof(long long) * DO_HIST * DO_HIST_W));
     H_ERR(hipHostMalloc((void **)&G->do_hctrl, sizeof(long long) * DO_CTRL_WORDS));
   }
   dobfs_t d;
@@ -857,6 +860,7 @@ gsw::dobfs_t do_view(gsw_graph *G) {
   d.heavy = G->do_heavy;
   d.hpre = G->do_hpre;
   d.ctrl = G->do_ctrl;
+  d.hist = G->do_hist;
   // switching thresholds and the heavy-vertex degree (defaults measured on
   // MI355X; the variables exist for sweeps)
   d.alpha = (float)env_num("GSW_DO_ALPHA", 14.0);
@@ -914,7 +918,32 @@ int bfs_device_run(gsw_graph *G, int32_t root, int per_launch, int grid, int max
     halted = h[DO_HALT] != 0;
   }
   H_ERR(hipEventRecord(G->ev1, st));
-  return halted ? (int)h[DO_LEVELS] : (int)h[DO_LEVEL];
+  const int levels = halted ? (int)h[DO_LEVELS] : (int)h[DO_LEVEL];
+  // one record per expanded level from the device history: top-down levels
+  // are wavefront-mapped pushes over the frontier bitmap, bottom-up levels
+  // thread-mapped pulls, both with the inspection folded in
+  G->records.clear();
+  const int nrec = MIN(levels, DO_HIST);
+  if (nrec > 0) {
+    std::vector<long long> hv((size_t)nrec * DO_HIST_W);
+    H_ERR(hipMemcpy(hv.data(), G->do_hist, sizeof(long long) * hv.size(),
+                    hipMemcpyDeviceToHost));
+    for (int L = 0; L < nrec; ++L) {
+      const long long *r = &hv[(size_t)L * DO_HIST_W];
+      gsw_level_record rec{};
+      rec.level = L;
+      rec.dir = r[0] ? 1 : 0;
+      rec.fmt = 1;
+      rec.lb = r[0] ? 0 : 1;
+      rec.fused = 1;
+      rec.frontier = r[1];
+      rec.active = r[1];
+      rec.edges = r[2];
+      rec.verts = r[3];
+      G->records.push_back(rec);
+    }
+  }
+  return levels;
 }
 
 // default workgroups of the device loop: one frontier word per wavefront
@@ -939,6 +968,9 @@ void cc_sample_phase(const gsw_graph *G, int32_t *lab, hipStream_t st) {
   const int64_t n = G->hg.n;
   const int grid = grid_for(n);
   hipLaunchKernelGGL(k_cc_init, dim3(grid), dim3(BLOCK), 0, st, lab, n);
+  // a compress after each round: sampled links on a grid build chains
+  // thousands long, which the second round's links would walk (4096-square
+  // grid: 107 ms with one compress at the end, 3.3 ms with two)
   for (int r = 0; r < 2; ++r) {
     hipLaunchKernelGGL(k_cc_sample, dim3(grid), dim3(BLOCK), 0, st,
                        (const int64_t *)G->dg.dg_row_ptr,
@@ -1321,7 +1353,6 @@ gsw_status gsw_bfs(gsw_graph *G, int32_t root, const gsw_run_opts *opts,
       const run_opts_t o = to_opts(opts);
       const int grid = opts->grid > 0 ? opts->grid : default_do_grid(G);
       levels = bfs_device_run(G, root, opts->capture, grid, o.max_levels);
-      G->records.clear();
     } else {
       levels = bfs_engine_run(G, opts, root, nullptr, 1, 0.0);
       collect_records(G, G->bfs->en.executor, G->bfs->as);

@@ -38,38 +38,22 @@ __device__ __forceinline__ int cc_get(const int *p, int64_t i) {
   return __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Root of v's tree with path halving on the way up (Jaiganesh and
-// Burtscher, "A High-Performance Connected Components Implementation for
-// GPUs", HPDC 2018): every vertex passed is pointed at its grandparent with a
-// plain store. Only non-roots are rewritten, always to an ancestor, and a
-// non-root never becomes a root again, so concurrent halving and hooking
-// (which changes roots only, by CAS) keep p a forest whose roots are the
-// smallest ids of their trees; trees stay shallow while links run.
-__device__ __forceinline__ int cc_find(int v, int *p) {
-  int cur = cc_get(p, v);
-  if (cur == v) return v;
-  int prev = v;
-  for (int next = cc_get(p, cur); next != cur; next = cc_get(p, cur)) {
-    __hip_atomic_store(p + prev, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    prev = cur;
-    cur = next;
-  }
-  return cur;
-}
-
-// join the trees of u and v: hook the larger root under the smaller one; a
-// failed CAS returns the root's new parent (smaller), and the hook is
-// retried from there
+// join the trees of u and v: hook the larger root under the smaller one
+// with a CAS, walking up from the parents (two steps at a time on the
+// larger side) while neither is a root. Measured against finds that halve
+// the path as they walk (Jaiganesh and Burtscher, HPDC 2018): on a union of
+// 5000 random graphs the halving stores cost 180 ms against 74 ms for the
+// whole batched BFS + CC, and the compress after each sampling round
+// already keeps grid chains short (4096-square grid 3.3 vs 2.7 ms).
 __device__ __forceinline__ void cc_link(int u, int v, int *p) {
-  int a = cc_find(u, p), b = cc_find(v, p);
+  int a = cc_get(p, u), b = cc_get(p, v);
   while (a != b) {
     const int hi = a > b ? a : b, lo = a > b ? b : a;
-    const int r = atomicCAS(p + hi, hi, lo);
-    if (r == hi) return;
-    // hi was hooked meanwhile (under a smaller id): continue from the
-    // current roots of both sides
-    a = cc_find(r, p);
-    b = cc_find(lo, p);
+    const int ph = cc_get(p, hi);
+    if (ph == lo) return;
+    if (ph == hi && atomicCAS(p + hi, hi, lo) == hi) return;
+    a = cc_get(p, cc_get(p, hi));
+    b = cc_get(p, lo);
   }
 }
 

@@ -47,6 +47,8 @@ constexpr int DO_HCAP = 16384;   // listed heavy vertices per level
 constexpr int DO_SLOTS = 64;     // partial-counter slots, one line each
 constexpr int DO_STRIDE = 16;    // long longs per slot
 constexpr int DO_DECIDE_BLOCK = 1024;
+constexpr int DO_HIST = 4096;    // levels with a history row
+constexpr int DO_HIST_W = 4;     // direction, frontier, edges, vertices
 
 // control block (long long words)
 enum : int {
@@ -81,6 +83,7 @@ struct dobfs_t {
   int32_t *heavy = nullptr;        // [2 * DO_HCAP] heavy lists (level parity)
   int64_t *hpre = nullptr;         // [DO_HCAP + 1] degree prefix of the list
   long long *ctrl = nullptr;       // [DO_TOTAL]
+  long long *hist = nullptr;       // [DO_HIST][DO_HIST_W] per-level history
   int heavy_deg = 4096;
   float alpha = 14.f, beta = 24.f;
 };
@@ -177,7 +180,8 @@ __global__ void __launch_bounds__(BLOCK) k_do_expand(dobfs_t d) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t wave = tid >> LOG_WAVE;
   const int64_t nwaves = stride >> LOG_WAVE;
-  long long nf = 0, mf = 0;
+  long long nf = 0, mf = 0;  // next frontier: vertices, out-edges
+  long long ne = 0, nv = 0;  // this level: edges inspected, vertices expanded
   for (int64_t i = tid; i < d.nw; i += stride) Fz[i] = 0ull;
 
   if (!pull) {
@@ -188,6 +192,7 @@ __global__ void __launch_bounds__(BLOCK) k_do_expand(dobfs_t d) {
       const int hn = (int)ctrl[DO_HN];
       const int32_t *hcur = d.heavy + (t.L & 1) * DO_HCAP;
       for (int64_t k = tid; k < HE; k += stride) {
+        ++ne;
         const int o = owner_search(d.hpre, hn, k);
         const int v = hcur[o];
         const int u = d.cols[d.rows[v] + (k - d.hpre[o])];
@@ -208,6 +213,7 @@ __global__ void __launch_bounds__(BLOCK) k_do_expand(dobfs_t d) {
         int64_t start = 0;
         bool alone = false;
         if ((bits >> lane) & 1ull) {
+          ++nv;
           deg = d.odeg[v];
           if (deg > d.heavy_deg) {
             alone = hovf;  // walked below by this wavefront
@@ -216,6 +222,7 @@ __global__ void __launch_bounds__(BLOCK) k_do_expand(dobfs_t d) {
             start = d.rows[v];
           }
         }
+        ne += deg;
         int tot;
         const int ex = wave_exclusive_scan(deg, tot);
         s_scan[off + lane] = ex;
@@ -237,7 +244,10 @@ __global__ void __launch_bounds__(BLOCK) k_do_expand(dobfs_t d) {
           al &= al - 1;
           const int va = __shfl(v, a, 64);
           const int64_t s = d.rows[va], e = d.rows[va + 1];
-          for (int64_t k = s + lane; k < e; k += WAVE) do_visit(d, t, d.cols[k], nf, mf);
+          for (int64_t k = s + lane; k < e; k += WAVE) {
+            ++ne;
+            do_visit(d, t, d.cols[k], nf, mf);
+          }
         }
       }
     }
@@ -248,14 +258,17 @@ __global__ void __launch_bounds__(BLOCK) k_do_expand(dobfs_t d) {
       const uint64_t vw = d.vis[w];
       bool found = false;
       if (v < d.n && !((vw >> lane) & 1ull)) {
+        ++nv;
         const int id = d.ideg[v];
         if (id > 0) {
           int u = d.first[v];
+          ++ne;
           found = (t.Fc[u >> 6] >> (u & 63)) & 1ull;
           if (!found && id > 1) {
             const int64_t s = d.rrows[v];
             for (int k = 1; k < id; ++k) {
               u = d.rcols[s + k];
+              ++ne;
               if ((t.Fc[u >> 6] >> (u & 63)) & 1ull) {
                 found = true;
                 break;
@@ -278,30 +291,44 @@ __global__ void __launch_bounds__(BLOCK) k_do_expand(dobfs_t d) {
   if (threadIdx.x == 0 && a) atomicAdd((unsigned long long *)&slot[0], (unsigned long long)a);
   const long long b = block_sum(mf, sh);
   if (threadIdx.x == 0 && b) atomicAdd((unsigned long long *)&slot[1], (unsigned long long)b);
+  const long long c = block_sum(ne, sh);
+  if (threadIdx.x == 0 && c) atomicAdd((unsigned long long *)&slot[2], (unsigned long long)c);
+  const long long e = block_sum(nv, sh);
+  if (threadIdx.x == 0 && e) atomicAdd((unsigned long long *)&slot[3], (unsigned long long)e);
 }
 
 // End of level L: counters of level L + 1, halt on an empty frontier, the
 // direction of level L + 1, its heavy list and the list's degree prefix.
 __global__ void __launch_bounds__(DO_DECIDE_BLOCK) k_do_decide(dobfs_t d) {
-  __shared__ long long s_red[2][DO_DECIDE_BLOCK / WAVE];
+  __shared__ long long s_red[4][DO_DECIDE_BLOCK / WAVE];
   __shared__ int s_go;
   __shared__ long long s_w[DO_DECIDE_BLOCK / WAVE];
   long long *ctrl = d.ctrl;
   if (ctrl[DO_HALT]) return;
-  long long nf = 0, mf = 0;
+  long long nf = 0, mf = 0, ne = 0, nv = 0;
   if (threadIdx.x < DO_SLOTS) {
     long long *slot = ctrl + DO_SLOT_BASE + threadIdx.x * DO_STRIDE;
     nf = slot[0];
     mf = slot[1];
-    slot[0] = 0;
-    slot[1] = 0;
+    ne = slot[2];
+    nv = slot[3];
+    slot[0] = slot[1] = slot[2] = slot[3] = 0;
   }
   nf = block_sum(nf, s_red[0]);
   mf = block_sum(mf, s_red[1]);
+  ne = block_sum(ne, s_red[2]);
+  nv = block_sum(nv, s_red[3]);
   const int nth = DO_DECIDE_BLOCK;
   if (threadIdx.x == 0) {
     const long long L = ctrl[DO_LEVEL];
     s_go = 0;
+    if (L < DO_HIST) {
+      long long *hrow = d.hist + L * DO_HIST_W;
+      hrow[0] = ctrl[DO_DIR];
+      hrow[1] = ctrl[DO_NF];
+      hrow[2] = ne;
+      hrow[3] = nv;
+    }
     if (nf == 0) {
       ctrl[DO_HALT] = 1;
       ctrl[DO_LEVELS] = L + 1;

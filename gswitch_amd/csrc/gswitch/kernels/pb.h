@@ -43,7 +43,6 @@ namespace gsw {
 constexpr int PB_BLOCK = 1024;
 constexpr int PB_LDS_BYTES = 64 << 10;  // per phase: messages / accumulators
 constexpr int PB_UNROLL = 8;            // slots a lane keeps in flight
-constexpr int PB_WIN = PB_BLOCK;        // phase-1 tiles staged at a time
 
 template <typename F, typename = void> struct pb_msg {
   using type = typename F::wa_t;
@@ -111,63 +110,57 @@ __global__ void __launch_bounds__(BLOCK)
   }
 }
 
-// Phase 1: one workgroup per piece (chunk c, bins [b0, b1)). The piece's
-// tiles are taken PB_WIN at a time: their starts and an inclusive prefix
-// of their lengths go to LDS, and the window's edges are numbered
-// 0 .. total-1 across its tiles; lane work k finds its tile by a binary
-// search in LDS, so every lane has PB_UNROLL independent slots in flight
-// whatever the tile lengths (consecutive k are consecutive slots inside a
-// tile: the stores coalesce).
+// Phase 1: one workgroup per piece (chunk c, bins [b0, b1)). The chunk's
+// messages are staged in LDS once; then every wavefront takes whole tiles
+// of the piece in turn (tile b0 + w, b0 + w + waves, ...) and walks each
+// tile PB_UNROLL * 64 slots per step: consecutive lanes read consecutive
+// 16-bit source offsets and store to consecutive slots (both coalesced),
+// with PB_UNROLL independent slots in flight per lane. Tiles hold about
+// m / (B * C) edges (about 400 on R-MAT scale 22, 3% of the edges in tiles
+// shorter than a wavefront), so a tile per wavefront wastes few lanes and
+// no lane searches for its tile.
 template <class Bd, typename MSG>
 __global__ void __launch_bounds__(PB_BLOCK) k_pb_scatter(Bd body, pb_t pb) {
   constexpr int CAP = PB_LDS_BYTES / (int)sizeof(MSG);
   __shared__ MSG s_msg[CAP];
-  __shared__ int64_t s_start[PB_WIN];
-  __shared__ int32_t s_pre[PB_WIN + 1];
-  __shared__ int s_sh[PB_BLOCK / WAVE];
   const int32_t *pc = pb.p1 + 4 * (int64_t)blockIdx.x;
   const int c = pc[0], b0 = pc[1], b1 = pc[2];
   const int64_t lo = (int64_t)c * pb.S;
   const int w = (int)MIN((int64_t)pb.S, body.g.nvertexs - lo);
   for (int i = threadIdx.x; i < w; i += PB_BLOCK)
     s_msg[i] = (MSG)body.msg((int)(lo + i), 0);
+  __syncthreads();
   MSG *vals = (MSG *)pb.vals;
-  for (int tb = b0; tb < b1; tb += PB_WIN) {
-    const int nt = MIN(PB_WIN, b1 - tb);
-    int len = 0;
-    if ((int)threadIdx.x < nt) {
-      const int64_t t = (int64_t)c * pb.B + tb + threadIdx.x;
-      len = pb.cm_len[t];
-      s_start[threadIdx.x] = pb.cm_start[t];
-    }
-    int total = 0;
-    const int ex = block_exclusive_scan(len, total, s_sh);
-    if ((int)threadIdx.x < nt) s_pre[threadIdx.x + 1] = ex + len;
-    if (threadIdx.x == 0) s_pre[0] = 0;
-    __syncthreads();  // also orders the message staging above
-    for (int k0 = 0; k0 < total; k0 += PB_BLOCK * PB_UNROLL) {
-      int64_t slot[PB_UNROLL];
+  const int lane = lane_id();
+  const int wv = threadIdx.x >> LOG_WAVE;
+  constexpr int NW = PB_BLOCK / WAVE;
+  const int64_t row = (int64_t)c * pb.B;
+  for (int b = b0 + wv; b < b1; b += NW) {
+    const int64_t start = pb.cm_start[row + b];
+    const int len = pb.cm_len[row + b];
+    for (int k0 = 0; k0 < len; k0 += WAVE * PB_UNROLL) {
       uint16_t si[PB_UNROLL];
 #pragma unroll
       for (int j = 0; j < PB_UNROLL; ++j) {
-        const int k = k0 + j * PB_BLOCK + (int)threadIdx.x;
-        slot[j] = -1;
-        if (k < total) {
-          const int t = owner_search(s_pre, nt + 1, k);  // s_pre[t] <= k < s_pre[t+1]
-          slot[j] = s_start[t] + (k - s_pre[t]);
-          si[j] = pb.src[slot[j]];
-        }
+        const int k = k0 + j * WAVE + lane;
+        si[j] = k < len ? pb.src[start + k] : 0;
       }
 #pragma unroll
-      for (int j = 0; j < PB_UNROLL; ++j)
-        if (slot[j] >= 0) vals[slot[j]] = s_msg[si[j]];
+      for (int j = 0; j < PB_UNROLL; ++j) {
+        const int k = k0 + j * WAVE + lane;
+        if (k < len) vals[start + k] = s_msg[si[j]];
+      }
     }
-    __syncthreads();  // s_start / s_pre are refilled by the next window
   }
 }
 
 // Phase 2: one workgroup per piece (bin b, slots [e0, e1), whole bin or
-// part of a one-target bin).
+// part of a one-target bin). Slots are combined with compAtomic into LDS
+// accumulators (native ds_add_f64 for PageRank). Measured on R-MAT scale 22
+// and kept plain: combining runs of one target in registers first (a
+// segmented shuffle reduction, also only for 64-slot groups of at most 8
+// runs) cost more in shuffles than it saved in atomics (852 and 691 us per
+// launch against 554 us).
 template <class Bd, typename MSG>
 __global__ void __launch_bounds__(PB_BLOCK) k_pb_gather(Bd body, pb_t pb) {
   using WA = typename Bd::WA;
@@ -181,6 +174,7 @@ __global__ void __launch_bounds__(PB_BLOCK) k_pb_gather(Bd body, pb_t pb) {
   const int w = pb.bin_lo[b + 1] - lo;
   const Status want = body.conf.want();
   const MSG *vals = (const MSG *)pb.vals;
+  const int lane = lane_id();
   if (whole) {
     for (int i = threadIdx.x; i < w; i += PB_BLOCK) s_acc[i] = *body.f.wa_of(lo + i);
     __syncthreads();
@@ -218,7 +212,7 @@ __global__ void __launch_bounds__(PB_BLOCK) k_pb_gather(Bd body, pb_t pb) {
     }
     bin_combine(body, acc, has, WAVE);
     const int wv = threadIdx.x >> LOG_WAVE;
-    if (lane_id() == 0) {
+    if (lane == 0) {
       s_acc[wv] = acc;
       s_has[wv] = has;
     }

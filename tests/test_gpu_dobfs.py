@@ -61,10 +61,12 @@ def test_device_bfs_other_graphs(gsw, orc, spec):
     g.close()
 
 
-@pytest.mark.parametrize("alpha,beta", [(1e-6, 1e-6), (1e9, 24), (14, 1e9), (0.5, 2)])
+@pytest.mark.parametrize("alpha,beta", [(1e-6, 24), (1e9, 1e-6), (1e9, 1e9), (14, 1e9), (0.5, 2)])
 def test_device_bfs_switching_thresholds(gsw, orc, rmat14, alpha, beta):
-    """Always bottom-up after the root, never bottom-up, switch back at once,
-    and a flip-flopping setting: the levels do not depend on the directions."""
+    """Top-down to bottom-up when the frontier's edges exceed the unexplored
+    edges / alpha: never (alpha tiny), whenever the frontier grows (alpha
+    huge) with an immediate switch back (beta tiny) or none (beta huge),
+    and in between: the levels do not depend on the directions."""
     h, g = rmat14
     root = h.max_degree_vertex()
     with env(GSW_DO_ALPHA=alpha, GSW_DO_BETA=beta):
@@ -74,7 +76,7 @@ def test_device_bfs_switching_thresholds(gsw, orc, rmat14, alpha, beta):
 
 @pytest.mark.parametrize("heavy", [64, 100, 100000])
 def test_device_bfs_heavy_vertices(gsw, orc, heavy):
-    """Top-down only (alpha huge) with a low heavy degree: the listed heavy
+    """Top-down only (alpha tiny) with a low heavy degree: the listed heavy
     vertices' edges are split over the grid; on R-MAT 19 levels 1 and 2
     hold 25416 and 18320 vertices of degree > 64, more than the list's
     16384 entries, so the overflow path (one wavefront per heavy vertex)
@@ -85,7 +87,7 @@ def test_device_bfs_heavy_vertices(gsw, orc, heavy):
     root = h.max_degree_vertex()
     ref = orc.bfs(h.row_ptr, h.col, root)
     assert ((ref == 1) & (deg > 64)).sum() > 16384
-    with env(GSW_DO_ALPHA=1e9, GSW_DO_HEAVY=heavy):
+    with env(GSW_DO_ALPHA=1e-9, GSW_DO_HEAVY=heavy):
         for cap in (1, 5):
             assert np.array_equal(g.bfs(root, capture=cap), ref)
     g.close()
@@ -146,3 +148,32 @@ def test_device_bfs_rejects_configs(gsw, rmat14):
         g.bfs(0, capture=4, configs="Push-x-x-x")
     with pytest.raises(gsw.GswError):
         g.bfs(0, capture=4, selector=1)
+
+
+@pytest.mark.parametrize("alpha", [1e-6, 14, 1e9])
+def test_device_bfs_level_records(gsw, orc, rmat14, alpha):
+    """The device history gives one record per level: the frontier is the
+    level's vertex count, a top-down level walks every out-edge of its
+    frontier, a bottom-up level examines every vertex still unvisited."""
+    h, g = rmat14
+    root = h.max_degree_vertex()
+    ref = orc.bfs(h.row_ptr, h.col, root)
+    deg = np.diff(h.row_ptr)
+    with env(GSW_DO_ALPHA=alpha):
+        g.bfs(root, capture=4, out=False)
+    recs = g.records()
+    assert len(recs) == int(ref.max()) + 1 == g.last_stats.levels
+    for L, r in enumerate(recs):
+        assert r["level"] == L
+        assert r["frontier"] == int((ref == L).sum())
+        if r["dir"] == "Push":
+            assert r["verts"] == r["frontier"]
+            assert r["edges"] == int(deg[ref == L].sum())
+        else:
+            assert r["verts"] == int(((ref == -1) | (ref > L)).sum())
+    dirs = [r["dir"] for r in recs]
+    assert dirs[0] == "Push"
+    if alpha == 1e-6:
+        assert set(dirs) == {"Push"}
+    if alpha == 1e9:
+        assert "Pull" in dirs
