@@ -49,7 +49,7 @@ class RunStats(C.Structure):
 class LevelRecord(C.Structure):
     _fields_ = [("level", C.c_int32), ("dir", C.c_int8), ("fmt", C.c_int8),
                 ("lb", C.c_int8), ("fused", C.c_int8), ("sorted", C.c_int8),
-                ("pad", C.c_int8 * 3), ("frontier", C.c_int64),
+                ("kernel", C.c_int8), ("pad", C.c_int8 * 2), ("frontier", C.c_int64),
                 ("active", C.c_int64), ("inactive", C.c_int64),
                 ("edges", C.c_int64), ("verts", C.c_int64), ("ms", C.c_float),
                 ("pad2", C.c_float)]
@@ -313,12 +313,29 @@ def _f32(a, n=None):
     return a
 
 
-SELECTOR_APPS = {"bfs": 0, "sssp": 1}
+SELECTOR_APPS = {"bfs": 0, "sssp": 1, "pr": 2}
+# PageRank's learned classes (the network's first 8 outputs): one from-all
+# pull for the whole run, as (forced variant, pull_kernel)
+PR_CLASSES = [("Pull-Bitmap-TM-Standalone", 2), ("Pull-Bitmap-WM-Standalone", 2),
+              ("Pull-Bitmap-CM-Standalone", 2), ("Pull-Bitmap-STRICT-Standalone", 0),
+              ("Pull-Bitmap-STRICT-Standalone", 1), ("Pull-Bitmap-TWOD-Standalone", 0),
+              ("Pull-Bitmap-TWOD-Standalone", 1), ("Pull-Bitmap-TWOD-Standalone", 2)]
+PR_CLASS_MASK = (1 << len(PR_CLASSES)) - 1
+
+
+def pr_class_of_record(r):
+    """PageRank class of one expand record (a from-all pull)."""
+    lb = r["lb"] if r["lb"] in ("TM", "WM", "CM", "STRICT", "TWOD") else "STRICT"
+    name = "Pull-Bitmap-%s-Standalone" % lb
+    if lb in ("TM", "WM", "CM"):
+        return [c for c, _ in PR_CLASSES].index(name)
+    pk = 0 if r["kernel"] == 2 and lb == "STRICT" else r["kernel"]
+    return PR_CLASSES.index((name, pk))
 
 
 def selector_load(w, app="bfs"):
     """Install an application's learned selector weights, used by its runs
-    with selector=1 ("bfs" or "sssp")."""
+    with selector=1 ("bfs", "sssp" or "pr")."""
     if app not in SELECTOR_APPS:
         raise ValueError("no learned selector for %r" % (app,))
     w = _f32(w, MLP_NPARAM)
@@ -394,7 +411,7 @@ def mlp_train(w0, x, t, mask, epochs=200, batch=256, lr=1e-3, seed=7):
 
 # --------------------------------------------------------------- device graphs
 def run_opts(configs=None, max_levels=0, verbose=False, twc_small=0,
-             twc_large=0, profile=True, pull_kernel=0, capture=0, grid=0,
+             twc_large=0, profile=True, pull_kernel=2, capture=0, grid=0,
              selector=0):
     o = RunOpts()
     _lib.gsw_run_opts_init(C.byref(o))
@@ -599,7 +616,8 @@ class Graph:
             out.append(dict(level=r.level, dir="Pull" if r.dir else "Push",
                             fmt="Bitmap" if r.fmt else "Queue",
                             lb=LB_NAMES[r.lb], fused=bool(r.fused),
-                            sorted=bool(r.sorted), frontier=r.frontier,
+                            sorted=bool(r.sorted), kernel=int(r.kernel),
+                            frontier=r.frontier,
                             active=r.active, inactive=r.inactive, edges=r.edges,
                             verts=r.verts, ms=r.ms))
         return out

@@ -292,7 +292,7 @@ struct run_opts_t {
 // init_conf: the first level's choice (fused or not) is the network's.
 inline void enable_learned(engine_t &en, const float *w_dev, const float *w_host,
                            const graph_attr_t &attr, double n, double m,
-                           bool fromall, bool toall) {
+                           bool fromall, bool toall, bool pr_classes = false) {
   gsw_mlp::sel_dev_t &d = en.stats.mlp;
   d.w = w_dev;
   d.n = (float)n;
@@ -300,7 +300,8 @@ inline void enable_learned(engine_t &en, const float *w_dev, const float *w_host
   d.avg_deg = (float)attr.avg_deg;
   d.std_deg = (float)attr.std_deg;
   d.gini = (float)attr.GI;
-  d.mask = gsw_mlp::valid_mask(fromall, toall);
+  d.mask = pr_classes ? gsw_mlp::PR_CLASSES : gsw_mlp::valid_mask(fromall, toall);
+  en.conf.mlp_pr_classes = pr_classes;
   en.stats.mlp_host_w = w_host;
   en.conf.policy_mlp = true;
   en.conf.mlp_class = -1;

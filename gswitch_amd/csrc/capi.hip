@@ -365,18 +365,20 @@ run_opts_t to_opts(const gsw_run_opts *o) {
   r.verbose = o->verbose != 0;
   if (o->twc_small > 0) r.twc_small = o->twc_small;
   if (o->twc_large > 0) r.twc_large = o->twc_large;
-  r.mp_pull = o->pull_kernel == 0;
+  GSW_REQUIRE(o->pull_kernel >= 0 && o->pull_kernel <= 2, GSW_E_INVALID,
+              "pull_kernel must be 0, 1 or 2");
+  r.mp_pull = o->pull_kernel != 1;
   r.pb_pull = o->pull_kernel == 2;
   r.selector = o->selector;
   return r;
 }
 
 // options of traversals whose variant choice stays with the rule trees (the
-// learned selector is trained on BFS and SSSP levels)
+// learned selectors are trained on BFS, SSSP and PageRank runs)
 run_opts_t to_opts_rule(const gsw_run_opts *o) {
   run_opts_t r = to_opts(o);
   GSW_REQUIRE(r.selector == 0, GSW_E_UNSUPPORTED,
-              "the learned selector is available for BFS and SSSP only");
+              "the learned selector is available for BFS, SSSP and PageRank only");
   return r;
 }
 
@@ -415,6 +417,7 @@ void collect_records(gsw_graph *G, executor_t &ex, active_set_t &as) {
     o.lb = (int8_t)r.lb;
     o.fused = (int8_t)r.fused;
     o.sorted = (int8_t)r.sorted;
+    o.kernel = (int8_t)r.kernel;
     o.frontier = r.frontier;
     o.active = r.active;
     o.inactive = r.inactive;
@@ -452,6 +455,7 @@ gsw_tree::table_t g_tree[GSW_APPS];
 template <typename F>
 void maybe_learned(app_ctx_t<F> &c, const run_opts_t &o, const gsw_graph *G,
                    bool fromall, bool toall, int app) {
+  const bool pr = app == GSW_APP_PR;
   if (o.selector == 0) return;
   GSW_REQUIRE(o.selector == 1 || o.selector == 2, GSW_E_INVALID, "unknown selector");
   if (o.selector == 2) {
@@ -459,7 +463,7 @@ void maybe_learned(app_ctx_t<F> &c, const run_opts_t &o, const gsw_graph *G,
                 "class-tree selector requested but no tree is loaded for this "
                 "application");
     enable_learned(c.en, nullptr, nullptr, G->attr, (double)G->hg.n,
-                   (double)G->hg.m, fromall, toall);
+                   (double)G->hg.m, fromall, toall, pr);
     c.en.stats.tree = &g_tree[app];
     return;
   }
@@ -467,7 +471,7 @@ void maybe_learned(app_ctx_t<F> &c, const run_opts_t &o, const gsw_graph *G,
               "learned selector requested but no weights are loaded for this "
               "application");
   enable_learned(c.en, g_mlp_dev[app], g_mlp_host[app].data(), G->attr,
-                 (double)G->hg.n, (double)G->hg.m, fromall, toall);
+                 (double)G->hg.n, (double)G->hg.m, fromall, toall, pr);
 }
 
 app_ctx_t<BFS> &bfs_ctx(gsw_graph *G) {
@@ -1070,6 +1074,7 @@ void gsw_run_opts_init(gsw_run_opts *o) {
   o->twc_small = 32;
   o->twc_large = 2048;
   o->profile = 1;
+  o->pull_kernel = 2;
 }
 
 // ---- host ingest -----------------------------------------------------------
@@ -1751,7 +1756,7 @@ gsw_status gsw_pagerank(gsw_graph *G, float damping, int32_t iters, float eps,
     GSW_REQUIRE(G, GSW_E_INVALID, "null graph");
     GSW_REQUIRE(iters >= 0 && damping >= 0 && damping <= 1 && eps >= 0,
                 GSW_E_INVALID, "invalid PageRank parameters");
-    const run_opts_t o = to_opts_rule(opts);
+    const run_opts_t o = to_opts(opts);
     if (!G->pr) {
       G->pr = std::make_unique<app_ctx_t<PR>>();
       G->pr->as.build(G->hg.n, Normal);
@@ -1770,6 +1775,7 @@ gsw_status gsw_pagerank(gsw_graph *G, float damping, int32_t iters, float eps,
     fets.set_graph((double)G->hg.n, (double)(G->hg.m / 2), G->attr);
     view_t<GU> gv{&G->hg, G->attr, 0.0};
     init_conf(c.en.stats, fets, c.en.conf, gv, c.f);
+    maybe_learned(c, o, G, true, true, GSW_APP_PR);
 
     hipStream_t st = gsw::stream();
     H_ERR(hipEventRecord(G->ev0, st));

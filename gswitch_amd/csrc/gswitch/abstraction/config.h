@@ -72,6 +72,8 @@ struct config_t {
   // (mlp_class, written by the inspector / the fused-level selection)
   bool policy_mlp = false;
   int mlp_class = -1;
+  // the class is one of the from-all pull kernels (gsw_mlp::pr_variant_of_class)
+  bool mlp_pr_classes = false;
   // fused levels under the learned selector: the frontier size when the
   // network last chose a fused class (re-inspection waits for growth)
   bool mlp_entered_fused = false;

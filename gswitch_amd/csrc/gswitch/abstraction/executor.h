@@ -32,6 +32,8 @@
 struct level_record_t {
   int level;
   int dir, fmt, lb, fused, sorted;
+  int kernel;              // pulls: 0 length-binned / merge-path, 1 segmented,
+                           // 2 propagation blocking (pull_kernel in effect)
   long long frontier;      // input frontier length (queue) or n (bitmap)
   long long active, inactive;
   long long edges;         // edges inspected by the expand (device counter)
@@ -209,6 +211,7 @@ struct executor_t {
     rec.lb = conf.conf_lb;
     rec.fused = conf.conf_fuse_inspect;
     rec.sorted = conf.conf_sorted;
+    rec.kernel = conf.conf_dir == Pull ? (!conf.mp_pull ? 1 : conf.pb_pull ? 2 : 0) : 0;
     rec.frontier = as.get_size_host();
     rec.active = as.known_active;
     rec.inactive = as.known_inactive;

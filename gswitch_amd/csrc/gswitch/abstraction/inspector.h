@@ -89,7 +89,8 @@ struct inspector_t {
     if (need) {
       // the partial slots are zero between passes: k_stat_finish clears them
       gsw_mlp::sel_dev_t d;
-      if (conf.policy_mlp && stat.mlp.w) {
+      // (a run that keeps its first choice needs no inference after it)
+      if (conf.policy_mlp && stat.mlp.w && !conf.conf_inherit) {
         d = stat.mlp;
         d.level = (float)g.level;
       }

@@ -174,6 +174,22 @@ struct selector_t {
   // fused expand (the next level then runs the inspector), so a fused level
   // given a standalone class keeps the fused form with the class's LB.
   static void select_learned(config_t &conf) {
+    if (conf.mlp_pr_classes) {
+      // one from-all pull for every iteration of the run
+      const gsw_mlp::pr_variant_t p = gsw_mlp::pr_variant_of_class(conf.mlp_class);
+      conf.conf_fuse_inspect = conf.conf_fusion = false;
+      conf.conf_dir = Pull;
+      conf.conf_asfmt = Bitmap;
+      conf.conf_sorted = false;
+      conf.conf_lb = p.lb;
+      conf.mp_pull = p.pull_kernel != 1;
+      conf.pb_pull = p.pull_kernel == 2;
+      if (conf.force.has_direction) conf.conf_dir = conf.force.d;
+      if (conf.force.has_asfmt) conf.conf_asfmt = conf.force.asf;
+      if (conf.force.has_lb) conf.conf_lb = conf.force.lb;
+      set_target(conf);
+      return;
+    }
     const gsw_mlp::variant_t v = gsw_mlp::variant_of_class(conf.mlp_class);
     if (v.fused || conf.conf_fuse_inspect) {
       if (!conf.conf_fuse_inspect) {

@@ -42,7 +42,12 @@ namespace gsw {
 
 constexpr int PB_BLOCK = 1024;
 constexpr int PB_LDS_BYTES = 64 << 10;  // per phase: messages / accumulators
-constexpr int PB_UNROLL = 8;            // slots a lane keeps in flight
+constexpr int PB_QUADS = 2;             // aligned 4-slot quads a lane keeps in flight
+
+// four consecutive slots, loaded and stored as one access
+template <typename T> struct alignas(4 * sizeof(T)) quad_t {
+  T v[4];
+};
 
 template <typename F, typename = void> struct pb_msg {
   using type = typename F::wa_t;
@@ -110,15 +115,49 @@ __global__ void __launch_bounds__(BLOCK)
   }
 }
 
+// One step of a phase-1 tile walk: the tile's bounds, its unaligned head
+// and tail slots (one per lane) and its first PB_QUADS * 64 aligned quads
+// of 16-bit source offsets, loaded before they are needed.
+struct pb_step_t {
+  int64_t start = 0, end = 0, a0 = 0, a1 = 0, nq = 0;
+  uint16_t hs = 0, ts = 0;
+  quad_t<uint16_t> si[PB_QUADS] = {};
+};
+
+__device__ __forceinline__ int64_t readlane64(int64_t x, int j) {
+  const int lo = readlane((int)(uint32_t)x, j);
+  const int hi = readlane((int)(x >> 32), j);
+  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+__device__ __forceinline__ void pb_load_step(pb_step_t &st, const uint16_t *src,
+                                             int64_t start, int64_t len, int lane) {
+  st.start = start;
+  st.end = start + len;
+  st.a0 = MIN(st.end, (start + 3) & ~(int64_t)3);
+  st.a1 = st.a0 + ((st.end - st.a0) & ~(int64_t)3);
+  st.nq = (st.a1 - st.a0) >> 2;
+  if (lane < st.a0 - start) st.hs = src[start + lane];
+  if (lane < st.end - st.a1) st.ts = src[st.a1 + lane];
+  const quad_t<uint16_t> *sq = (const quad_t<uint16_t> *)(src + st.a0);
+#pragma unroll
+  for (int j = 0; j < PB_QUADS; ++j) {
+    const int64_t q = j * WAVE + lane;
+    if (q < st.nq) st.si[j] = sq[q];
+  }
+}
+
 // Phase 1: one workgroup per piece (chunk c, bins [b0, b1)). The chunk's
 // messages are staged in LDS once; then every wavefront takes whole tiles
-// of the piece in turn (tile b0 + w, b0 + w + waves, ...) and walks each
-// tile PB_UNROLL * 64 slots per step: consecutive lanes read consecutive
-// 16-bit source offsets and store to consecutive slots (both coalesced),
-// with PB_UNROLL independent slots in flight per lane. Tiles hold about
-// m / (B * C) edges (about 400 on R-MAT scale 22, 3% of the edges in tiles
-// shorter than a wavefront), so a tile per wavefront wastes few lanes and
-// no lane searches for its tile.
+// of the piece in turn (tile b0 + w, b0 + w + waves, ...): the bounds of
+// up to 64 of its tiles load at once (one per lane), and the source offsets
+// of the next tile load while the current tile's messages are stored, so a
+// wavefront keeps two tiles in flight. A lane moves aligned quads of slots:
+// one 8-byte load of four 16-bit source offsets, four LDS reads, one 16-byte
+// store of four messages (for 4-byte messages); the at most three slots
+// before the first aligned quad and after the last one take a lane each.
+// Tiles hold about m / (B * C) edges (about 330 on R-MAT scale 22), so a
+// tile per wavefront wastes few lanes and no lane searches for its tile.
 template <class Bd, typename MSG>
 __global__ void __launch_bounds__(PB_BLOCK) k_pb_scatter(Bd body, pb_t pb) {
   constexpr int CAP = PB_LDS_BYTES / (int)sizeof(MSG);
@@ -127,40 +166,80 @@ __global__ void __launch_bounds__(PB_BLOCK) k_pb_scatter(Bd body, pb_t pb) {
   const int c = pc[0], b0 = pc[1], b1 = pc[2];
   const int64_t lo = (int64_t)c * pb.S;
   const int w = (int)MIN((int64_t)pb.S, body.g.nvertexs - lo);
-  for (int i = threadIdx.x; i < w; i += PB_BLOCK)
-    s_msg[i] = (MSG)body.msg((int)(lo + i), 0);
-  __syncthreads();
   MSG *vals = (MSG *)pb.vals;
   const int lane = lane_id();
   const int wv = threadIdx.x >> LOG_WAVE;
   constexpr int NW = PB_BLOCK / WAVE;
   const int64_t row = (int64_t)c * pb.B;
-  for (int b = b0 + wv; b < b1; b += NW) {
-    const int64_t start = pb.cm_start[row + b];
-    const int len = pb.cm_len[row + b];
-    for (int k0 = 0; k0 < len; k0 += WAVE * PB_UNROLL) {
-      uint16_t si[PB_UNROLL];
+  const int mine = b1 - b0 > wv ? (b1 - b0 - wv + NW - 1) / NW : 0;
+  // the first tiles' bounds and source offsets load before the messages
+  // are staged
+  int64_t my_start = 0, my_len = 0;
+  pb_step_t cur, nxt;
+  auto load_batch = [&](int t0) {
+    const int cnt = MIN(WAVE, mine - t0);
+    if (lane < cnt) {
+      const int64_t t = row + b0 + wv + (int64_t)(t0 + lane) * NW;
+      my_start = pb.cm_start[t];
+      my_len = pb.cm_len[t];
+    }
+    pb_load_step(cur, pb.src, readlane64(my_start, 0), readlane64(my_len, 0), lane);
+  };
+  if (mine > 0) load_batch(0);
+  for (int i = threadIdx.x; i < w; i += PB_BLOCK)
+    s_msg[i] = (MSG)body.msg((int)(lo + i), 0);
+  __syncthreads();
+  for (int t0 = 0; t0 < mine; t0 += WAVE) {
+    const int cnt = MIN(WAVE, mine - t0);
+    if (t0 > 0) load_batch(t0);
+    for (int j = 0; j < cnt; ++j) {
+      if (j + 1 < cnt)
+        pb_load_step(nxt, pb.src, readlane64(my_start, j + 1),
+                     readlane64(my_len, j + 1), lane);
+      if (lane < cur.a0 - cur.start) vals[cur.start + lane] = s_msg[cur.hs];
+      if (lane < cur.end - cur.a1) vals[cur.a1 + lane] = s_msg[cur.ts];
+      quad_t<MSG> *vq = (quad_t<MSG> *)(vals + cur.a0);
 #pragma unroll
-      for (int j = 0; j < PB_UNROLL; ++j) {
-        const int k = k0 + j * WAVE + lane;
-        si[j] = k < len ? pb.src[start + k] : 0;
-      }
+      for (int k = 0; k < PB_QUADS; ++k) {
+        const int64_t q = k * WAVE + lane;
+        if (q < cur.nq) {
+          quad_t<MSG> o;
 #pragma unroll
-      for (int j = 0; j < PB_UNROLL; ++j) {
-        const int k = k0 + j * WAVE + lane;
-        if (k < len) vals[start + k] = s_msg[si[j]];
+          for (int r = 0; r < 4; ++r) o.v[r] = s_msg[cur.si[k].v[r]];
+          vq[q] = o;
+        }
       }
+      // the rest of a long tile
+      const quad_t<uint16_t> *sq = (const quad_t<uint16_t> *)(pb.src + cur.a0);
+      for (int64_t q0 = WAVE * PB_QUADS; q0 < cur.nq; q0 += WAVE * PB_QUADS) {
+        quad_t<uint16_t> si[PB_QUADS] = {};
+#pragma unroll
+        for (int k = 0; k < PB_QUADS; ++k) {
+          const int64_t q = q0 + k * WAVE + lane;
+          if (q < cur.nq) si[k] = sq[q];
+        }
+#pragma unroll
+        for (int k = 0; k < PB_QUADS; ++k) {
+          const int64_t q = q0 + k * WAVE + lane;
+          if (q < cur.nq) {
+            quad_t<MSG> o;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) o.v[r] = s_msg[si[k].v[r]];
+            vq[q] = o;
+          }
+        }
+      }
+      cur = nxt;
     }
   }
 }
 
 // Phase 2: one workgroup per piece (bin b, slots [e0, e1), whole bin or
 // part of a one-target bin). Slots are combined with compAtomic into LDS
-// accumulators (native ds_add_f64 for PageRank). Measured on R-MAT scale 22
-// and kept plain: combining runs of one target in registers first (a
-// segmented shuffle reduction, also only for 64-slot groups of at most 8
-// runs) cost more in shuffles than it saved in atomics (852 and 691 us per
-// launch against 554 us).
+// accumulators (native ds_add_f64 for PageRank), after the slots of one
+// target within an aligned quad are combined in registers. (A segmented
+// shuffle reduction over 64-slot groups cost more in shuffles than it saved
+// in atomics.)
 template <class Bd, typename MSG>
 __global__ void __launch_bounds__(PB_BLOCK) k_pb_gather(Bd body, pb_t pb) {
   using WA = typename Bd::WA;
@@ -176,22 +255,61 @@ __global__ void __launch_bounds__(PB_BLOCK) k_pb_gather(Bd body, pb_t pb) {
   const MSG *vals = (const MSG *)pb.vals;
   const int lane = lane_id();
   if (whole) {
+    // aligned quads of slots (one 16-byte message load and one 8-byte
+    // offset load per quad for 4-byte messages), the next step's quads
+    // loading while the current step's are combined (the first step's
+    // before the accumulators are initialised); the at most three slots
+    // before the first aligned quad and after the last one take a thread each
+    const int t = threadIdx.x;
+    const int64_t a0 = MIN(e1, (e0 + 3) & ~(int64_t)3);
+    const int64_t a1 = a0 + ((e1 - a0) & ~(int64_t)3);
+    const int64_t nq = (a1 - a0) >> 2;
+    const quad_t<MSG> *vq = (const quad_t<MSG> *)(vals + a0);
+    const quad_t<uint16_t> *dq = (const quad_t<uint16_t> *)(pb.dst + a0);
+    constexpr int64_t STEP = (int64_t)PB_BLOCK * PB_QUADS;
+    quad_t<MSG> mv[PB_QUADS], mn[PB_QUADS];
+    quad_t<uint16_t> di[PB_QUADS], dn[PB_QUADS];
+    auto load = [&](int64_t q0, quad_t<MSG> *m, quad_t<uint16_t> *d) {
+#pragma unroll
+      for (int j = 0; j < PB_QUADS; ++j) {
+        const int64_t q = q0 + (int64_t)j * PB_BLOCK + t;
+        const int64_t qq = q < nq ? q : nq - 1;
+        m[j] = vq[qq];
+        d[j] = dq[qq];
+      }
+    };
+    if (nq > 0) load(0, mv, di);
     for (int i = threadIdx.x; i < w; i += PB_BLOCK) s_acc[i] = *body.f.wa_of(lo + i);
     __syncthreads();
-    for (int64_t k0 = e0; k0 < e1; k0 += (int64_t)PB_BLOCK * PB_UNROLL) {
-      MSG mv[PB_UNROLL];
-      uint16_t di[PB_UNROLL];
+    if (t < a0 - e0) body.f.compAtomic(&s_acc[pb.dst[e0 + t]], (WA)vals[e0 + t], body.g);
+    if (t < e1 - a1) body.f.compAtomic(&s_acc[pb.dst[a1 + t]], (WA)vals[a1 + t], body.g);
+    for (int64_t q0 = 0; q0 < nq; q0 += STEP) {
+      if (q0 + STEP < nq) load(q0 + STEP, mn, dn);
 #pragma unroll
-      for (int j = 0; j < PB_UNROLL; ++j) {
-        const int64_t k = k0 + (int64_t)j * PB_BLOCK + threadIdx.x;
-        const int64_t kk = k < e1 ? k : e1 - 1;
-        mv[j] = vals[kk];
-        di[j] = pb.dst[kk];
+      for (int j = 0; j < PB_QUADS; ++j)
+        if (q0 + (int64_t)j * PB_BLOCK + t < nq) {
+          // slots of one target are mostly consecutive (a target's in-edges
+          // from one source chunk form a run), so a quad's slots of one
+          // target combine in registers and cost one LDS atomic
+          WA acc = (WA)mv[j].v[0];
+          int d = di[j].v[0];
+#pragma unroll
+          for (int r = 1; r < 4; ++r) {
+            if (di[j].v[r] == d) {
+              body.f.comp(&acc, (WA)mv[j].v[r], body.g);
+            } else {
+              body.f.compAtomic(&s_acc[d], acc, body.g);
+              d = di[j].v[r];
+              acc = (WA)mv[j].v[r];
+            }
+          }
+          body.f.compAtomic(&s_acc[d], acc, body.g);
+        }
+#pragma unroll
+      for (int j = 0; j < PB_QUADS; ++j) {
+        mv[j] = mn[j];
+        di[j] = dn[j];
       }
-#pragma unroll
-      for (int j = 0; j < PB_UNROLL; ++j)
-        if (k0 + (int64_t)j * PB_BLOCK + threadIdx.x < e1)
-          body.f.compAtomic(&s_acc[di[j]], (WA)mv[j], body.g);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < w; i += PB_BLOCK) {

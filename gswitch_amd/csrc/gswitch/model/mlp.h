@@ -81,6 +81,32 @@ inline int class_of(Direction dir, ASFmt fmt, bool sorted, bool fused, LB lb) {
   return g * NLB + index_of_lb(lb);
 }
 
+// PageRank-like runs (from-all and to-all: every iteration pulls every
+// vertex in the same way) choose once per run among the from-all pulls of a
+// bitmap frontier; their classes are the network's first NPR outputs:
+//   0 TM   1 WM   2 CM   3 STRICT, merge-path walk   4 STRICT, segmented
+//   5 TWOD, length-binned source slices   6 TWOD, segmented slices
+//   7 TWOD, propagation blocking
+// (pull_kernel of each: the from-all pull kernel option of gsw_run_opts).
+constexpr int NPR = 8;
+constexpr uint32_t PR_CLASSES = (1u << NPR) - 1;
+struct pr_variant_t {
+  LB lb;
+  int pull_kernel;
+};
+__host__ __device__ inline pr_variant_t pr_variant_of_class(int c) {
+  switch (c) {
+    case 0: return {TM, 2};
+    case 1: return {WM, 2};
+    case 2: return {CM, 2};
+    case 3: return {STRICT, 0};
+    case 4: return {STRICT, 1};
+    case 5: return {TWOD, 0};
+    case 6: return {TWOD, 1};
+    default: return {TWOD, 2};
+  }
+}
+
 // classes that make sense for an algorithm: a from-all and to-all one
 // (PageRank-like) has no fused form, as init_conf keeps it standalone
 inline uint32_t valid_mask(bool fromall, bool toall) {

@@ -1,5 +1,5 @@
-"""Training pipeline of the learned variant selectors (BFS and SSSP, one
-network each).
+"""Training pipeline of the learned variant selectors (BFS, SSSP and
+PageRank, one network each).
 
 1. `collect`: the engine times all 30 variant classes at every probed level
    of seeded synthetic graphs (gsw_explore_bfs / gsw_explore_sssp: snapshot,
@@ -18,11 +18,21 @@ network each).
    and every leaf class is chosen to minimise the summed time of the
    levels it covers (selector 2 of the engine).
 
-Run as a module on a GPU machine (--app bfs | sssp, default bfs):
+PageRank (--app pr) pulls every vertex the same way in every iteration,
+so it chooses once per run: a sample is one graph, its features are those
+of the run's first level (everything active) and its timings are the
+device time per iteration of each of the 8 from-all pull classes
+(gswitch_amd.PR_CLASSES: TM, WM, CM, STRICT as a merge-path walk or a
+segmented reduction, TWOD as length-binned slices, segmented slices or
+propagation blocking); the network's other outputs are masked out.
+
+Run as a module on a GPU machine (--app bfs | sssp | pr, default bfs):
     python -m gswitch_amd.learn collect --app A --set train --out DIR
     python -m gswitch_amd.learn train --app A --data DIR/train.npz --out DIR
     python -m gswitch_amd.learn collect --app A --set eval --out DIR
     python -m gswitch_amd.learn evaluate --app A --data DIR/eval.npz --model M --out DIR
+    python -m gswitch_amd.learn evaluate --app pr --model M --out DIR   (PageRank:
+        the held-out graphs are timed per class during the end-to-end pass)
     python -m gswitch_amd.learn fit-trees --app A --data DIR/train.npz --out DIR
 """
 import argparse
@@ -36,7 +46,11 @@ import numpy as np
 import gswitch_amd as gsw
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-APPS = ("bfs", "sssp")
+APPS = ("bfs", "sssp", "pr")
+# classes each application's selector may pick
+MASK = {"bfs": (1 << 30) - 1, "sssp": (1 << 30) - 1, "pr": gsw.PR_CLASS_MASK}
+PR_ITERS = 3          # iterations per timed PageRank run while collecting
+PR_SLOW = 50.0        # a class this many times slower than the best is timed once
 
 
 def model_path(app="bfs"):
@@ -161,7 +175,60 @@ def rule_trajectory(g, root, app="bfs"):
     return {r["level"] - 1: class_of_record(r) for r in g.records()}
 
 
+def pagerank_features(h):
+    """The 12 inputs of a PageRank run's first level (every vertex active,
+    push workload m), as the engine computes them for its choice."""
+    st = gsw.degree_stats(h.row_ptr)
+    n, m = float(h.n), float(h.m)
+    return gsw.mlp_features([n, m, st["avg"], st["std"], st["GI"], 0.0, n, 0.0, m,
+                             0.0, m])
+
+
+def time_pr_classes(g, reps=2, iters=PR_ITERS, slow=PR_SLOW):
+    """Device ms per PageRank iteration of each class (PR_CLASSES order).
+    Each class runs one warm-up iteration (which also builds its layout)
+    and then `reps` runs of `iters` iterations; a class whose warm-up is
+    `slow` times slower than the best time so far keeps the warm-up time."""
+    t = np.full(len(gsw.PR_CLASSES), np.inf)
+    for c in (7, 5, 6, 3, 4, 2, 1, 0):
+        cfg, pk = gsw.PR_CLASSES[c]
+        g.pagerank(0.85, 1, configs=cfg, pull_kernel=pk, out=False, profile=False)
+        w1 = g.last_stats.device_ms
+        best = t[np.isfinite(t)].min() if np.isfinite(t).any() else np.inf
+        if w1 > slow * best:
+            t[c] = w1
+            continue
+        for _ in range(reps):
+            g.pagerank(0.85, iters, configs=cfg, pull_kernel=pk, out=False,
+                       profile=False)
+            t[c] = min(t[c], g.last_stats.device_ms / iters)
+    return t
+
+
+def collect_pr(specs, reps=2, log=print):
+    rows = dict(features=[], ms=[], rule=[], level=[], graph=[])
+    t0 = time.time()
+    for gi, spec in enumerate(specs):
+        h = build(spec)
+        g = gsw.Graph(h)
+        g.pagerank(0.85, 1, out=False)
+        rule = gsw.pr_class_of_record(g.records()[0])
+        ms = time_pr_classes(g, reps)
+        g.close()
+        rows["features"].append(pagerank_features(h)[None])
+        rows["ms"].append(ms[None].astype(np.float32))
+        rows["rule"].append(np.array([rule], np.int32))
+        rows["level"].append(np.zeros(1, np.int32))
+        rows["graph"].append(np.full(1, gi, np.int32))
+        log("graph %d/%d %s n=%d m=%d rule %d best %d (%.3f ms/it) (%.0fs)" % (
+            gi + 1, len(specs), spec["kind"], h.n, h.m, rule, int(np.argmin(ms)),
+            float(ms.min()), time.time() - t0))
+    return {k: np.concatenate(v) for k, v in rows.items()}
+
+
 def collect(specs, reps=2, max_levels=32, log=print, app="bfs"):
+    if app == "pr":
+        return collect_pr(specs, reps, log)
     rows = dict(features=[], ms=[], rule=[], level=[], graph=[])
     t0 = time.time()
     for gi, spec in enumerate(specs):
@@ -191,9 +258,16 @@ def collect(specs, reps=2, max_levels=32, log=print, app="bfs"):
 
 
 def targets(ms):
+    """Regression targets and mask over the network's 30 outputs; an
+    application with fewer classes leaves the rest masked out."""
     best = ms.min(axis=1, keepdims=True)
-    y = np.log2(np.maximum(ms, 1e-6) / np.maximum(best, 1e-6))
-    return np.minimum(y, CLIP).astype(np.float32), np.ones_like(y, np.float32)
+    y = np.minimum(np.log2(np.maximum(ms, 1e-6) / np.maximum(best, 1e-6)), CLIP)
+    k = ms.shape[1]
+    yt = np.zeros((len(ms), gsw.MLP_NOUT), np.float32)
+    mk = np.zeros((len(ms), gsw.MLP_NOUT), np.float32)
+    yt[:, :k] = y
+    mk[:, :k] = 1.0
+    return yt, mk
 
 
 def init_weights(seed=1):
@@ -324,6 +398,38 @@ def load_model(path=MODEL_PATH):
 FIXED = ["Push-Queue-WM-Standalone", "Push-Queue-WM-Fused", None]
 
 
+def end_to_end_pr(specs, w, iters=20, reps=3, log=print):
+    """Median device time of a PageRank run of `iters` iterations per graph
+    under the rule trees and the network, and the exhaustive per-class
+    optimum (results within 1e-5 relative of the rule's)."""
+    gsw.selector_load(w, app="pr")
+    out = []
+    for spec in specs:
+        h = build(spec)
+        g = gsw.Graph(h)
+        ref = g.pagerank(0.85, iters).astype(np.float64)
+        rec = dict(kind=spec["kind"], n=h.n, m=h.m)
+        for name, kw in [("rule", {}), ("mlp", dict(selector=1))]:
+            pr = g.pagerank(0.85, iters, **kw)
+            err = float(np.max(np.abs(pr - ref) / np.abs(ref))) if h.n else 0.0
+            assert err <= 1e-5, (name, spec, err)
+            rec[name + "_class"] = gsw.pr_class_of_record(g.records()[0])
+            ts = []
+            for _ in range(reps):
+                g.pagerank(0.85, iters, out=False, profile=False, **kw)
+                ts.append(g.last_stats.device_ms)
+            rec[name] = float(np.median(ts))
+        per_it = time_pr_classes(g)
+        rec["classes_ms_per_iter"] = [float(x) for x in per_it]
+        rec["best_class"] = int(np.argmin(per_it))
+        g.close()
+        log("%s n=%d rule %.3f (class %d) mlp %.3f (class %d) best class %d: %.4f ms/it" % (
+            spec["kind"], h.n, rec["rule"], rec["rule_class"], rec["mlp"],
+            rec["mlp_class"], rec["best_class"], per_it.min()))
+        out.append(rec)
+    return out
+
+
 def end_to_end(specs, w, reps=3, log=print, app="bfs"):
     """Median device time per graph under the rule trees, the network and
     fixed strategies (results checked identical)."""
@@ -393,24 +499,40 @@ def main(argv=None):
             d = {k: z[k] for k in z.files}
         w, curve = train(d, epochs=a.epochs)
         log("loss %.4f -> %.4f" % (curve[0], curve[-1]))
-        cls = gsw.mlp_select(w, d["features"])
+        cls = gsw.mlp_select(w, d["features"], mask=MASK[a.app])
         log("train regret: mlp %.3f rule %.3f" % (per_level_regret(d["ms"], cls),
                                                   per_level_regret(d["ms"], d["rule"])))
         save_model(os.path.join(a.out, "selector_%s.npz" % a.app), w,
                    dict(app=a.app, samples=int(len(d["ms"])), epochs=a.epochs,
                         loss_first=float(curve[0]), loss_last=float(curve[-1])))
+    elif a.app == "pr" and not a.data:
+        # one run per graph: the end-to-end pass times every class as well
+        w = load_model(a.model or model_path(a.app))
+        e2e = end_to_end_pr(eval_specs(), w, log=log)
+        ms = np.array([r["classes_ms_per_iter"] for r in e2e])
+        res = dict(graphs=len(e2e),
+                   regret_mlp=per_level_regret(ms, np.array([r["mlp_class"] for r in e2e])),
+                   regret_rule=per_level_regret(ms, np.array([r["rule_class"] for r in e2e])),
+                   end_to_end=e2e)
+        log("eval per-graph regret: mlp %.3f rule %.3f" % (res["regret_mlp"],
+                                                           res["regret_rule"]))
+        with open(os.path.join(a.out, "eval.json"), "w") as f:
+            json.dump(res, f, indent=1)
     else:
         with np.load(a.data, allow_pickle=False) as z:
             d = {k: z[k] for k in z.files}
         w = load_model(a.model or model_path(a.app))
-        cls = gsw.mlp_select(w, d["features"])
+        cls = gsw.mlp_select(w, d["features"], mask=MASK[a.app])
         res = dict(levels=int(len(d["ms"])),
                    regret_mlp=per_level_regret(d["ms"], cls),
                    regret_rule=per_level_regret(d["ms"], d["rule"]))
         log("eval per-level regret: mlp %.3f rule %.3f" % (res["regret_mlp"],
                                                            res["regret_rule"]))
-        res["end_to_end"] = end_to_end(json.loads(str(d["specs"])), w, log=log,
-                                       app=a.app)
+        specs = json.loads(str(d["specs"]))
+        if a.app == "pr":
+            res["end_to_end"] = end_to_end_pr(specs, w, log=log)
+        else:
+            res["end_to_end"] = end_to_end(specs, w, log=log, app=a.app)
         with open(os.path.join(a.out, "eval.json"), "w") as f:
             json.dump(res, f, indent=1)
 

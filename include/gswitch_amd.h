@@ -120,8 +120,11 @@ typedef struct {
   int32_t twc_small;  /* TWC bins: degree < twc_small -> thread */
   int32_t twc_large;  /*           degree < twc_large -> wavefront */
   int32_t profile;    /* time every expand with hipEvents */
-  int32_t pull_kernel; /* from-all pulls: 0 merge-path walk, 1 segmented
-                          reduction, 2 propagation blocking */
+  int32_t pull_kernel; /* from-all pulls of unweighted graphs: 0 TWOD as
+                          length-binned source slices, STRICT as a merge-path
+                          walk; 1 both as segmented reductions; 2 (default)
+                          TWOD by propagation blocking (LDS-sized source
+                          chunks and target bins), STRICT as in 0 */
   int32_t capture;     /* gsw_bfs_multi: >0 = hipGraph-captured fused level loop,
                           2*capture levels per replay; 0 = per-level host loop */
   int32_t grid;        /* gsw_bfs_multi captured loop: workgroups per launch
@@ -148,7 +151,8 @@ typedef struct {
   int8_t lb;     /* 0 TM, 1 WM, 2 CM, 3 STRICT, 4 TWOD, 5 ELB, 6 TWC */
   int8_t fused;  /* 1 if inspect+filter were fused into the expand */
   int8_t sorted; /* queue format: 1 sorted, 0 unsorted */
-  int8_t pad[3];
+  int8_t kernel; /* pulls: the from-all pull kernel in effect (gsw_run_opts.pull_kernel) */
+  int8_t pad[2];
   int64_t frontier; /* input frontier length */
   int64_t active;   /* active vertices (edges for CC) at this level */
   int64_t inactive;
@@ -300,13 +304,14 @@ gsw_status gsw_tree_eval(const char *name, const double *fv, int32_t nfv,
 /* Applications with a learned selector of their own. */
 #define GSW_APP_BFS 0
 #define GSW_APP_SSSP 1
-#define GSW_APPS 2
+#define GSW_APP_PR 2  /* classes: the from-all pull kernels, chosen once per run */
+#define GSW_APPS 3
 /* Install the weights that `app`'s runs use when gsw_run_opts.selector == 1. */
 gsw_status gsw_selector_load_app(int32_t app, const float *w, int32_t nparam);
 /* = gsw_selector_load_app(GSW_APP_BFS, w, nparam). */
 gsw_status gsw_selector_load(const float *w, int32_t nparam);
 /* Install the class tree that `app`'s runs use when gsw_run_opts.selector
- * == 2: the reference's offline-trained decision trees (src/model/*.h,
+ * == 2: the reference's offline-trained decision trees (src/model/select_*.h,
  * trained from feature_t::record dumps, README.md:13) retrained for this
  * engine, mapping the 12 inputs of gsw_mlp_features straight to a variant
  * class. Node i is internal if feat[i] >= 0 (x[feat[i]] <= thr[i] -> left[i],

@@ -137,17 +137,19 @@ def test_learned_policy_bit_exact(gsw, orc, name, make):
 def test_learned_policy_errors(gsw):
     h = gsw.rmat(10, 8, seed=3, weighted=True)
     g = gsw.Graph(h)
-    with pytest.raises(gsw.GswError):  # no learned selector for CC / PageRank
+    with pytest.raises(gsw.GswError):  # no learned selector for CC
         g.cc(selector=1)
-    with pytest.raises(gsw.GswError):
-        g.pagerank(0.85, 3, selector=1)
+    with pytest.raises(gsw.GswError):  # no such selector
+        g.pagerank(0.85, 3, selector=3)
+    with pytest.raises(gsw.GswError):  # no such pull kernel
+        g.pagerank(0.85, 3, pull_kernel=3)
     with pytest.raises(ValueError):
         gsw.selector_load(np.zeros(10, np.float32))
     with pytest.raises(ValueError):
         gsw.selector_load(np.zeros(NP, np.float32), app="pagerank")
     bad = np.zeros(NP, np.float32)
     assert gsw._lib.gsw_selector_load(bad.ctypes.data, 10) == 1  # GSW_E_INVALID
-    assert gsw._lib.gsw_selector_load_app(2, bad.ctypes.data, NP) == 1
+    assert gsw._lib.gsw_selector_load_app(3, bad.ctypes.data, NP) == 1
     bad[7] = np.nan
     assert gsw._lib.gsw_selector_load(bad.ctypes.data, NP) == 1
 
@@ -218,3 +220,65 @@ def test_trained_model_file(gsw, orc):
     g = gsw.Graph(h)
     root = h.max_degree_vertex()
     assert np.array_equal(g.bfs(root, selector=1), orc.bfs(h.row_ptr, h.col, root))
+
+
+PR_GRAPHS = [("rmat", lambda gsw: gsw.rmat(12, 16, seed=5)),
+             ("grid", lambda gsw: gsw.grid2d(60, 60, 0.1, 0.01, 3)),
+             ("er", lambda gsw: gsw.erdos_renyi(5000, 6.0, 9))]
+
+
+def pr_max_rel(a, b):
+    return float(np.max(np.abs(a.astype(np.float64) - b) / np.abs(b)))
+
+
+@pytest.mark.parametrize("name,make", PR_GRAPHS)
+def test_learned_pagerank(gsw, orc, name, make):
+    """PageRank under a learned network picks one from-all pull class for
+    the whole run: the class the network gives the host-side features of
+    the first level (every vertex active), and the ranks stay within 1e-5
+    of the FP64 oracle whatever class that is."""
+    from gswitch_amd import learn
+    h = make(gsw)
+    g = gsw.Graph(h)
+    ref = orc.pagerank(h.row_ptr, h.col, 0.85, 30)
+    x = learn.pagerank_features(h)
+    seen = set()
+    for seed in range(12):
+        w = learn.init_weights(seed)
+        w[4992:6912] *= 30.0  # spread the outputs so that seeds differ
+        want = int(gsw.mlp_select(w, x, mask=gsw.PR_CLASS_MASK)[0])
+        gsw.selector_load(w, app="pr")
+        pr = g.pagerank(0.85, 30, selector=1)
+        recs = g.records()
+        got = {gsw.pr_class_of_record(r) for r in recs}
+        assert got == {want} and len(recs) > 0, (seed, got, want)
+        assert pr_max_rel(pr, ref) <= 1e-5, (seed, want)
+        seen.add(want)
+    assert len(seen) >= 2
+
+
+@pytest.mark.parametrize("cls", range(8))
+def test_pagerank_class_tree(gsw, orc, cls):
+    """A one-leaf class tree pins every PageRank class (selector 2)."""
+    h = gsw.rmat(11, 16, seed=6)
+    g = gsw.Graph(h)
+    gsw.selector_load_tree(dict(feat=[-1], thr=[0.0], left=[-1], right=[-1],
+                                cls=[cls]), app="pr")
+    pr = g.pagerank(0.85, 20, selector=2)
+    assert {gsw.pr_class_of_record(r) for r in g.records()} == {cls}
+    assert pr_max_rel(pr, orc.pagerank(h.row_ptr, h.col, 0.85, 20)) <= 1e-5
+
+
+def test_trained_pagerank_model_file(gsw, orc):
+    from gswitch_amd import learn
+    path = learn.model_path("pr")
+    if not os.path.exists(path):
+        pytest.skip("no trained PageRank selector committed")
+    w = learn.load_model(path)
+    gsw.selector_load(w, app="pr")
+    for h in (gsw.rmat(14, 16, seed=11), gsw.grid2d(120, 120, 0.1, 0.01, 4)):
+        g = gsw.Graph(h)
+        pr = g.pagerank(0.85, 50, selector=1)
+        want = int(gsw.mlp_select(w, learn.pagerank_features(h), mask=gsw.PR_CLASS_MASK)[0])
+        assert {gsw.pr_class_of_record(r) for r in g.records()} == {want}
+        assert pr_max_rel(pr, orc.pagerank(h.row_ptr, h.col, 0.85, 50)) <= 1e-5

@@ -34,9 +34,12 @@ def test_class_numbering():
 def test_targets_and_regret():
     ms = np.array([[1.0, 2.0, 4.0, 1000.0], [3.0, 1.5, 1.5, 6.0]], np.float32)
     y, m = learn.targets(ms)
-    np.testing.assert_allclose(y[0], [0, 1, 2, learn.CLIP], atol=1e-6)
-    np.testing.assert_allclose(y[1], [1, 0, 0, 2], atol=1e-6)
-    assert np.all(m == 1)
+    # four classes fill the first four of the network's 30 outputs; the
+    # rest are masked out of the loss
+    assert y.shape == m.shape == (2, gsw.MLP_NOUT)
+    np.testing.assert_allclose(y[0, :4], [0, 1, 2, learn.CLIP], atol=1e-6)
+    np.testing.assert_allclose(y[1, :4], [1, 0, 0, 2], atol=1e-6)
+    assert np.all(m[:, :4] == 1) and np.all(m[:, 4:] == 0) and np.all(y[:, 4:] == 0)
     best = ms.argmin(axis=1)
     assert learn.per_level_regret(ms, best) == pytest.approx(1.0)
     # choosing class 0 everywhere: (1 + 3) / (1 + 1.5)
@@ -92,14 +95,15 @@ def test_committed_training_sets(orc, app):
     with np.load(path, allow_pickle=False) as z:
         d = {k: z[k] for k in z.files}
     n = len(d["ms"])
-    assert d["features"].shape == (n, 12) and d["ms"].shape == (n, 30)
+    k = 8 if app == "pr" else 30  # PageRank: the from-all pull classes
+    assert d["features"].shape == (n, 12) and d["ms"].shape == (n, k)
     assert np.all(d["ms"] > 0) and np.isfinite(d["features"]).all()
-    assert np.all((d["rule"] >= 0) & (d["rule"] < 30))
+    assert np.all((d["rule"] >= 0) & (d["rule"] < k))
     assert len(np.unique(d["graph"])) == len(json.loads(str(d["specs"]))) == 300
     # the committed network's choices (FP64 oracle network) cost less than
     # the rule's over its own training levels
     w = learn.load_model(learn.model_path(app))
-    choice = orc.mlp_forward(w, d["features"]).argmin(axis=1)
+    choice = orc.mlp_forward(w, d["features"])[:, :k].argmin(axis=1)
     r_mlp = learn.per_level_regret(d["ms"], choice)
     r_rule = learn.per_level_regret(d["ms"], d["rule"])
     assert r_mlp < r_rule and r_mlp < 1.2, (app, r_mlp, r_rule)
@@ -128,3 +132,38 @@ def test_fit_tree_closed_forms():
     ms2[:, 7] = 1.0
     t2 = learn.fit_tree(X[:50], ms2)
     assert t2["feat"] == [-1] and t2["cls"] == [7]
+
+
+def test_pagerank_classes():
+    """The PageRank classes are the from-all pulls of a bitmap frontier,
+    each (variant, pull kernel) once; records map back to their class."""
+    assert len(gsw.PR_CLASSES) == 8 and len(set(gsw.PR_CLASSES)) == 8
+    assert gsw.PR_CLASS_MASK == 0xFF
+    assert learn.MASK["pr"] == 0xFF and learn.MASK["bfs"] == (1 << 30) - 1
+
+    def rec(lb, kernel):
+        return dict(dir="Pull", fmt="Bitmap", lb=lb, kernel=kernel, fused=False,
+                    sorted=False)
+    for c, (cfg, pk) in enumerate(gsw.PR_CLASSES):
+        lb = cfg.split("-")[2]
+        assert gsw.pr_class_of_record(rec(lb, pk)) == c
+    # TM/WM/CM do not depend on the pull kernel; STRICT under propagation
+    # blocking keeps the merge-path walk
+    assert gsw.pr_class_of_record(rec("WM", 0)) == 1
+    assert gsw.pr_class_of_record(rec("STRICT", 2)) == 3
+    assert gsw.pr_class_of_record(rec("ELB", 1)) == 4
+
+
+def test_pagerank_features_are_the_first_level_inputs():
+    """A PageRank run's first level has every vertex active: the inputs are
+    the graph's, with level 0 and the whole edge count as push workload."""
+    h = gsw.rmat(10, 8, seed=3)
+    x = learn.pagerank_features(h)
+    st = gsw.degree_stats(h.row_ptr)
+    n, m = h.n, h.m
+    assert x[0] == pytest.approx(np.log2(1 + n) / 24, rel=1e-6)
+    assert x[1] == pytest.approx(np.log2(1 + m) / 32, rel=1e-6)
+    assert x[2] == pytest.approx(np.log2(1 + st["avg"]) / 8, rel=1e-6)
+    assert x[5] == 0 and x[6] == pytest.approx(1.0) and x[7] == 0
+    assert x[8] == pytest.approx(np.log2(1 + m) / 32, rel=1e-6)
+    assert x[9] == 0
