@@ -119,6 +119,7 @@ _SIGS = {
                                      C.POINTER(C.c_double)]),
     "gsw_selector_load": (C.c_int32, [_P, C.c_int32]),
     "gsw_selector_load_app": (C.c_int32, [C.c_int32, _P, C.c_int32]),
+    "gsw_last_class": (C.c_int32, [_P, _P]),
     "gsw_selector_load_tree": (C.c_int32, [C.c_int32, C.c_int32, _P, _P, _P, _P, _P]),
     "gsw_mlp_features": (C.c_int32, [_P, _P]),
     "gsw_mlp_forward": (C.c_int32, [_P, _P, C.c_int64, _P]),
@@ -313,7 +314,7 @@ def _f32(a, n=None):
     return a
 
 
-SELECTOR_APPS = {"bfs": 0, "sssp": 1, "pr": 2}
+SELECTOR_APPS = {"bfs": 0, "sssp": 1, "pr": 2, "cc": 3}
 # PageRank's learned classes (the network's first 8 outputs): one from-all
 # pull for the whole run, as (forced variant, pull_kernel)
 PR_CLASSES = [("Pull-Bitmap-TM-Standalone", 2), ("Pull-Bitmap-WM-Standalone", 2),
@@ -321,6 +322,11 @@ PR_CLASSES = [("Pull-Bitmap-TM-Standalone", 2), ("Pull-Bitmap-WM-Standalone", 2)
               ("Pull-Bitmap-STRICT-Standalone", 1), ("Pull-Bitmap-TWOD-Standalone", 0),
               ("Pull-Bitmap-TWOD-Standalone", 1), ("Pull-Bitmap-TWOD-Standalone", 2)]
 PR_CLASS_MASK = (1 << len(PR_CLASSES)) - 1
+# CC's learned classes (the first 4 outputs): sampled hooking (the default
+# run), or the edge-centric level loop with the rule's frontier formats, a
+# bitmap or an edge queue, as forced configs
+CC_CLASSES = [None, "x-x-x-x", "x-Bitmap-x-x", "x-Queue-x-x"]
+CC_CLASS_MASK = (1 << len(CC_CLASSES)) - 1
 
 
 def pr_class_of_record(r):
@@ -335,7 +341,7 @@ def pr_class_of_record(r):
 
 def selector_load(w, app="bfs"):
     """Install an application's learned selector weights, used by its runs
-    with selector=1 ("bfs", "sssp" or "pr")."""
+    with selector=1 ("bfs", "sssp", "pr" or "cc")."""
     if app not in SELECTOR_APPS:
         raise ValueError("no learned selector for %r" % (app,))
     w = _f32(w, MLP_NPARAM)
@@ -605,6 +611,13 @@ class Graph:
                                      C.byref(ms)))
         return dict(lb=LB_NAMES[lb.value], twc_small=s.value, twc_large=l.value,
                     grid=g.value, ms=ms.value)
+
+    def last_class(self):
+        """Class chosen by the learned selector in the latest PageRank or CC
+        run (PR_CLASSES / CC_CLASSES index), -1 if none."""
+        c = C.c_int32()
+        _check(_lib.gsw_last_class(self._h, C.byref(c)))
+        return c.value
 
     def records(self):
         cap = max(1, self.last_stats.n_records if self.last_stats else 1 << 16)

@@ -199,6 +199,7 @@ struct gsw_graph {
   int32_t *h_flag = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<gsw_level_record> records;
+  int32_t last_class = -1;  // per-run learned choice (PageRank, CC)
   // 64-source BFS state (allocated on first use)
   uint64_t *ms_buf = nullptr;  // visit | A | B, n each
   gsw::ms_counters_t *ms_cnt = nullptr;
@@ -472,6 +473,41 @@ void maybe_learned(app_ctx_t<F> &c, const run_opts_t &o, const gsw_graph *G,
               "application");
   enable_learned(c.en, g_mlp_dev[app], g_mlp_host[app].data(), G->attr,
                  (double)G->hg.n, (double)G->hg.m, fromall, toall, pr);
+}
+
+// Host-side choice for an application that chooses once per run from the
+// inputs of an everything-active first level (n, m, degree statistics,
+// level 0, all n vertices active, push workload m), as the engine's first
+// round of a from-all run computes them. -1: a class tree's leaf outside
+// `mask` (the caller keeps its default).
+int choose_once(const gsw_graph *G, int app, int selector, uint32_t mask) {
+  GSW_REQUIRE(selector == 1 || selector == 2, GSW_E_INVALID, "unknown selector");
+  gsw_mlp::counts_t c;
+  c.n = (float)G->hg.n;
+  c.m = (float)G->hg.m;
+  c.avg_deg = (float)G->attr.avg_deg;
+  c.std_deg = (float)G->attr.std_deg;
+  c.gini = (float)G->attr.GI;
+  c.level = 0.f;
+  c.active = (float)G->hg.n;
+  c.inactive = 0.f;
+  c.push_wl = (float)G->hg.m;
+  c.pull_wl = 0.f;
+  c.max_deg_active = (float)G->hg.m;
+  float x[gsw_mlp::NIN], y[gsw_mlp::NOUT];
+  gsw_mlp::features(c, x);
+  if (selector == 2) {
+    GSW_REQUIRE(!g_tree[app].empty(), GSW_E_INVALID,
+                "class-tree selector requested but no tree is loaded for this "
+                "application");
+    const int k = g_tree[app].eval(x);
+    return (k >= 0 && k < gsw_mlp::NOUT && ((mask >> k) & 1u)) ? k : -1;
+  }
+  GSW_REQUIRE(g_mlp_dev[app], GSW_E_INVALID,
+              "learned selector requested but no weights are loaded for this "
+              "application");
+  gsw_mlp::forward_host(g_mlp_host[app].data(), x, y);
+  return gsw_mlp::argmin_valid(y, mask);
 }
 
 app_ctx_t<BFS> &bfs_ctx(gsw_graph *G) {
@@ -1671,8 +1707,17 @@ gsw_status gsw_cc(gsw_graph *G, const gsw_run_opts *opts, int32_t *labels_out,
                   gsw_run_stats *stats) {
   return guarded([&] {
     GSW_REQUIRE(G, GSW_E_INVALID, "null graph");
-    const run_opts_t o = to_opts_rule(opts);
-    if (!(opts && opts->configs)) {
+    run_opts_t o = to_opts(opts);
+    const bool forced = opts && opts->configs;
+    int cls = -1;
+    if (o.selector != 0 && !forced) cls = choose_once(G, GSW_APP_CC, o.selector,
+                                                      gsw_mlp::CC_CLASSES);
+    G->last_class = cls;
+    if (cls >= 2) {
+      o.force.has_asfmt = true;
+      o.force.asf = cls == 2 ? Bitmap : Queue;
+    }
+    if (cls == 0 || (cls < 0 && !forced)) {
       // sampled hooking over the CSR
       hipStream_t st = gsw::stream();
       H_ERR(hipEventRecord(G->ev0, st));
@@ -1794,6 +1839,7 @@ gsw_status gsw_pagerank(gsw_graph *G, float damping, int32_t iters, float eps,
     H_ERR(hipEventRecord(G->ev1, st));
     const double ms = elapsed(G);
     collect_records(G, c.en.executor, c.as);
+    G->last_class = o.selector != 0 ? c.en.conf.mlp_applied : -1;
     if (pr_out)
       H_ERR(hipMemcpy(pr_out, G->d_out_f, sizeof(float) * G->hg.n,
                       hipMemcpyDeviceToHost));
@@ -2063,6 +2109,13 @@ gsw_status gsw_selector_load_tree(int32_t app, int32_t nodes, const int32_t *fea
     t.left.assign(left, left + nodes);
     t.right.assign(right, right + nodes);
     t.cls.assign(cls, cls + nodes);
+  });
+}
+
+gsw_status gsw_last_class(const gsw_graph *G, int32_t *cls) {
+  return guarded([&] {
+    GSW_REQUIRE(G && cls, GSW_E_INVALID, "null argument");
+    *cls = G->last_class;
   });
 }
 

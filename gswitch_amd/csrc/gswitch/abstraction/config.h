@@ -72,8 +72,10 @@ struct config_t {
   // (mlp_class, written by the inspector / the fused-level selection)
   bool policy_mlp = false;
   int mlp_class = -1;
-  // the class is one of the from-all pull kernels (gsw_mlp::pr_variant_of_class)
+  // the class is one of the from-all pull kernels (gsw_mlp::pr_variant_of_class),
+  // chosen once, on the first level
   bool mlp_pr_classes = false;
+  int mlp_applied = -1;  // class of the latest learned selection applied
   // fused levels under the learned selector: the frontier size when the
   // network last chose a fused class (re-inspection waits for growth)
   bool mlp_entered_fused = false;

@@ -90,7 +90,8 @@ struct inspector_t {
       // the partial slots are zero between passes: k_stat_finish clears them
       gsw_mlp::sel_dev_t d;
       // (a run that keeps its first choice needs no inference after it)
-      if (conf.policy_mlp && stat.mlp.w && !conf.conf_inherit) {
+      if (conf.policy_mlp && stat.mlp.w && !conf.conf_inherit &&
+          !conf.mlp_pr_classes) {
         d = stat.mlp;
         d.level = (float)g.level;
       }

@@ -174,6 +174,7 @@ struct selector_t {
   // fused expand (the next level then runs the inspector), so a fused level
   // given a standalone class keeps the fused form with the class's LB.
   static void select_learned(config_t &conf) {
+    conf.mlp_applied = conf.mlp_class;
     if (conf.mlp_pr_classes) {
       // one from-all pull for every iteration of the run
       const gsw_mlp::pr_variant_t p = gsw_mlp::pr_variant_of_class(conf.mlp_class);

@@ -107,6 +107,13 @@ __host__ __device__ inline pr_variant_t pr_variant_of_class(int c) {
   }
 }
 
+// Connected components choose once per run, from the same first-level
+// inputs (every vertex active), among the network's first NCC outputs:
+//   0 sampled hooking over the CSR   1 edge-centric level loop, frontier
+//   formats by the rule   2 edge-centric, bitmap   3 edge-centric, edge queue
+constexpr int NCC = 4;
+constexpr uint32_t CC_CLASSES = (1u << NCC) - 1;
+
 // classes that make sense for an algorithm: a from-all and to-all one
 // (PageRank-like) has no fused form, as init_conf keeps it standalone
 inline uint32_t valid_mask(bool fromall, bool toall) {

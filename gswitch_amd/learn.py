@@ -18,6 +18,11 @@ PageRank, one network each).
    and every leaf class is chosen to minimise the summed time of the
    levels it covers (selector 2 of the engine).
 
+CC (--app cc) also chooses once per run, from the same first-level
+features, among sampled hooking (its default) and the edge-centric level
+loop with the rule's, bitmap or queue frontiers (gswitch_amd.CC_CLASSES);
+its "rule" baseline is the default run.
+
 PageRank (--app pr) pulls every vertex the same way in every iteration,
 so it chooses once per run: a sample is one graph, its features are those
 of the run's first level (everything active) and its timings are the
@@ -31,8 +36,9 @@ Run as a module on a GPU machine (--app bfs | sssp | pr, default bfs):
     python -m gswitch_amd.learn train --app A --data DIR/train.npz --out DIR
     python -m gswitch_amd.learn collect --app A --set eval --out DIR
     python -m gswitch_amd.learn evaluate --app A --data DIR/eval.npz --model M --out DIR
-    python -m gswitch_amd.learn evaluate --app pr --model M --out DIR   (PageRank:
-        the held-out graphs are timed per class during the end-to-end pass)
+    python -m gswitch_amd.learn evaluate --app pr|cc --model M --out DIR
+        (PageRank, CC: the held-out graphs are timed per class during the
+        end-to-end pass)
     python -m gswitch_amd.learn fit-trees --app A --data DIR/train.npz --out DIR
 """
 import argparse
@@ -46,9 +52,10 @@ import numpy as np
 import gswitch_amd as gsw
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-APPS = ("bfs", "sssp", "pr")
+APPS = ("bfs", "sssp", "pr", "cc")
 # classes each application's selector may pick
-MASK = {"bfs": (1 << 30) - 1, "sssp": (1 << 30) - 1, "pr": gsw.PR_CLASS_MASK}
+MASK = {"bfs": (1 << 30) - 1, "sssp": (1 << 30) - 1, "pr": gsw.PR_CLASS_MASK,
+        "cc": gsw.CC_CLASS_MASK}
 PR_ITERS = 3          # iterations per timed PageRank run while collecting
 PR_SLOW = 50.0        # a class this many times slower than the best is timed once
 
@@ -176,8 +183,8 @@ def rule_trajectory(g, root, app="bfs"):
 
 
 def pagerank_features(h):
-    """The 12 inputs of a PageRank run's first level (every vertex active,
-    push workload m), as the engine computes them for its choice."""
+    """The 12 inputs of a PageRank (or CC) run's first level (every vertex
+    active, push workload m), as the engine computes them for its choice."""
     st = gsw.degree_stats(h.row_ptr)
     n, m = float(h.n), float(h.m)
     return gsw.mlp_features([n, m, st["avg"], st["std"], st["GI"], 0.0, n, 0.0, m,
@@ -226,9 +233,46 @@ def collect_pr(specs, reps=2, log=print):
     return {k: np.concatenate(v) for k, v in rows.items()}
 
 
+def time_cc_classes(g, reps=3):
+    """Device ms of a CC run under each class (CC_CLASSES order), best of
+    `reps` after a warm-up, and the labels of the last class (all equal)."""
+    t = np.full(len(gsw.CC_CLASSES), np.inf)
+    ref = None
+    for c, cfg in enumerate(gsw.CC_CLASSES):
+        lab = g.cc(configs=cfg, profile=False)
+        if ref is None:
+            ref = lab
+        assert np.array_equal(lab, ref), c
+        for _ in range(reps):
+            g.cc(configs=cfg, out=False, profile=False)
+            t[c] = min(t[c], g.last_stats.device_ms)
+    return t, ref
+
+
+def collect_cc(specs, reps=3, log=print):
+    rows = dict(features=[], ms=[], rule=[], level=[], graph=[])
+    t0 = time.time()
+    for gi, spec in enumerate(specs):
+        h = build(spec)
+        g = gsw.Graph(h)
+        ms, _ = time_cc_classes(g, reps)
+        g.close()
+        rows["features"].append(pagerank_features(h)[None])
+        rows["ms"].append(ms[None].astype(np.float32))
+        rows["rule"].append(np.zeros(1, np.int32))  # the default run
+        rows["level"].append(np.zeros(1, np.int32))
+        rows["graph"].append(np.full(1, gi, np.int32))
+        log("graph %d/%d %s n=%d m=%d best %d (%s ms) (%.0fs)" % (
+            gi + 1, len(specs), spec["kind"], h.n, h.m, int(np.argmin(ms)),
+            " ".join("%.3f" % x for x in ms), time.time() - t0))
+    return {k: np.concatenate(v) for k, v in rows.items()}
+
+
 def collect(specs, reps=2, max_levels=32, log=print, app="bfs"):
     if app == "pr":
         return collect_pr(specs, reps, log)
+    if app == "cc":
+        return collect_cc(specs, max(reps, 3), log)
     rows = dict(features=[], ms=[], rule=[], level=[], graph=[])
     t0 = time.time()
     for gi, spec in enumerate(specs):
@@ -398,6 +442,37 @@ def load_model(path=MODEL_PATH):
 FIXED = ["Push-Queue-WM-Standalone", "Push-Queue-WM-Fused", None]
 
 
+def end_to_end_cc(specs, w, reps=3, log=print):
+    """Median device time of CC per graph under the default run (sampled
+    hooking) and the network, and the exhaustive per-class optimum; labels
+    bit-exact against the default run's."""
+    gsw.selector_load(w, app="cc")
+    out = []
+    for spec in specs:
+        h = build(spec)
+        g = gsw.Graph(h)
+        ref = g.cc()
+        rec = dict(kind=spec["kind"], n=h.n, m=h.m)
+        for name, kw in [("rule", {}), ("mlp", dict(selector=1))]:
+            assert np.array_equal(g.cc(**kw), ref), (name, spec)
+            rec[name + "_class"] = g.last_class() if kw else 0
+            ts = []
+            for _ in range(reps):
+                g.cc(out=False, profile=False, **kw)
+                ts.append(g.last_stats.device_ms)
+            rec[name] = float(np.median(ts))
+        per, lab = time_cc_classes(g)
+        assert np.array_equal(lab, ref)
+        rec["classes_ms"] = [float(x) for x in per]
+        rec["best_class"] = int(np.argmin(per))
+        g.close()
+        log("%s n=%d default %.3f mlp %.3f (class %d) best class %d: %.3f ms" % (
+            spec["kind"], h.n, rec["rule"], rec["mlp"], rec["mlp_class"],
+            rec["best_class"], per.min()))
+        out.append(rec)
+    return out
+
+
 def end_to_end_pr(specs, w, iters=20, reps=3, log=print):
     """Median device time of a PageRank run of `iters` iterations per graph
     under the rule trees and the network, and the exhaustive per-class
@@ -505,11 +580,15 @@ def main(argv=None):
         save_model(os.path.join(a.out, "selector_%s.npz" % a.app), w,
                    dict(app=a.app, samples=int(len(d["ms"])), epochs=a.epochs,
                         loss_first=float(curve[0]), loss_last=float(curve[-1])))
-    elif a.app == "pr" and not a.data:
-        # one run per graph: the end-to-end pass times every class as well
+    elif a.app in ("pr", "cc") and not a.data:
+        # one choice per graph: the end-to-end pass times every class as well
         w = load_model(a.model or model_path(a.app))
-        e2e = end_to_end_pr(eval_specs(), w, log=log)
-        ms = np.array([r["classes_ms_per_iter"] for r in e2e])
+        if a.app == "pr":
+            e2e = end_to_end_pr(eval_specs(), w, log=log)
+            ms = np.array([r["classes_ms_per_iter"] for r in e2e])
+        else:
+            e2e = end_to_end_cc(eval_specs(), w, log=log)
+            ms = np.array([r["classes_ms"] for r in e2e])
         res = dict(graphs=len(e2e),
                    regret_mlp=per_level_regret(ms, np.array([r["mlp_class"] for r in e2e])),
                    regret_rule=per_level_regret(ms, np.array([r["rule_class"] for r in e2e])),

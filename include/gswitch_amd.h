@@ -305,9 +305,19 @@ gsw_status gsw_tree_eval(const char *name, const double *fv, int32_t nfv,
 #define GSW_APP_BFS 0
 #define GSW_APP_SSSP 1
 #define GSW_APP_PR 2  /* classes: the from-all pull kernels, chosen once per run */
-#define GSW_APPS 3
+#define GSW_APP_CC 3  /* classes: sampled hooking or the edge-centric loop, once per run */
+#define GSW_APPS 4
 /* Install the weights that `app`'s runs use when gsw_run_opts.selector == 1. */
 gsw_status gsw_selector_load_app(int32_t app, const float *w, int32_t nparam);
+/* The class the learned selector (selector 1 or 2) chose in the graph's
+ * latest PageRank or CC run, applications that choose once per run
+ * (PageRank: 0 TM, 1 WM, 2 CM, 3 STRICT merge-path walk, 4 STRICT
+ * segmented, 5 TWOD length-binned slices, 6 TWOD segmented slices, 7 TWOD
+ * propagation blocking; CC: 0 sampled hooking, 1 edge-centric with the
+ * rule's formats, 2 edge-centric bitmap, 3 edge-centric queue); -1 if that
+ * run used no learned selector (or a class tree's leaf it cannot run).
+ * No counterpart in the reference. */
+gsw_status gsw_last_class(const gsw_graph *g, int32_t *cls);
 /* = gsw_selector_load_app(GSW_APP_BFS, w, nparam). */
 gsw_status gsw_selector_load(const float *w, int32_t nparam);
 /* Install the class tree that `app`'s runs use when gsw_run_opts.selector

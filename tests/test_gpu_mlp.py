@@ -137,8 +137,8 @@ def test_learned_policy_bit_exact(gsw, orc, name, make):
 def test_learned_policy_errors(gsw):
     h = gsw.rmat(10, 8, seed=3, weighted=True)
     g = gsw.Graph(h)
-    with pytest.raises(gsw.GswError):  # no learned selector for CC
-        g.cc(selector=1)
+    with pytest.raises(gsw.GswError):  # no such selector
+        g.cc(selector=4)
     with pytest.raises(gsw.GswError):  # no such selector
         g.pagerank(0.85, 3, selector=3)
     with pytest.raises(gsw.GswError):  # no such pull kernel
@@ -149,7 +149,7 @@ def test_learned_policy_errors(gsw):
         gsw.selector_load(np.zeros(NP, np.float32), app="pagerank")
     bad = np.zeros(NP, np.float32)
     assert gsw._lib.gsw_selector_load(bad.ctypes.data, 10) == 1  # GSW_E_INVALID
-    assert gsw._lib.gsw_selector_load_app(3, bad.ctypes.data, NP) == 1
+    assert gsw._lib.gsw_selector_load_app(4, bad.ctypes.data, NP) == 1
     bad[7] = np.nan
     assert gsw._lib.gsw_selector_load(bad.ctypes.data, NP) == 1
 
@@ -252,6 +252,7 @@ def test_learned_pagerank(gsw, orc, name, make):
         recs = g.records()
         got = {gsw.pr_class_of_record(r) for r in recs}
         assert got == {want} and len(recs) > 0, (seed, got, want)
+        assert g.last_class() == want
         assert pr_max_rel(pr, ref) <= 1e-5, (seed, want)
         seen.add(want)
     assert len(seen) >= 2
@@ -282,3 +283,57 @@ def test_trained_pagerank_model_file(gsw, orc):
         want = int(gsw.mlp_select(w, learn.pagerank_features(h), mask=gsw.PR_CLASS_MASK)[0])
         assert {gsw.pr_class_of_record(r) for r in g.records()} == {want}
         assert pr_max_rel(pr, orc.pagerank(h.row_ptr, h.col, 0.85, 50)) <= 1e-5
+
+
+CC_GRAPHS = [("rmat", lambda gsw: gsw.rmat(12, 8, seed=7)),
+             ("grid", lambda gsw: gsw.grid2d(70, 70, 0.3, 0.0, 5)),
+             ("er", lambda gsw: gsw.erdos_renyi(6000, 1.5, 4))]
+
+
+@pytest.mark.parametrize("name,make", CC_GRAPHS)
+def test_learned_cc(gsw, orc, name, make):
+    """CC under a learned network runs the class the network gives the
+    first-level features (sampled hooking or one of the edge-centric
+    loops); the labels are bit-exact against the oracle whatever it is."""
+    from gswitch_amd import learn
+    h = make(gsw)
+    g = gsw.Graph(h)
+    ref = orc.cc(h.row_ptr, h.col)
+    x = learn.pagerank_features(h)
+    seen = set()
+    for seed in range(12):
+        w = learn.init_weights(seed + 100)
+        w[4992:6912] *= 30.0
+        want = int(gsw.mlp_select(w, x, mask=gsw.CC_CLASS_MASK)[0])
+        gsw.selector_load(w, app="cc")
+        assert np.array_equal(g.cc(selector=1), ref), (seed, want)
+        assert g.last_class() == want
+        seen.add(want)
+    assert len(seen) >= 2
+    # forced configs take precedence over the selector
+    assert np.array_equal(g.cc(configs="x-Queue-x-x", selector=1), ref)
+    assert g.last_class() == -1
+
+
+@pytest.mark.parametrize("cls", range(4))
+def test_cc_class_tree(gsw, orc, cls):
+    h = gsw.grid2d(40, 40, 0.25, 0.01, 8)
+    g = gsw.Graph(h)
+    gsw.selector_load_tree(dict(feat=[-1], thr=[0.0], left=[-1], right=[-1],
+                                cls=[cls]), app="cc")
+    assert np.array_equal(g.cc(selector=2), orc.cc(h.row_ptr, h.col))
+    assert g.last_class() == cls
+
+
+def test_trained_cc_model_file(gsw, orc):
+    from gswitch_amd import learn
+    path = learn.model_path("cc")
+    if not os.path.exists(path):
+        pytest.skip("no trained CC selector committed")
+    w = learn.load_model(path)
+    gsw.selector_load(w, app="cc")
+    for h in (gsw.rmat(14, 16, seed=11), gsw.grid2d(120, 120, 0.1, 0.01, 4)):
+        g = gsw.Graph(h)
+        assert np.array_equal(g.cc(selector=1), orc.cc(h.row_ptr, h.col))
+        want = int(gsw.mlp_select(w, learn.pagerank_features(h), mask=gsw.CC_CLASS_MASK)[0])
+        assert g.last_class() == want

@@ -95,7 +95,7 @@ def test_committed_training_sets(orc, app):
     with np.load(path, allow_pickle=False) as z:
         d = {k: z[k] for k in z.files}
     n = len(d["ms"])
-    k = 8 if app == "pr" else 30  # PageRank: the from-all pull classes
+    k = {"pr": 8, "cc": 4}.get(app, 30)  # PageRank / CC: their own classes
     assert d["features"].shape == (n, 12) and d["ms"].shape == (n, k)
     assert np.all(d["ms"] > 0) and np.isfinite(d["features"]).all()
     assert np.all((d["rule"] >= 0) & (d["rule"] < k))
@@ -140,6 +140,7 @@ def test_pagerank_classes():
     assert len(gsw.PR_CLASSES) == 8 and len(set(gsw.PR_CLASSES)) == 8
     assert gsw.PR_CLASS_MASK == 0xFF
     assert learn.MASK["pr"] == 0xFF and learn.MASK["bfs"] == (1 << 30) - 1
+    assert learn.MASK["cc"] == 0xF and gsw.CC_CLASSES[0] is None
 
     def rec(lb, kernel):
         return dict(dir="Pull", fmt="Bitmap", lb=lb, kernel=kernel, fused=False,
