@@ -5,10 +5,11 @@ Workload (one "step"): SSSP from the highest-degree vertex plus a
 50-iteration PageRank (damping 0.85) on R-MAT scale 22, edge factor 16,
 (a, b, c, d) = (0.57, 0.19, 0.19, 0.05), seed 2, symmetric int32 CSR with
 uint32 weights in [1, 64] -- the SSSP / PageRank configuration of the
-engine's headline metric. SSSP picks its variants with the learned selector
-(the committed network; --rule-only for the rule trees), PageRank with the
-rule trees. The graph is resident in HBM before the timed region; runs keep
-their results on the device.
+engine's headline metric. SSSP and PageRank pick their variants with the
+learned selectors (the committed networks; --rule-only for the rule
+trees). The graph is resident in HBM before the timed region (PageRank's
+propagation-blocked layout is built during the warm-up); runs keep their
+results on the device.
 
 GTEPS (Graph500 convention): SSSP counts sum(deg(v), v reached)/2,
 PageRank counts m_undirected per iteration; value = total / wall time.
@@ -319,11 +320,23 @@ def extended(gsw, args):
     for _ in range(3):
         g.cc(configs="x-Bitmap-x-x", out=False)
         t_ec.append(g.last_stats.device_ms)
-    g.close()
     grid["cc_ms"] = round(min(t_cc), 3)
     grid["cc_gteps"] = round((h.m // 2) / (min(t_cc) * 1e-3) / 1e9, 2)
     grid["cc_edge_centric_ms"] = round(min(t_ec), 3)
     grid["cc_edge_centric_bit_exact"] = bool(np.array_equal(lab, lab_ec))
+    ms_cc_learned = min(t_cc)
+    if os.path.exists(learn.model_path("cc")):
+        gsw.selector_load(learn.load_model(learn.model_path("cc")), app="cc")
+        lab_l = g.cc(selector=1)
+        t_l = []
+        for _ in range(5):
+            g.cc(selector=1, out=False)
+            t_l.append(g.last_stats.device_ms)
+        ms_cc_learned = min(t_l)
+        grid["cc_learned_ms"] = round(ms_cc_learned, 3)
+        grid["cc_learned_class"] = gsw.CC_CLASSES[g.last_class()] or "sampled"
+        grid["cc_learned_bit_exact"] = bool(np.array_equal(lab, lab_l))
+    g.close()
     cpu["c3_bfs_grid"], ref_l = cpu_leg(lambda: oracle.bfs(h.row_ptr, h.col, 0),
                                         graph500_edges(h, lvg))
     cpu["c3_cc_grid"], ref_c = cpu_leg(lambda: oracle.cc(h.row_ptr, h.col), h.m // 2)
@@ -462,13 +475,16 @@ def extended(gsw, args):
                                     % (len(samp) - 1, args.batch))
     out["cpu_legs"] = cpu
     # --- aggregate: batched + learned vs plain per-query fixed strategy
-    t_new = ms_batch + ms64 + (learned if learned is not None else ms) + ms_g_learned
-    t_old = t_loop + t_seq + sweep24["best_ms"] + sweepg["best_ms"]
+    t_new = ms_batch + ms64 + (learned if learned is not None else ms) + ms_g_learned \
+        + ms_cc_learned
+    t_old = t_loop + t_seq + sweep24["best_ms"] + sweepg["best_ms"] + \
+        min(grid["cc_ms"], grid["cc_edge_centric_ms"])
     out["batched_learned_vs_fixed"] = {
         "speedup": round(t_old / t_new, 2),
-        "note": "same work (ER batch BFS+CC, 64 BFS, one R-MAT 24 BFS, one grid BFS); "
-                "device ms; fixed = the fastest of the 30 variant classes per workload "
-                "(per-graph loop: fastest on a 40-instance sample)"}
+        "note": "same work (ER batch BFS+CC, 64 BFS, one R-MAT 24 BFS, one grid BFS, "
+                "one grid CC); device ms; fixed = the fastest of the 30 variant classes "
+                "per workload (per-graph loop: fastest on a 40-instance sample; grid CC: "
+                "the faster of sampled hooking and edge-centric bitmap)"}
     return out
 
 
@@ -514,20 +530,24 @@ def main():
     copy_gbs = gsw.stream_copy(1 << 30, 10)
     log("stream copy: %.0f GB/s" % copy_gbs)
 
-    # SSSP runs under its learned selector when a trained network is
-    # committed (and no variant is forced); PageRank keeps the rule trees
+    # SSSP and PageRank run under their learned selectors when trained
+    # networks are committed (and no variant is forced)
     from gswitch_amd import learn
-    sel = {}
+    sel, sel_pr = {}, {}
     if os.path.exists(learn.model_path("sssp")) and args.configs_sssp is None \
             and not args.rule_only:
         gsw.selector_load(learn.load_model(learn.model_path("sssp")), app="sssp")
         sel = {"selector": 1}
+    if os.path.exists(learn.model_path("pr")) and args.configs_pr is None \
+            and not args.rule_only:
+        gsw.selector_load(learn.load_model(learn.model_path("pr")), app="pr")
+        sel_pr = {"selector": 1}
 
     def step():
         g.sssp(root, configs=args.configs_sssp, out=False, **sel)
         s_ms = g.last_stats.device_ms
         s_rec = g.records()
-        g.pagerank(DAMPING, PR_ITERS, configs=args.configs_pr, out=False)
+        g.pagerank(DAMPING, PR_ITERS, configs=args.configs_pr, out=False, **sel_pr)
         return s_ms, s_rec, g.last_stats.device_ms, g.records()
 
     for _ in range(args.warmup):
@@ -542,7 +562,7 @@ def main():
         t1 = time.perf_counter()
         s_ms, s_rec = g.last_stats.device_ms, g.records()
         t2 = time.perf_counter()
-        g.pagerank(DAMPING, PR_ITERS, configs=args.configs_pr, out=False)
+        g.pagerank(DAMPING, PR_ITERS, configs=args.configs_pr, out=False, **sel_pr)
         t3 = time.perf_counter()
         p_ms, p_rec = g.last_stats.device_ms, g.records()
         wall += (t1 - t0) + (t3 - t2)
@@ -592,37 +612,55 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu, cdist, cpr5 = cpu_baseline(h, root, sssp_edges, pr_edges)
-        gpr5 = g.pagerank(DAMPING, 5, configs=args.configs_pr)
+        gpr5 = g.pagerank(DAMPING, 5, configs=args.configs_pr, **sel_pr)
         detail["parity"] = {
             "sssp_bit_exact": bool(np.array_equal(dist, cdist)),
             "pagerank5_max_rel_err": float(np.max(np.abs(gpr5 - cpr5) / np.abs(cpr5))),
         }
 
     if not args.quick:
-        # the same SSSP and PageRank under one fixed strategy (no selector)
-        fixed_s, fixed_p = "Push-Queue-WM-Standalone", "Push-Bitmap-WM-Standalone"
-        dist_f = g.sssp(root, configs=fixed_s)
-        s_fixed = g.last_stats.device_ms
-        pr_rule = g.pagerank(DAMPING, PR_ITERS, configs=args.configs_pr)
-        pr_f = g.pagerank(DAMPING, PR_ITERS, configs=fixed_p)
-        p_fixed = g.last_stats.device_ms
+        # the same SSSP and PageRank under every fixed strategy: SSSP's 30
+        # variant classes, PageRank's 8 from-all pulls (no selector)
+        def sssp_run(c):
+            d = g.sssp(root, configs=c, profile=False)
+            return d, g.last_stats.device_ms
+        sweep_s = fixed_sweep(sssp_run, dist)
+        pr_ref = g.pagerank(DAMPING, PR_ITERS, configs=args.configs_pr, **sel_pr)
+        pr_times, pr_err = {}, 0.0
+        for cfg, pk in gsw.PR_CLASSES:
+            ts = []
+            for _ in range(2):
+                pr_f = g.pagerank(DAMPING, PR_ITERS, configs=cfg, pull_kernel=pk,
+                                  profile=False)
+                ts.append(g.last_stats.device_ms)
+            pr_times["%s/pull_kernel=%d" % (cfg, pk)] = min(ts)
+            pr_err = max(pr_err, float(np.max(np.abs(pr_f.astype(np.float64) - pr_ref)
+                                              / np.abs(pr_ref))))
+        best_p = min(pr_times, key=pr_times.get)
         detail["fixed_strategy"] = {
-            "configs": [fixed_s, fixed_p], "sssp_ms": round(s_fixed, 3),
-            "sssp_bit_exact": bool(np.array_equal(dist_f, dist)),
-            "pagerank_ms": round(p_fixed, 3),
-            "pagerank_max_rel_vs_rule": float(np.max(np.abs(pr_f.astype(np.float64) - pr_rule)
-                                                     / np.abs(pr_rule))),
-            "speedup_rule_vs_fixed": round((s_fixed + p_fixed) /
-                                           ((dev_sssp + dev_pr) / args.steps), 2)}
-        # the same SSSP under the rule trees
-        t_r = []
+            "sssp": sweep_s,
+            "pagerank": {"best": best_p, "best_ms": round(pr_times[best_p], 3),
+                         "classes_ms": {k: round(v, 3) for k, v in pr_times.items()},
+                         "max_rel_vs_headline": pr_err},
+            "speedup_selected_vs_best_fixed": round(
+                (sweep_s["best_ms"] + pr_times[best_p]) /
+                ((dev_sssp + dev_pr) / args.steps), 3)}
+        # the same SSSP and PageRank under the rule trees
+        t_r, t_pr = [], []
         for _ in range(args.steps):
             dist_r = g.sssp(root, configs=args.configs_sssp)
             t_r.append(g.last_stats.device_ms)
-        detail["sssp_selectors"] = {"headline": "learned" if sel else "rule",
-                                    "headline_ms": round(dev_sssp / args.steps, 3),
-                                    "rule_ms": round(min(t_r), 3),
-                                    "rule_bit_exact": bool(np.array_equal(dist_r, dist))}
+            pr_r = g.pagerank(DAMPING, PR_ITERS, configs=args.configs_pr)
+            t_pr.append(g.last_stats.device_ms)
+        detail["selectors"] = {
+            "headline": "learned" if sel else "rule",
+            "sssp_headline_ms": round(dev_sssp / args.steps, 3),
+            "sssp_rule_ms": round(min(t_r), 3),
+            "sssp_rule_bit_exact": bool(np.array_equal(dist_r, dist)),
+            "pagerank_headline_ms": round(dev_pr / args.steps, 3),
+            "pagerank_rule_ms": round(min(t_pr), 3),
+            "pagerank_rule_max_rel": float(np.max(np.abs(pr_r.astype(np.float64) - pr_ref)
+                                                  / np.abs(pr_ref)))}
 
     g.close()
     if not args.quick:
@@ -645,7 +683,8 @@ def main():
                                % (args.scale, args.seed),
                    "scale": args.scale, "edge_factor": 16, "n": n,
                    "m_undirected": m // 2, "root": root,
-                   "selector": ("learned (SSSP), rule (PageRank)" if sel else "rule"),
+                   "selector": ("learned (SSSP, PageRank)" if sel and sel_pr else
+                                "learned (SSSP)" if sel else "rule"),
                    "parallelism": "single-gpu"},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -115,6 +115,82 @@ __global__ void __launch_bounds__(BLOCK)
   }
 }
 
+// The same count / fill with one workgroup per bin, for layouts of at most
+// PB_BIN_C chunks: the bin's tile counters (count) or cursors and tile
+// offsets (fill) live in LDS, so a run of edges of one tile costs one LDS
+// atomic instead of a global one. The bin's in-edges are one contiguous
+// range (its targets are consecutive rows), walked 64 edges per wavefront
+// step whatever the row lengths (a one-target bin of a hub is split over
+// every wavefront); a lane finds its row by binary search among the bin's
+// rows from the row of the wavefront's first edge (empty rows included).
+constexpr int PB_BIN_C = 8192;
+
+__device__ __forceinline__ int64_t pb_row_of(const int64_t *__restrict__ rows,
+                                             int64_t lo, int64_t hi, int64_t k) {
+  // the last r in [lo, hi) with rows[r] <= k
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (rows[mid] <= k)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(PB_BLOCK)
+    k_pb_tiles_bin(const int64_t *__restrict__ rows, const int32_t *__restrict__ cols,
+                   const int32_t *__restrict__ bin_lo, int S, int64_t C,
+                   unsigned long long *cnt, const int64_t *__restrict__ tile_off,
+                   uint16_t *src, uint16_t *dst) {
+  __shared__ int64_t s_off[PB_BIN_C];
+  __shared__ int s_cur[PB_BIN_C];
+  const int64_t b = blockIdx.x;
+  const bool fill = tile_off != nullptr;
+  const int64_t vlo = bin_lo[b], vhi = bin_lo[b + 1];
+  for (int64_t c = threadIdx.x; c < C; c += PB_BLOCK) {
+    s_cur[c] = 0;
+    if (fill) s_off[c] = tile_off[b * C + c];
+  }
+  __syncthreads();
+  const int lane = lane_id();
+  const int wv = threadIdx.x >> LOG_WAVE;
+  constexpr int NW = PB_BLOCK / WAVE;
+  const int64_t E0 = rows[vlo], E1 = rows[vhi];
+  for (int64_t k0 = E0 + (int64_t)wv * WAVE; k0 < E1; k0 += (int64_t)NW * WAVE) {
+    const int64_t r0 = pb_row_of(rows, vlo, vhi, k0);
+    const int64_t k = k0 + lane;
+    const bool ok = k < E1;
+    const int64_t v = ok ? pb_row_of(rows, r0, vhi, k) : vlo;
+    const int u = ok ? cols[k] : 0;
+    const int c = u / S;
+    const int prev = __shfl_up(c, 1, 64);
+    const bool head = ok && (lane == 0 || prev != c);
+    const uint64_t hm = ballot(head);
+    const int nvalid = (int)MIN((int64_t)WAVE, E1 - k0);
+    const uint64_t later = hm & ~((2ull << lane) - 1);
+    const int run = head ? (later ? __builtin_ctzll(later) : nvalid) - lane : 0;
+    if (!fill) {
+      if (head) atomicAdd(&s_cur[c], run);
+    } else {
+      int base = 0;
+      if (head) base = atomicAdd(&s_cur[c], run);
+      const int hl = 63 - __clzll((long long)(hm & ((2ull << lane) - 1)));
+      base = __shfl(base, hl, 64);
+      if (ok) {
+        const int64_t pos = s_off[c] + base + (lane - hl);
+        src[pos] = (uint16_t)(u - c * S);
+        dst[pos] = (uint16_t)(v - vlo);
+      }
+    }
+  }
+  if (!fill) {
+    __syncthreads();
+    for (int64_t c = threadIdx.x; c < C; c += PB_BLOCK)
+      cnt[b * C + c] = (unsigned long long)s_cur[c];
+  }
+}
+
 // One step of a phase-1 tile walk: the tile's bounds, its unaligned head
 // and tail slots (one per lane) and its first PB_QUADS * 64 aligned quads
 // of 16-bit source offsets, loaded before they are needed.
@@ -401,17 +477,33 @@ template <typename G> void pb_ensure(G &g, int S, int D) {
   fill_async<int64_t>(pb.tile_off, 0, T + 1);
   fill_async<int32_t>(cursor, 0, T + 1);
   const int grid = grid_for(n * WAVE);
-  hipLaunchKernelGGL(k_pb_tiles, dim3(grid), dim3(BLOCK), 0, stream(),
-                     (const int64_t *)g.dgr_row_ptr, (const int32_t *)g.dgr_adj_list,
-                     n, (const int32_t *)pb.bin_lo, pb.B, S, pb.C,
-                     (unsigned long long *)pb.tile_off, nullptr, nullptr, nullptr,
-                     nullptr);
+  // (int LDS counters: a bin holds at most m edges)
+  const bool per_bin = pb.C <= PB_BIN_C && pb.B <= INT32_MAX && m <= INT32_MAX;
+  if (per_bin) {
+    hipLaunchKernelGGL(k_pb_tiles_bin, dim3((unsigned)pb.B), dim3(PB_BLOCK), 0, stream(),
+                       (const int64_t *)g.dgr_row_ptr, (const int32_t *)g.dgr_adj_list,
+                       (const int32_t *)pb.bin_lo, S, pb.C,
+                       (unsigned long long *)pb.tile_off, nullptr, nullptr, nullptr);
+  } else {
+    hipLaunchKernelGGL(k_pb_tiles, dim3(grid), dim3(BLOCK), 0, stream(),
+                       (const int64_t *)g.dgr_row_ptr, (const int32_t *)g.dgr_adj_list,
+                       n, (const int32_t *)pb.bin_lo, pb.B, S, pb.C,
+                       (unsigned long long *)pb.tile_off, nullptr, nullptr, nullptr,
+                       nullptr);
+  }
   H_ERR(hipGetLastError());
   scan_exclusive_inplace(pb.tile_off, T, bsum);
-  hipLaunchKernelGGL(k_pb_tiles, dim3(grid), dim3(BLOCK), 0, stream(),
-                     (const int64_t *)g.dgr_row_ptr, (const int32_t *)g.dgr_adj_list,
-                     n, (const int32_t *)pb.bin_lo, pb.B, S, pb.C, nullptr, cursor,
-                     (const int64_t *)pb.tile_off, pb.src, pb.dst);
+  if (per_bin) {
+    hipLaunchKernelGGL(k_pb_tiles_bin, dim3((unsigned)pb.B), dim3(PB_BLOCK), 0, stream(),
+                       (const int64_t *)g.dgr_row_ptr, (const int32_t *)g.dgr_adj_list,
+                       (const int32_t *)pb.bin_lo, S, pb.C, nullptr,
+                       (const int64_t *)pb.tile_off, pb.src, pb.dst);
+  } else {
+    hipLaunchKernelGGL(k_pb_tiles, dim3(grid), dim3(BLOCK), 0, stream(),
+                       (const int64_t *)g.dgr_row_ptr, (const int32_t *)g.dgr_adj_list,
+                       n, (const int32_t *)pb.bin_lo, pb.B, S, pb.C, nullptr, cursor,
+                       (const int64_t *)pb.tile_off, pb.src, pb.dst);
+  }
   H_ERR(hipGetLastError());
   std::vector<int64_t> off((size_t)T + 1);
   H_ERR(hipMemcpyAsync(off.data(), pb.tile_off, sizeof(int64_t) * (T + 1),

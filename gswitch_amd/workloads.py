@@ -34,7 +34,7 @@ def main(argv=None):
     ap.add_argument("--configs", default=None)
     ap.add_argument("--mib", type=int, default=1024)
     ap.add_argument("--records", action="store_true")
-    ap.add_argument("--pull-kernel", type=int, default=0)
+    ap.add_argument("--pull-kernel", type=int, default=2)
     ap.add_argument("--er-batch", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--selector", type=int, default=0)
