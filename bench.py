@@ -58,9 +58,11 @@ def sssp_level_bytes(rec, next_active, n):
 
 
 def kernel_key(app, r):
-    return "%s:%s-%s-%s-%s" % (app, r["dir"], "SortedQueue" if r["sorted"] and
-                               r["fmt"] == "Queue" else r["fmt"], r["lb"],
-                               "Fused" if r["fused"] else "Standalone")
+    """Variant of an expand record; pulls name their from-all pull kernel."""
+    k = "%s:%s-%s-%s-%s" % (app, r["dir"], "SortedQueue" if r["sorted"] and
+                            r["fmt"] == "Queue" else r["fmt"], r["lb"],
+                            "Fused" if r["fused"] else "Standalone")
+    return k + ("/pull_kernel=%d" % r["kernel"] if r["dir"] == "Pull" else "")
 
 
 def accumulate(acc, app, recs, n, m):

@@ -19,10 +19,10 @@ def test_more_than_one_gpu_is_refused():
 
 
 def rec(d, fmt, lb, fused=False, sorted_=False, frontier=0, edges=0, verts=0,
-        active=0, ms=1.0):
+        active=0, ms=1.0, kernel=2):
     return {"dir": d, "fmt": fmt, "lb": lb, "fused": fused, "sorted": sorted_,
             "frontier": frontier, "edges": edges, "verts": verts, "active": active,
-            "ms": ms}
+            "ms": ms, "kernel": kernel}
 
 
 def test_kernel_keys():
@@ -30,7 +30,12 @@ def test_kernel_keys():
         "sssp:Push-SortedQueue-WM-Standalone"
     # sortedness only names queue frontiers
     assert bench.kernel_key("pagerank", rec("Pull", "Bitmap", "TWOD", sorted_=True)) == \
-        "pagerank:Pull-Bitmap-TWOD-Standalone"
+        "pagerank:Pull-Bitmap-TWOD-Standalone/pull_kernel=2"
+    # pulls name their from-all pull kernel, pushes do not
+    assert bench.kernel_key("pagerank", rec("Pull", "Bitmap", "TWOD", kernel=0)) == \
+        "pagerank:Pull-Bitmap-TWOD-Standalone/pull_kernel=0"
+    assert bench.kernel_key("sssp", rec("Push", "Queue", "WM", kernel=0)) == \
+        "sssp:Push-Queue-WM-Standalone"
     assert bench.kernel_key("sssp", rec("Push", "Queue", "CM", fused=True)) == \
         "sssp:Push-Queue-CM-Fused"
 
@@ -40,7 +45,7 @@ def test_accumulate_groups_launches_by_variant():
     n, m = 1000, 16000
     recs = [rec("Pull", "Bitmap", "TWOD", ms=2.0) for _ in range(3)]
     bench.accumulate(acc, "pagerank", recs, n, m)
-    e = acc["pagerank:Pull-Bitmap-TWOD-Standalone"]
+    e = acc["pagerank:Pull-Bitmap-TWOD-Standalone/pull_kernel=2"]
     assert e["launches"] == 3 and e["ms"] == pytest.approx(6.0)
     assert e["bytes"] == pytest.approx(3 * bench.pr_bytes(n, m))
     # SSSP levels: each charged with the next level's active count

@@ -14,8 +14,9 @@ with the CPU oracle on it:
     (seed 5) through the autotuned captured BFS and the captured CC of
     gsw_batch_bfs_cc, every instance against its own oracle runs, plus
     seeded samples of instances from the rest of the batch;
-  * held-out selector evaluation graphs (R-MAT scale 20, Erdos-Renyi, grid)
-    under the rule trees and under both committed learned selectors.
+  * held-out selector evaluation graphs (R-MAT scale 20, Erdos-Renyi, grid,
+    Barabasi-Albert): BFS, SSSP, 50-iteration PageRank and CC under the rule
+    trees and under the committed learned selectors.
 The oracle runs of one test are spread over threads (the ctypes calls
 release the GIL); each is still the single-threaded restated routine.
 """
@@ -49,7 +50,7 @@ def max_rel(a, b):
 def load_selectors(gsw):
     from gswitch_amd import learn
     have = {}
-    for app in ("bfs", "sssp"):
+    for app in ("bfs", "sssp", "pr", "cc"):
         p = learn.model_path(app)
         if os.path.exists(p):
             gsw.selector_load(learn.load_model(p), app=app)
@@ -221,10 +222,14 @@ def test_c5_heldout_selectors(gsw, orc, k):
     g = gsw.Graph(h)
     root = learn.pick_root(spec, h)
     sel = load_selectors(gsw)
-    assert sel.get("bfs") and sel.get("sssp"), "committed networks missing"
+    assert all(sel.get(a) for a in ("bfs", "sssp", "pr", "cc")), "committed networks missing"
     ref_l = orc.bfs(h.row_ptr, h.col, root)
     ref_d = orc.spfa(h.row_ptr, h.col, h.w, root)
+    ref_p = orc.pagerank(h.row_ptr, h.col, 0.85, 50)
+    ref_c = orc.cc(h.row_ptr, h.col)
     for s in (0, 1):
         assert np.array_equal(g.bfs(root, selector=s), ref_l), ("bfs", s)
         assert np.array_equal(g.sssp(root, selector=s), ref_d), ("sssp", s)
+        assert max_rel(g.pagerank(0.85, 50, selector=s), ref_p) <= PR_TOL, ("pr", s)
+        assert np.array_equal(g.cc(selector=s), ref_c), ("cc", s)
     g.close()
