@@ -177,7 +177,10 @@ gsw_status gsw_sssp(gsw_graph *g, int32_t root, const gsw_run_opts *opts,
  * remaining edges of every vertex outside the most frequent sampled
  * component). With opts->configs set: the reference's edge-centric hooking
  * (cc.cu:15-32) + pointer jumping (src/utils/intrinsics.cuh:328-362) through
- * the level loop, honouring the pinned choices. */
+ * the level loop, honouring the pinned choices. With opts->selector 1 or 2
+ * and no configs, the learned network or class tree of GSW_APP_CC chooses
+ * between sampled hooking and the edge-centric loop (rule, bitmap or queue
+ * frontiers) once per run (gsw_last_class). */
 gsw_status gsw_cc(gsw_graph *g, const gsw_run_opts *opts, int32_t *labels_out,
                   gsw_run_stats *stats);
 /* Allocation-free form of gsw_cc's default path (sampled hooking): no
@@ -198,7 +201,11 @@ gsw_status gsw_cc_noalloc(const gsw_graph *g, int32_t samples, int32_t *labels,
 /* PageRank, un-normalised: pr_0 = 1-d, pr_{k+1}(v) = 1-d + d*sum pr_k(u)/deg(u),
  * `iters` iterations; eps > 0 additionally deactivates vertices whose
  * relative change is below eps as the reference does. Replaces
- * run_pagerank (application/pagerank.cu:81-103), functor pagerank.cu:18-42. */
+ * run_pagerank (application/pagerank.cu:81-103), functor pagerank.cu:18-42.
+ * Every iteration pulls every vertex the same way, so the variant is chosen
+ * once per run: by the rule trees (selector 0), or among the 8 from-all
+ * pull classes by the learned network or class tree of GSW_APP_PR
+ * (selector 1 / 2; gsw_last_class). */
 gsw_status gsw_pagerank(gsw_graph *g, float damping, int32_t iters, float eps,
                         const gsw_run_opts *opts, float *pr_out,
                         gsw_run_stats *stats);
