@@ -40,6 +40,8 @@ Run as a module on a GPU machine (--app bfs | sssp | pr, default bfs):
         (PageRank, CC: the held-out graphs are timed per class during the
         end-to-end pass)
     python -m gswitch_amd.learn fit-trees --app A --data DIR/train.npz --out DIR
+    python -m gswitch_amd.learn e2e --app A --out DIR      (held-out graphs end to end:
+        rule trees, committed network, committed class tree)
 """
 import argparse
 import json
@@ -442,20 +444,26 @@ def load_model(path=MODEL_PATH):
 FIXED = ["Push-Queue-WM-Standalone", "Push-Queue-WM-Fused", None]
 
 
-def end_to_end_cc(specs, w, reps=3, log=print):
+def end_to_end_cc(specs, w, reps=3, log=print, tree=None):
     """Median device time of CC per graph under the default run (sampled
-    hooking) and the network, and the exhaustive per-class optimum; labels
-    bit-exact against the default run's."""
+    hooking), the network and the class tree (when given), and the
+    exhaustive per-class optimum; labels bit-exact against the default
+    run's."""
     gsw.selector_load(w, app="cc")
+    if tree is not None:
+        gsw.selector_load_tree(tree, app="cc")
     out = []
     for spec in specs:
         h = build(spec)
         g = gsw.Graph(h)
         ref = g.cc()
         rec = dict(kind=spec["kind"], n=h.n, m=h.m)
-        for name, kw in [("rule", {}), ("mlp", dict(selector=1))]:
+        runs = [("rule", {}), ("mlp", dict(selector=1))]
+        if tree is not None:
+            runs.append(("tree", dict(selector=2)))
+        for name, kw in runs:
             assert np.array_equal(g.cc(**kw), ref), (name, spec)
-            rec[name + "_class"] = g.last_class() if kw else 0
+            rec[name + "_class"] = max(g.last_class(), 0) if kw else 0
             ts = []
             for _ in range(reps):
                 g.cc(out=False, profile=False, **kw)
@@ -473,18 +481,24 @@ def end_to_end_cc(specs, w, reps=3, log=print):
     return out
 
 
-def end_to_end_pr(specs, w, iters=20, reps=3, log=print):
+def end_to_end_pr(specs, w, iters=20, reps=3, log=print, tree=None):
     """Median device time of a PageRank run of `iters` iterations per graph
-    under the rule trees and the network, and the exhaustive per-class
-    optimum (results within 1e-5 relative of the rule's)."""
+    under the rule trees, the network and the class tree (when given), and
+    the exhaustive per-class optimum (results within 1e-5 relative of the
+    rule's)."""
     gsw.selector_load(w, app="pr")
+    if tree is not None:
+        gsw.selector_load_tree(tree, app="pr")
     out = []
     for spec in specs:
         h = build(spec)
         g = gsw.Graph(h)
         ref = g.pagerank(0.85, iters).astype(np.float64)
         rec = dict(kind=spec["kind"], n=h.n, m=h.m)
-        for name, kw in [("rule", {}), ("mlp", dict(selector=1))]:
+        runs = [("rule", {}), ("mlp", dict(selector=1))]
+        if tree is not None:
+            runs.append(("tree", dict(selector=2)))
+        for name, kw in runs:
             pr = g.pagerank(0.85, iters, **kw)
             err = float(np.max(np.abs(pr - ref) / np.abs(ref))) if h.n else 0.0
             assert err <= 1e-5, (name, spec, err)
@@ -505,10 +519,13 @@ def end_to_end_pr(specs, w, iters=20, reps=3, log=print):
     return out
 
 
-def end_to_end(specs, w, reps=3, log=print, app="bfs"):
-    """Median device time per graph under the rule trees, the network and
-    fixed strategies (results checked identical)."""
+def end_to_end(specs, w, reps=3, log=print, app="bfs", tree=None):
+    """Median device time per graph under the rule trees, the network, the
+    class tree (when given) and fixed strategies (results checked
+    identical)."""
     gsw.selector_load(w, app=app)
+    if tree is not None:
+        gsw.selector_load_tree(tree, app=app)
     out = []
     for spec in specs:
         h = build(spec, weighted=(app == "sssp"))
@@ -516,9 +533,12 @@ def end_to_end(specs, w, reps=3, log=print, app="bfs"):
         root = pick_root(spec, h)
         ref = run(g, app, root)
         rec = dict(kind=spec["kind"], n=h.n, m=h.m)
-        for name, kw in [("rule", {}), ("mlp", dict(selector=1)),
-                         ("fixed_wm", dict(configs=FIXED[0])),
-                         ("fixed_fused", dict(configs=FIXED[1]))]:
+        runs = [("rule", {}), ("mlp", dict(selector=1)),
+                ("fixed_wm", dict(configs=FIXED[0])),
+                ("fixed_fused", dict(configs=FIXED[1]))]
+        if tree is not None:
+            runs.insert(2, ("tree", dict(selector=2)))
+        for name, kw in runs:
             ts = []
             for _ in range(reps):
                 res = run(g, app, root, profile=False, **kw)
@@ -535,7 +555,7 @@ def end_to_end(specs, w, reps=3, log=print, app="bfs"):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=["collect", "train", "evaluate", "fit-trees"])
+    ap.add_argument("cmd", choices=["collect", "train", "evaluate", "fit-trees", "e2e"])
     ap.add_argument("--app", default="bfs", choices=APPS)
     ap.add_argument("--set", default="train", choices=["train", "eval"])
     ap.add_argument("--count", type=int, default=300)
@@ -562,6 +582,31 @@ def main(argv=None):
         save_tree(os.path.join(a.out, "trees_gfx950_%s.json" % a.app), tree,
                   dict(app=a.app, depth=TREE_DEPTH, min_leaf=TREE_MIN_LEAF,
                        samples=int(len(d["ms"])), data=os.path.basename(path)))
+        return 0
+    if a.cmd == "e2e":
+        # held-out graphs end to end under the rule trees, the committed
+        # network and the committed class tree
+        w = load_model(a.model or model_path(a.app))
+        tree = load_tree(tree_path(a.app))
+        specs = eval_specs()
+        if a.app == "pr":
+            e2e = end_to_end_pr(specs, w, log=log, tree=tree)
+        elif a.app == "cc":
+            e2e = end_to_end_cc(specs, w, log=log, tree=tree)
+        else:
+            e2e = end_to_end(specs, w, log=log, app=a.app, tree=tree)
+        gm = lambda k: float(np.exp(np.mean([np.log(r["rule"] / r[k]) for r in e2e])))
+        res = dict(graphs=len(e2e), speedup_mlp_over_rule=gm("mlp"),
+                   speedup_tree_over_rule=gm("tree"), end_to_end=e2e)
+        if a.app in ("pr", "cc"):
+            key = "classes_ms_per_iter" if a.app == "pr" else "classes_ms"
+            ms = np.array([r[key] for r in e2e])
+            for k in ("rule", "mlp", "tree"):
+                res["regret_" + k] = per_level_regret(
+                    ms, np.array([r[k + "_class"] for r in e2e]))
+        log(json.dumps({k: v for k, v in res.items() if k != "end_to_end"}))
+        with open(os.path.join(a.out, "e2e_%s.json" % a.app), "w") as f:
+            json.dump(res, f, indent=1)
         return 0
     if a.cmd == "collect":
         specs = training_specs() if a.set == "train" else eval_specs()
