@@ -647,6 +647,26 @@ def main():
             "speedup_selected_vs_best_fixed": round(
                 (sweep_s["best_ms"] + pr_times[best_p]) /
                 ((dev_sssp + dev_pr) / args.steps), 3)}
+        # the same SSSP under other stepping windows (the reference's initial
+        # window times each scale), and the scale the window autotuner picks
+        # from 4 seeded sample roots other than the query's
+        wt, wexact = {}, True
+        for k in (1.0, 0.5, 0.25, 0.125, 0.0625):
+            g.sssp_window(k)
+            ts = []
+            for _ in range(3):
+                dist_w = g.sssp(root, configs=args.configs_sssp, **sel)
+                ts.append(g.last_stats.device_ms)
+                wexact &= bool(np.array_equal(dist_w, dist))
+            wt[str(k)] = round(min(ts), 3)
+        rng = np.random.default_rng(7)
+        wsample = rng.choice(np.flatnonzero(np.diff(h.row_ptr) > 0), 4,
+                             replace=False).astype(np.int32)
+        wscale, _ = g.autotune_sssp(wsample, configs=args.configs_sssp, **sel)
+        g.sssp_window(1.0)
+        detail["sssp_window"] = {"query_ms_by_scale": wt, "bit_exact": wexact,
+                                 "autotuned_scale": wscale,
+                                 "autotune_roots": [int(x) for x in wsample]}
         # the same SSSP and PageRank under the rule trees
         t_r, t_pr = [], []
         for _ in range(args.steps):

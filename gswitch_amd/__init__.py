@@ -120,6 +120,9 @@ _SIGS = {
     "gsw_selector_load": (C.c_int32, [_P, C.c_int32]),
     "gsw_selector_load_app": (C.c_int32, [C.c_int32, _P, C.c_int32]),
     "gsw_last_class": (C.c_int32, [_P, _P]),
+    "gsw_sssp_set_window": (C.c_int32, [_P, C.c_double]),
+    "gsw_sssp_get_window": (C.c_int32, [_P, _P]),
+    "gsw_autotune_sssp": (C.c_int32, [_P, _P, C.c_int32, _P, _P, _P]),
     "gsw_selector_load_tree": (C.c_int32, [C.c_int32, C.c_int32, _P, _P, _P, _P, _P]),
     "gsw_mlp_features": (C.c_int32, [_P, _P]),
     "gsw_mlp_forward": (C.c_int32, [_P, _P, C.c_int64, _P]),
@@ -611,6 +614,24 @@ class Graph:
                                      C.byref(ms)))
         return dict(lb=LB_NAMES[lb.value], twc_small=s.value, twc_large=l.value,
                     grid=g.value, ms=ms.value)
+
+    def sssp_window(self, scale=None):
+        """The graph's SSSP window scale; with `scale`, set it first."""
+        if scale is not None:
+            _check(_lib.gsw_sssp_set_window(self._h, float(scale)))
+        k = C.c_double()
+        _check(_lib.gsw_sssp_get_window(self._h, C.byref(k)))
+        return k.value
+
+    def autotune_sssp(self, roots, configs=None, **kw):
+        """Pick (and keep) the SSSP window scale that is fastest from
+        `roots`; returns (scale, total ms)."""
+        r = np.ascontiguousarray(roots, np.int32)
+        o = self._opts(configs, kw)
+        k, ms = C.c_double(), C.c_double()
+        _check(_lib.gsw_autotune_sssp(self._h, _ptr(r), len(r), C.byref(o), C.byref(k),
+                                      C.byref(ms)))
+        return k.value, ms.value
 
     def last_class(self):
         """Class chosen by the learned selector in the latest PageRank or CC

@@ -200,6 +200,7 @@ struct gsw_graph {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<gsw_level_record> records;
   int32_t last_class = -1;  // per-run learned choice (PageRank, CC)
+  double sssp_window_scale = 1.0;  // SSSP initial window x this
   // 64-source BFS state (allocated on first use)
   uint64_t *ms_buf = nullptr;  // visit | A | B, n each
   gsw::ms_counters_t *ms_cnt = nullptr;
@@ -731,6 +732,10 @@ void sssp_prepare(gsw_graph *G, app_ctx_t<SSSP> &c, const run_opts_t &o,
   c.f.data.window = window_t{};
   view_t<GW> gv{&G->hg, G->attr, G->mean_weight};
   init_conf(c.en.stats, fets, c.en.conf, gv, c.f);
+  // the graph's window scale (gsw_sssp_set_window / gsw_autotune_sssp)
+  // multiplies init_conf's initial window; distances do not depend on it
+  if (G->sssp_window_scale != 1.0)
+    c.f.data.window.set_init_winsize(c.f.data.window.wsz_lim * G->sssp_window_scale);
 }
 
 // The engine for SSSP level `lvl` in the state the level loop carries into
@@ -1544,6 +1549,61 @@ gsw_status gsw_autotune_bfs(gsw_graph *G, const int32_t *roots, int32_t nroots,
           *best_grid = grids[gi];
         }
       }
+    *best_ms = best;
+  });
+}
+
+// ---- SSSP stepping window per graph --------------------------------------
+gsw_status gsw_sssp_set_window(gsw_graph *G, double scale) {
+  return guarded([&] {
+    GSW_REQUIRE(G && scale > 0 && scale <= 1024, GSW_E_INVALID, "invalid window scale");
+    G->sssp_window_scale = scale;
+  });
+}
+
+gsw_status gsw_sssp_get_window(const gsw_graph *G, double *scale) {
+  return guarded([&] {
+    GSW_REQUIRE(G && scale, GSW_E_INVALID, "null argument");
+    *scale = G->sssp_window_scale;
+  });
+}
+
+gsw_status gsw_autotune_sssp(gsw_graph *G, const int32_t *roots, int32_t nroots,
+                             const gsw_run_opts *opts, double *best_scale,
+                             double *best_ms) {
+  return guarded([&] {
+    GSW_REQUIRE(G && roots && nroots > 0 && best_scale && best_ms, GSW_E_INVALID,
+                "invalid arguments");
+    GSW_REQUIRE(G->has_w, GSW_E_INVALID, "SSSP needs a weighted graph");
+    for (int32_t i = 0; i < nroots; ++i)
+      GSW_REQUIRE(roots[i] >= 0 && roots[i] < G->hg.n, GSW_E_INVALID,
+                  "root out of range");
+    gsw_run_opts o;
+    gsw_run_opts_init(&o);
+    if (opts) o = *opts;
+    o.profile = 0;
+    const double cands[] = {1.0, 0.5, 0.25, 0.125, 0.0625};
+    double best = 1e300, pick = 1.0;
+    for (double k : cands) {
+      G->sssp_window_scale = k;
+      double total = 0;
+      for (int32_t i = 0; i < nroots; ++i) {
+        double t = 1e300;
+        for (int rep = 0; rep < 2; ++rep) {
+          gsw_run_stats st{};
+          const gsw_status rc = gsw_sssp(G, roots[i], &o, nullptr, &st);
+          GSW_REQUIRE(rc == GSW_OK, rc, std::string("SSSP failed: ") + gsw_last_error());
+          t = std::min(t, st.device_ms);
+        }
+        total += t;
+      }
+      if (total < best) {
+        best = total;
+        pick = k;
+      }
+    }
+    G->sssp_window_scale = pick;
+    *best_scale = pick;
     *best_ms = best;
   });
 }

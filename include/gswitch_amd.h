@@ -262,6 +262,21 @@ gsw_status gsw_bfs_multi(gsw_graph *g, const int32_t *roots, int32_t nroots,
 gsw_status gsw_batch_bfs_cc(gsw_graph *g, const int64_t *offsets, int32_t count,
                             const gsw_run_opts *opts, int32_t *levels_out,
                             int32_t *labels_out, gsw_run_stats *stats);
+/* SSSP stepping window of a graph: gsw_sssp's initial window is the
+ * reference's init_conf size (8 * mean weight * 32 / mean degree,
+ * src/gswitch.h:19-63) times `scale` (default 1). The distances do not depend
+ * on it; the work per level does (a wide window re-relaxes vertices whose
+ * distance is not final yet, a narrow one runs more levels). No counterpart
+ * in the reference. */
+gsw_status gsw_sssp_set_window(gsw_graph *g, double scale);
+gsw_status gsw_sssp_get_window(const gsw_graph *g, double *scale);
+/* Autotuner for the window: times gsw_sssp from `roots` (best of 2 each,
+ * opts as given, e.g. the learned selector) under the scales 1, 1/2, 1/4,
+ * 1/8 and 1/16, keeps the fastest total as the graph's window scale and
+ * returns it. */
+gsw_status gsw_autotune_sssp(gsw_graph *g, const int32_t *roots, int32_t nroots,
+                             const gsw_run_opts *opts, double *best_scale,
+                             double *best_ms);
 /* Autotuner for the captured batched BFS: times {WM, TWC} x TWC bin
  * thresholds x workgroups per launch on this graph and roots (best of 3)
  * and returns the fastest (lb uses the gsw_level_record codes; feed the

@@ -140,6 +140,32 @@ def test_sssp_forced_variant(gsw, orc, wset, variant):
                           orc.spfa(h.row_ptr, h.col, h.w, root)), variant
 
 
+@pytest.mark.parametrize("gname", ["rmat12w", "gridw"])
+def test_sssp_window_scales(gsw, orc, wset, gname):
+    """The stepping window only shapes the work per level: every scale of
+    the initial window, and the autotuned one, give the oracle's distances
+    (the graph keeps its scale until changed)."""
+    h, g = wset[gname]
+    root = h.max_degree_vertex()
+    ref = orc.spfa(h.row_ptr, h.col, h.w, root)
+    try:
+        for k in (4.0, 1.0, 0.3, 0.0625, 0.001):
+            assert g.sssp_window(k) == k
+            for v in (None, "Push-Queue-WM-Standalone", "Push-Queue-CM-Fused"):
+                assert np.array_equal(g.sssp(root, configs=v), ref), (k, v)
+        k, ms = g.autotune_sssp([root, 0, h.n // 2])
+        assert k in (1.0, 0.5, 0.25, 0.125, 0.0625) and ms > 0
+        assert g.sssp_window() == k
+        assert np.array_equal(g.sssp(root), ref)
+        for bad in (0.0, -1.0):
+            with pytest.raises(gsw.GswError):
+                g.sssp_window(bad)
+        with pytest.raises(gsw.GswError):
+            g.autotune_sssp([h.n])
+    finally:
+        g.sssp_window(1.0)
+
+
 def test_sssp_requires_weights(gsw, gset):
     h, g = gset["er"]
     with pytest.raises(gsw.GswError):
