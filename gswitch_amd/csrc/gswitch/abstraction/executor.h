@@ -434,12 +434,22 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf,
         case TWOD: {
           if (use_pb) {
             // propagation blocking: messages to slots, then slots to targets
+            // group by group (consecutive bins): a group's phase 2 follows
+            // its own phase 1
             const pb_t &pb = *g.pb;
-            if (pb.n1 > 0)
-              hipLaunchKernelGGL((k_pb_scatter<Bd, MSG>), dim3(pb.n1), dim3(PB_BLOCK),
-                                 0, st, body, pb);
-            hipLaunchKernelGGL((k_pb_gather<Bd, MSG>), dim3(pb.n2), dim3(PB_BLOCK), 0,
-                               st, body, pb);
+            for (int64_t k = 0; k < pb.ng; ++k) {
+              pb_t q = pb;
+              q.p1 = pb.p1 + 4 * pb.g1h[k];
+              q.p2 = pb.p2 + 4 * pb.g2h[k];
+              const int64_t n1 = pb.g1h[k + 1] - pb.g1h[k];
+              const int64_t n2 = pb.g2h[k + 1] - pb.g2h[k];
+              if (n1 > 0)
+                hipLaunchKernelGGL((k_pb_scatter<Bd, MSG>), dim3(n1), dim3(PB_BLOCK),
+                                   0, st, body, q);
+              if (n2 > 0)
+                hipLaunchKernelGGL((k_pb_gather<Bd, MSG>), dim3(n2), dim3(PB_BLOCK), 0,
+                                   st, body, q);
+            }
             break;
           }
           // the slices one after another (each launch owns its targets)

@@ -128,7 +128,12 @@ struct pb_t {
   int64_t *p2 = nullptr;         // [n2][4]: bin, first slot, end slot, whole bin
   void *vals = nullptr;          // [m] messages (msg type of the functor)
   size_t vals_bytes = 0;
+  int64_t ng = 1;                // groups of consecutive bins (host arrays:
+  int64_t *g1h = nullptr;        //   [ng + 1] first phase-1 piece of each,
+  int64_t *g2h = nullptr;        //   [ng + 1] first phase-2 piece of each)
   void release() {
+    delete[] g1h;
+    delete[] g2h;
     hipFree(bin_lo);
     hipFree(src);
     hipFree(dst);

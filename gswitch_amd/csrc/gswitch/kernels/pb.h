@@ -512,10 +512,28 @@ template <typename G> void pb_ensure(G &g, int S, int D) {
   hipFree(cursor);
   hipFree(bsum);
   GSW_REQUIRE(off[T] == m, E_INTERNAL_CHECK, "PB tiles do not cover the edges");
-  // chunk-major tile table and phase-1 pieces (runs of bins of <= ecap edges)
+  // groups: runs of consecutive bins of about m / K edges (GSW_PB_GROUPS = K,
+  // default 1); an expand runs phase 1 then phase 2 group by group, so a
+  // group's phase 2 reads back messages its phase 1 has just written
+  int64_t K = 1;
+  if (const char *e = std::getenv("GSW_PB_GROUPS")) {
+    const long long x = std::atoll(e);
+    if (x > 1) K = MIN((int64_t)x, pb.B);
+  }
+  std::vector<int32_t> grp((size_t)pb.B);
+  {
+    int64_t k = 0;
+    for (int64_t b = 0; b < pb.B; ++b) {
+      // bin b starts a new group once the edges before it reach k+1 shares
+      while (k + 1 < K && off[b * pb.C] * K >= (k + 1) * m) ++k;
+      grp[b] = (int32_t)k;
+    }
+  }
+  // chunk-major tile table and phase-1 pieces (runs of bins of <= ecap
+  // edges within one group), listed group by group
   std::vector<int64_t> cs((size_t)T);
   std::vector<int32_t> cl((size_t)T);
-  std::vector<int32_t> p1;
+  std::vector<std::vector<int32_t>> p1g((size_t)K);
   for (int64_t c = 0; c < pb.C; ++c) {
     int64_t acc = 0, b0 = 0;
     for (int64_t b = 0; b < pb.B; ++b) {
@@ -523,14 +541,22 @@ template <typename G> void pb_ensure(G &g, int S, int D) {
       const int64_t len = off[t + 1] - off[t];
       cs[c * pb.B + b] = off[t];
       cl[c * pb.B + b] = (int32_t)len;
-      if (acc > 0 && acc + len > ecap) {
-        p1.insert(p1.end(), {(int32_t)c, (int32_t)b0, (int32_t)b, 0});
+      if (acc > 0 && (acc + len > ecap || grp[b] != grp[b0])) {
+        p1g[grp[b0]].insert(p1g[grp[b0]].end(), {(int32_t)c, (int32_t)b0, (int32_t)b, 0});
         b0 = b;
         acc = 0;
       }
+      if (acc == 0) b0 = b;
       acc += len;
     }
-    if (acc > 0) p1.insert(p1.end(), {(int32_t)c, (int32_t)b0, (int32_t)pb.B, 0});
+    if (acc > 0)
+      p1g[grp[b0]].insert(p1g[grp[b0]].end(), {(int32_t)c, (int32_t)b0, (int32_t)pb.B, 0});
+  }
+  std::vector<int32_t> p1;
+  std::vector<int64_t> g1(1, 0);
+  for (int64_t k = 0; k < K; ++k) {
+    p1.insert(p1.end(), p1g[k].begin(), p1g[k].end());
+    g1.push_back((int64_t)p1.size() / 4);
   }
   // phase-2 pieces: whole bins, or ecap-slot parts of a one-target bin
   std::vector<int64_t> p2;
@@ -545,6 +571,21 @@ template <typename G> void pb_ensure(G &g, int S, int D) {
   }
   pb.n1 = (int64_t)p1.size() / 4;
   pb.n2 = (int64_t)p2.size() / 4;
+  // phase-2 pieces are in bin order: group k starts at its first bin's piece
+  std::vector<int64_t> g2((size_t)K + 1);
+  for (int64_t k = 0, i = 0; k <= K; ++k) {
+    while (i < pb.n2 && grp[p2[4 * i]] < k) ++i;
+    g2[k] = i;
+  }
+  GSW_REQUIRE((int64_t)g1.size() == K + 1 && (int64_t)g2.size() == K + 1,
+              E_INTERNAL_CHECK, "PB groups");
+  pb.ng = K;
+  pb.g1h = new int64_t[K + 1];
+  pb.g2h = new int64_t[K + 1];
+  for (int64_t k = 0; k <= K; ++k) {
+    pb.g1h[k] = g1[k];
+    pb.g2h[k] = g2[k];
+  }
   // the pieces cover every edge once: phase 1 by chunk, phase 2 in order
   int64_t covered = 0;
   for (int64_t i = 0; i < pb.n1; ++i)

@@ -238,17 +238,21 @@ def test_pagerank_twod_slices(gsw, orc, monkeypatch, slice_kb):
         assert max_rel(pr, ref) <= PR_TOL, (slice_kb, pull_kernel, max_rel(pr, ref))
 
 
-@pytest.mark.parametrize("ecap", [0, 300])
-def test_pagerank_propagation_blocking(gsw, orc, monkeypatch, ecap):
+@pytest.mark.parametrize("ecap,groups", [(0, 0), (300, 0), (0, 3), (300, 5), (0, 64)])
+def test_pagerank_propagation_blocking(gsw, orc, monkeypatch, ecap, groups):
     """Propagation-blocked pulls (pull_kernel 2). R-MAT 15 has 32 768
     vertices, so its sources fall in two 16 384-id chunks and its targets
     in bins of at most 8 192; with pieces of 300 slots (GSW_PB_ECAP) the
     hubs (degrees in the thousands) become one-target bins split over
     several workgroups that add their parts atomically, and chunks are cut
     into several phase-1 pieces. Also a grid (no hubs) and a graph with
-    isolated vertices."""
+    isolated vertices. With GSW_PB_GROUPS the bins form groups whose phase 1
+    and phase 2 run group by group (a hub's one-target bin can hold more
+    than a group's share, so some groups are empty)."""
     if ecap:
         monkeypatch.setenv("GSW_PB_ECAP", str(ecap))
+    if groups:
+        monkeypatch.setenv("GSW_PB_GROUPS", str(groups))
     h = gsw.rmat(15, 16, seed=7)
     deg = np.diff(h.row_ptr)
     assert deg.max() > 3 * 300 and (deg == 0).any()
@@ -258,7 +262,7 @@ def test_pagerank_propagation_blocking(gsw, orc, monkeypatch, ecap):
         for rep in range(2):
             pr = g.pagerank(0.85, 30, configs="Pull-Bitmap-TWOD-Standalone",
                             pull_kernel=2)
-            assert max_rel(pr, ref) <= PR_TOL, (ecap, rep, max_rel(pr, ref))
+            assert max_rel(pr, ref) <= PR_TOL, (ecap, groups, rep, max_rel(pr, ref))
         g.close()
 
 
