@@ -1881,6 +1881,8 @@ gsw_status gsw_pagerank(gsw_graph *G, float damping, int32_t iters, float eps,
     view_t<GU> gv{&G->hg, G->attr, 0.0};
     init_conf(c.en.stats, fets, c.en.conf, gv, c.f);
     maybe_learned(c, o, G, true, true, GSW_APP_PR);
+    // with eps = 0 the filter keeps every vertex active: no per-level reads
+    c.en.conf.counts_known = eps == 0.f;
 
     hipStream_t st = gsw::stream();
     H_ERR(hipEventRecord(G->ev0, st));

@@ -83,6 +83,10 @@ struct config_t {
   // set by an expand that already inspected the next level (TM pull of a
   // level-synchronous functor): the inspector then only reads the counters
   bool conf_pre_inspected = false;
+  // every level has every vertex active (a from-all, to-all run whose filter
+  // never deactivates one, e.g. PageRank with eps = 0): the inspector runs
+  // the filter pass but reads no counters back, so levels need no host sync
+  bool counts_known = false;
   ins_t force;
   // one line per expand on stdout (the reference's --verbose)
   bool verbose = false;

@@ -77,6 +77,15 @@ struct inspector_t {
       if (conf.policy_mlp && stat.learned()) mlp_host_select(stat, fets, conf, g.level);
       need = false;
     }
+    if (need && conf.counts_known) {
+      hipLaunchKernelGGL((gsw::k_inspect_vc<device_graph_t<CSR, E>, F>),
+                         dim3(gsw::grid_for((g.nvertexs + 63) / 64 * 64)),
+                         dim3(gsw::BLOCK), 0, gsw::stream(), as, g, f,
+                         stat.slots());
+      H_ERR(hipGetLastError());
+      set_first_fets(as, fets, conf, (double)g.nvertexs, (double)g.nedges);
+      need = false;
+    }
     if (need && pre) {
       // the previous expand already classified this level (bitmaps written,
       // counters and, under the learned selector, the class in stat)
