@@ -143,22 +143,53 @@ __device__ __forceinline__ void do_found(const dobfs_t &d, const do_level_t &t, 
   }
 }
 
-// Top-down visit of u: the first visitor (atomic on the visited word) owns it.
-__device__ __forceinline__ void do_visit(const dobfs_t &d, const do_level_t &t, int u,
-                                         long long &nf, long long &mf) {
-  const int64_t w = u >> 6;
-  const uint64_t b = 1ull << (u & 63);
-  if (d.vis[w] & b) return;
-  const uint64_t old = atomicOr((unsigned long long *)&d.vis[w], (unsigned long long)b);
-  if (old & b) return;
-  atomicOr((unsigned long long *)&t.Fn[w], (unsigned long long)b);
-  do_found(d, t, u, nf, mf);
-}
-
 __device__ __forceinline__ uint64_t shfl_u64(uint64_t x, int src) {
   const int lo = __shfl((int)(uint32_t)x, src, 64);
   const int hi = __shfl((int)(uint32_t)(x >> 32), src, 64);
   return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// Top-down visits of a whole wavefront (every lane calls it; `valid` lanes
+// carry a target u). Lanes walk sorted adjacency, so consecutive lanes often
+// hit the same visited word: each run of consecutive lanes with one word
+// ORs its still-unvisited bits (segmented scan over the run) and its last
+// lane sets them with one atomicOr on the visited word and one on the next
+// frontier word, instead of two atomics per edge. A lane owns its target
+// if the word's old value lacks the bit and no lower lane of the run has
+// the same target (a target listed by two frontier vertices).
+__device__ __forceinline__ void do_visit_wave(const dobfs_t &d, const do_level_t &t,
+                                              bool valid, int u, long long &nf,
+                                              long long &mf) {
+  const int lane = lane_id();
+  const int64_t w = valid ? (int64_t)(u >> 6) : -1;
+  const uint64_t b = valid ? 1ull << (u & 63) : 0ull;
+  const bool cand = valid && !(d.vis[w] & b);
+  if (!ballot(cand)) return;
+  // runs of equal words among consecutive lanes
+  const long long wp = __shfl_up((long long)w, 1, 64);
+  const bool head = lane == 0 || wp != w;
+  const uint64_t heads = ballot(head);
+  const int start = 63 - __clzll((long long)(heads & ((2ull << lane) - 1)));
+  uint64_t x = cand ? b : 0ull;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const uint64_t y = __shfl_up(x, o, 64);
+    if (lane - o >= start) x |= y;
+  }
+  // x: OR of the run's candidate bits up to this lane
+  const uint64_t up = __shfl_up(x, 1, 64);
+  const uint64_t below = lane > start ? up : 0ull;
+  const bool tail = lane == WAVE - 1 || ((heads >> (lane + 1)) & 1ull);
+  uint64_t old = 0;
+  if (tail && x) {
+    old = atomicOr((unsigned long long *)&d.vis[w], (unsigned long long)x);
+    const uint64_t fresh = x & ~old;
+    if (fresh) atomicOr((unsigned long long *)&t.Fn[w], (unsigned long long)fresh);
+  }
+  const uint64_t tails = ballot(tail);
+  const int my_tail = __builtin_ctzll(tails & ~((1ull << lane) - 1));
+  old = __shfl(old, my_tail, 64);
+  if (cand && !(old & b) && !(below & b)) do_found(d, t, u, nf, mf);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_do_expand(dobfs_t d) {
@@ -191,12 +222,17 @@ __global__ void __launch_bounds__(BLOCK) k_do_expand(dobfs_t d) {
     if (HE > 0) {
       const int hn = (int)ctrl[DO_HN];
       const int32_t *hcur = d.heavy + (t.L & 1) * DO_HCAP;
-      for (int64_t k = tid; k < HE; k += stride) {
-        ++ne;
-        const int o = owner_search(d.hpre, hn, k);
-        const int v = hcur[o];
-        const int u = d.cols[d.rows[v] + (k - d.hpre[o])];
-        do_visit(d, t, u, nf, mf);
+      for (int64_t k0 = tid - lane; k0 < HE; k0 += stride) {
+        const int64_t k = k0 + lane;
+        const bool ok = k < HE;
+        int u = 0;
+        if (ok) {
+          ++ne;
+          const int o = owner_search(d.hpre, hn, k);
+          const int v = hcur[o];
+          u = d.cols[d.rows[v] + (k - d.hpre[o])];
+        }
+        do_visit_wave(d, t, ok, u, nf, mf);
       }
     }
     // the other frontier vertices: 64 words per wavefront step
@@ -230,11 +266,12 @@ __global__ void __launch_bounds__(BLOCK) k_do_expand(dobfs_t d) {
         wave_sync();
         for (int j0 = 0; j0 < tot; j0 += WAVE) {
           const int k = j0 + lane;
+          int u = 0;
           if (k < tot) {
             const int o = owner_search(&s_scan[off], WAVE, k);
-            const int u = d.cols[s_start[off + o] + (k - s_scan[off + o])];
-            do_visit(d, t, u, nf, mf);
+            u = d.cols[s_start[off + o] + (k - s_scan[off + o])];
           }
+          do_visit_wave(d, t, k < tot, u, nf, mf);
         }
         wave_sync();
         // heavy vertices of an overflowed list: one wavefront each
@@ -244,9 +281,11 @@ __global__ void __launch_bounds__(BLOCK) k_do_expand(dobfs_t d) {
           al &= al - 1;
           const int va = __shfl(v, a, 64);
           const int64_t s = d.rows[va], e = d.rows[va + 1];
-          for (int64_t k = s + lane; k < e; k += WAVE) {
-            ++ne;
-            do_visit(d, t, d.cols[k], nf, mf);
+          for (int64_t k0 = s; k0 < e; k0 += WAVE) {
+            const int64_t k = k0 + lane;
+            const bool ok = k < e;
+            if (ok) ++ne;
+            do_visit_wave(d, t, ok, ok ? d.cols[k] : 0, nf, mf);
           }
         }
       }
