@@ -49,6 +49,9 @@ constexpr int DO_STRIDE = 16;    // long longs per slot
 constexpr int DO_DECIDE_BLOCK = 1024;
 constexpr int DO_HIST = 4096;    // levels with a history row
 constexpr int DO_HIST_W = 4;     // direction, frontier, edges, vertices
+constexpr int DO_BU = 2;         // frontier words per wavefront step (bottom-up;
+                                 // R-MAT 24 level 2: 1 word 178 us, 2 words 169 us,
+                                 // 4 words 251 us: the arrays leave registers)
 
 // control block (long long words)
 enum : int {
@@ -291,21 +294,34 @@ __global__ void __launch_bounds__(BLOCK) k_do_expand(dobfs_t d) {
       }
     }
   } else {
-    // bottom-up: one frontier word per wavefront step
-    for (int64_t w = wave; w < d.nw; w += nwaves) {
-      const int64_t v = (w << 6) + lane;
-      const uint64_t vw = d.vis[w];
-      bool found = false;
-      if (v < d.n && !((vw >> lane) & 1ull)) {
-        ++nv;
-        const int id = d.ideg[v];
-        if (id > 0) {
-          int u = d.first[v];
+    // bottom-up: DO_BU consecutive frontier words per wavefront step, their
+    // first probes issued together (a probe is a chain of dependent loads)
+    for (int64_t w0 = wave * DO_BU; w0 < d.nw; w0 += nwaves * DO_BU) {
+      uint64_t vw[DO_BU];
+      int id[DO_BU], fu[DO_BU];
+#pragma unroll
+      for (int j = 0; j < DO_BU; ++j) vw[j] = w0 + j < d.nw ? d.vis[w0 + j] : ~0ull;
+#pragma unroll
+      for (int j = 0; j < DO_BU; ++j) {
+        const int64_t v = ((w0 + j) << 6) + lane;
+        const bool act = v < d.n && !((vw[j] >> lane) & 1ull);
+        id[j] = act ? d.ideg[v] : 0;
+        fu[j] = act && id[j] > 0 ? d.first[v] : -1;
+        if (act) ++nv;
+      }
+#pragma unroll
+      for (int j = 0; j < DO_BU; ++j) {
+        const int64_t w = w0 + j;
+        if (w >= d.nw) break;  // uniform
+        const int64_t v = (w << 6) + lane;
+        bool found = false;
+        int u = fu[j];
+        if (u >= 0) {
           ++ne;
           found = (t.Fc[u >> 6] >> (u & 63)) & 1ull;
-          if (!found && id > 1) {
+          if (!found && id[j] > 1) {
             const int64_t s = d.rrows[v];
-            for (int k = 1; k < id; ++k) {
+            for (int k = 1; k < id[j]; ++k) {
               u = d.rcols[s + k];
               ++ne;
               if ((t.Fc[u >> 6] >> (u & 63)) & 1ull) {
@@ -315,13 +331,13 @@ __global__ void __launch_bounds__(BLOCK) k_do_expand(dobfs_t d) {
             }
           }
         }
+        const uint64_t fm = ballot(found);
+        if (lane == 0) {
+          t.Fn[w] = fm;
+          if (fm) d.vis[w] = vw[j] | fm;
+        }
+        if (found) do_found(d, t, (int)v, nf, mf);
       }
-      const uint64_t fm = ballot(found);
-      if (lane == 0) {
-        t.Fn[w] = fm;
-        if (fm) d.vis[w] = vw | fm;
-      }
-      if (found) do_found(d, t, (int)v, nf, mf);
     }
   }
   // counters of level L + 1 into this workgroup's slot
