@@ -1584,6 +1584,15 @@ gsw_status gsw_autotune_sssp(gsw_graph *G, const int32_t *roots, int32_t nroots,
     o.profile = 0;
     const double cands[] = {1.0, 0.5, 0.25, 0.125, 0.0625};
     double best = 1e300, pick = 1.0;
+    // a failed run leaves the graph's scale as it was
+    struct keep_t {
+      gsw_graph *g;
+      double k;
+      bool done = false;
+      ~keep_t() {
+        if (!done) g->sssp_window_scale = k;
+      }
+    } keep{G, G->sssp_window_scale};
     for (double k : cands) {
       G->sssp_window_scale = k;
       double total = 0;
@@ -1603,6 +1612,7 @@ gsw_status gsw_autotune_sssp(gsw_graph *G, const int32_t *roots, int32_t nroots,
       }
     }
     G->sssp_window_scale = pick;
+    keep.done = true;
     *best_scale = pick;
     *best_ms = best;
   });

@@ -42,7 +42,8 @@ namespace gsw {
 
 constexpr int PB_BLOCK = 1024;
 constexpr int PB_LDS_BYTES = 64 << 10;  // per phase: messages / accumulators
-constexpr int PB_QUADS = 2;             // aligned 4-slot quads a lane keeps in flight
+constexpr int PB_QUADS = 2;             // aligned 4-slot quads a lane keeps in flight (phase 1)
+constexpr int PB_GQUADS = 4;            // the same in phase 2
 
 // four consecutive slots, loaded and stored as one access
 template <typename T> struct alignas(4 * sizeof(T)) quad_t {
@@ -342,12 +343,12 @@ __global__ void __launch_bounds__(PB_BLOCK) k_pb_gather(Bd body, pb_t pb) {
     const int64_t nq = (a1 - a0) >> 2;
     const quad_t<MSG> *vq = (const quad_t<MSG> *)(vals + a0);
     const quad_t<uint16_t> *dq = (const quad_t<uint16_t> *)(pb.dst + a0);
-    constexpr int64_t STEP = (int64_t)PB_BLOCK * PB_QUADS;
-    quad_t<MSG> mv[PB_QUADS], mn[PB_QUADS];
-    quad_t<uint16_t> di[PB_QUADS], dn[PB_QUADS];
+    constexpr int64_t STEP = (int64_t)PB_BLOCK * PB_GQUADS;
+    quad_t<MSG> mv[PB_GQUADS], mn[PB_GQUADS];
+    quad_t<uint16_t> di[PB_GQUADS], dn[PB_GQUADS];
     auto load = [&](int64_t q0, quad_t<MSG> *m, quad_t<uint16_t> *d) {
 #pragma unroll
-      for (int j = 0; j < PB_QUADS; ++j) {
+      for (int j = 0; j < PB_GQUADS; ++j) {
         const int64_t q = q0 + (int64_t)j * PB_BLOCK + t;
         const int64_t qq = q < nq ? q : nq - 1;
         m[j] = vq[qq];
@@ -362,7 +363,7 @@ __global__ void __launch_bounds__(PB_BLOCK) k_pb_gather(Bd body, pb_t pb) {
     for (int64_t q0 = 0; q0 < nq; q0 += STEP) {
       if (q0 + STEP < nq) load(q0 + STEP, mn, dn);
 #pragma unroll
-      for (int j = 0; j < PB_QUADS; ++j)
+      for (int j = 0; j < PB_GQUADS; ++j)
         if (q0 + (int64_t)j * PB_BLOCK + t < nq) {
           // slots of one target are mostly consecutive (a target's in-edges
           // from one source chunk form a run), so a quad's slots of one
@@ -382,7 +383,7 @@ __global__ void __launch_bounds__(PB_BLOCK) k_pb_gather(Bd body, pb_t pb) {
           body.f.compAtomic(&s_acc[d], acc, body.g);
         }
 #pragma unroll
-      for (int j = 0; j < PB_QUADS; ++j) {
+      for (int j = 0; j < PB_GQUADS; ++j) {
         mv[j] = mn[j];
         di[j] = dn[j];
       }
