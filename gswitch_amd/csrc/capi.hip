@@ -2073,8 +2073,14 @@ gsw_status gsw_msbfs64(gsw_graph *G, const int32_t *sources, int32_t nsrc,
     hipLaunchKernelGGL(k_msbfs_seed, dim3(1), dim3(64), 0, st, visit, front,
                        (const int32_t *)G->ms_src, nsrc);
     double front_deg = 0;
-    for (int s = 0; s < nsrc; ++s)
-      front_deg += (double)(G->hg.row_ptr[sources[s] + 1] - G->hg.row_ptr[sources[s]]);
+    ms_seeds_t seeds{};
+    seeds.nsrc = nsrc;
+    for (int s = 0; s < nsrc; ++s) {
+      const int64_t ds = G->hg.row_ptr[sources[s] + 1] - G->hg.row_ptr[sources[s]];
+      seeds.src[s] = sources[s];
+      seeds.pre[s + 1] = seeds.pre[s] + ds;
+      front_deg += (double)ds;
+    }
     G->records.clear();
     int level = 0;
     const int grid = grid_for((n + 63) / 64 * 64);
@@ -2093,6 +2099,14 @@ gsw_status gsw_msbfs64(gsw_graph *G, const int32_t *sources, int32_t nsrc,
                            (const int64_t *)G->dg.dgr_row_ptr,
                            (const int32_t *)G->dg.dgr_adj_list, rows, n, all,
                            visit, (const uint64_t *)front, next, G->ms_cnt);
+      } else if (level == 1) {
+        // the sources' edges spread over the grid (they are often hubs)
+        fill_async<uint64_t>(next, 0, n + 1);
+        hipLaunchKernelGGL(k_msbfs_push_seeds, dim3(grid_for((int64_t)front_deg)),
+                           dim3(BLOCK), 0, st, rows, cols, (const uint64_t *)visit,
+                           (const uint64_t *)front, next, seeds);
+        hipLaunchKernelGGL(k_msbfs_sweep, dim3(grid), dim3(BLOCK), 0, st, rows,
+                           n, visit, next, G->ms_cnt);
       } else {
         fill_async<uint64_t>(next, 0, n + 1);
         hipLaunchKernelGGL(k_msbfs_push, dim3(grid), dim3(BLOCK), 0, st, rows,

@@ -183,6 +183,50 @@ __global__ void __launch_bounds__(BLOCK)
   }
 }
 
+// First top-down level: the frontier is the sources, which are often the
+// graph's hubs, so their edges are spread evenly over the whole grid (edge k
+// of the sources' concatenated lists, found by binary search over their
+// degree prefix) instead of one wavefront per source vertex.
+struct ms_seeds_t {
+  int nsrc;
+  int32_t src[64];
+  int64_t pre[65];  // exclusive prefix of the sources' degrees
+};
+
+__global__ void __launch_bounds__(BLOCK)
+    k_msbfs_push_seeds(const int64_t *__restrict__ rows,
+                       const int32_t *__restrict__ cols,
+                       const uint64_t *__restrict__ visit,
+                       const uint64_t *__restrict__ front, uint64_t *next,
+                       ms_seeds_t sd) {
+  __shared__ int64_t s_pre[65];
+  __shared__ int64_t s_row[64];
+  __shared__ uint64_t s_bits[64];
+  const int t = threadIdx.x;
+  if (t <= sd.nsrc) s_pre[t] = sd.pre[t];
+  if (t < sd.nsrc) {
+    s_row[t] = rows[sd.src[t]];
+    s_bits[t] = front[sd.src[t]];
+  }
+  __syncthreads();
+  const int64_t total = s_pre[sd.nsrc];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + t; k < total; k += stride) {
+    int lo = 0, hi = sd.nsrc;  // the last source with s_pre[lo] <= k
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (s_pre[mid] <= k)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const int32_t u = cols[s_row[lo] + (k - s_pre[lo])];
+    const uint64_t b = s_bits[lo] & ~visit[u];
+    if (b && (next[u] & b) != b)
+      atomicOr((unsigned long long *)&next[u], (unsigned long long)b);
+  }
+}
+
 // Top-down level, sweep part: keep only new bits, fold them into visit.
 __global__ void __launch_bounds__(BLOCK)
     k_msbfs_sweep(const int64_t *__restrict__ rows, int64_t n,
