@@ -397,9 +397,26 @@ def extended(gsw, args):
         g = gsw.Graph(h)
         roots = off[:-1].astype(np.int32)
         if tune is None:
+            # the autotuned captured push loop, or the direction-optimising
+            # device loop from all the roots, whichever is faster here
             tune = g.autotune_bfs(roots, capture=8)
             kw = dict(configs="Push-x-%s-x" % tune["lb"], capture=8, grid=tune["grid"],
                       twc_small=tune["twc_small"], twc_large=tune["twc_large"])
+            t_do = []
+            for _ in range(3):
+                g.batch_bfs_cc(off, out=False, profile=False, capture=8)
+                t_do.append(g.last_stats.device_ms)
+            t_push = []
+            for _ in range(3):
+                g.batch_bfs_cc(off, out=False, profile=False, **kw)
+                t_push.append(g.last_stats.device_ms)
+            tune["bfs_cc_ms_push"] = round(min(t_push), 3)
+            tune["bfs_cc_ms_direction_optimising"] = round(min(t_do), 3)
+            if min(t_do) < min(t_push):
+                kw = dict(capture=8)
+                tune["chosen"] = "direction-optimising device loop"
+            else:
+                tune["chosen"] = "captured push loop"
         lvb, labb = g.batch_bfs_cc(off, **kw)
         bms = []
         for _ in range(3):
@@ -457,7 +474,8 @@ def extended(gsw, args):
     log("batch: %d graphs in %d pieces, host build %.1fs" % (args.batch, pieces, t_gen))
     out["er_batch"] = {"graphs": args.batch, "pieces": pieces, "n": n_tot,
                        "m_undirected": m_tot, "autotune": tune,
-                       "captured": "BFS level loop + CC launch sequence (gsw_batch_bfs_cc)",
+                       "captured": "BFS level loop (%s) + CC launch sequence "
+                                   "(gsw_batch_bfs_cc)" % tune["chosen"],
                        "batched_ms": round(ms_batch, 3),
                        "batched_gteps": round((e_bfs + e_cc) / (ms_batch * 1e-3) / 1e9, 2),
                        "per_graph_loop_ms": round(t_loop, 3),

@@ -922,15 +922,22 @@ gsw::dobfs_t do_view(gsw_graph *G) {
 // launch, then `per_launch` captured levels (expand + decide) per hipGraph
 // replay and one read of the control block per replay. Returns the number
 // of non-empty frontiers expanded.
-int bfs_device_run(gsw_graph *G, int32_t root, int per_launch, int grid, int max_levels) {
+// root < 0: the `nroots` distinct roots at G->d_roots (upload_roots), every
+// vertex's level its distance to the nearest root.
+int bfs_device_run(gsw_graph *G, int32_t root, int per_launch, int grid, int max_levels,
+                   int64_t nroots = 0) {
   using namespace gsw;
   GSW_REQUIRE(per_launch >= 1 && per_launch <= 256 && grid >= 1 && grid <= MAX_GRID,
               GSW_E_INVALID, "invalid device-loop shape");
+  GSW_REQUIRE(root >= 0 || (nroots >= 1 && G->d_roots), E_INTERNAL_CHECK, "no roots");
   const dobfs_t d = do_view(G);
   hipStream_t st = stream();
   H_ERR(hipEventRecord(G->ev0, st));
   hipLaunchKernelGGL(k_do_reset, dim3(grid_for(MAX(d.n, d.nw))), dim3(BLOCK), 0, st, d,
                      root);
+  if (root < 0)
+    hipLaunchKernelGGL(k_do_seed, dim3(grid_for(nroots)), dim3(BLOCK), 0, st, d,
+                       (const int32_t *)G->d_roots, nroots);
   H_ERR(hipGetLastError());
   const double key[5] = {(double)per_launch, (double)grid, (double)d.heavy_deg,
                          (double)d.alpha, (double)d.beta};
@@ -1491,10 +1498,21 @@ gsw_status gsw_batch_bfs_cc(gsw_graph *G, const int64_t *offsets, int32_t count,
     }
     H_ERR(hipMemcpy(G->inst_off, offsets, sizeof(int64_t) * (count + 1),
                     hipMemcpyHostToDevice));
-    // BFS: captured fused level loop from every instance's first vertex
-    const int levels = bfs_captured_run(G, (int64_t)roots.size(), lb, o.twc_small,
-                                        o.twc_large, grid, 2 * capture,
-                                        o.max_levels);
+    // BFS from every instance's first vertex: a forced push runs the
+    // captured fused level loop; otherwise the device-resident
+    // direction-optimising loop from all the roots at once
+    int levels;
+    if (o.force.has_direction) {
+      GSW_REQUIRE(o.force.d == Push, GSW_E_UNSUPPORTED,
+                  "the captured batch loop pushes; leave the direction free for the "
+                  "direction-optimising loop");
+      levels = bfs_captured_run(G, (int64_t)roots.size(), lb, o.twc_small, o.twc_large,
+                                grid, 2 * capture, o.max_levels);
+    } else {
+      const int dgrid = (opts && opts->grid > 0) ? opts->grid : default_do_grid(G);
+      levels = bfs_device_run(G, -1, capture, dgrid, o.max_levels,
+                              (int64_t)roots.size());
+    }
     // CC: one captured launch sequence over the same graph
     cc_batched_captured(G, count);
     H_ERR(hipEventRecord(G->ev1, gsw::stream()));

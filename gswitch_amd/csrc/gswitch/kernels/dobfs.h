@@ -94,6 +94,8 @@ struct dobfs_t {
 // Query reset: levels -1 (root 0), visited and frontier words zero except
 // the root's bit, control block for level 0 (top-down from the root; the
 // root is the heavy list of level 0 when its degree is above heavy_deg).
+// root < 0: every vertex unreached and every word zero; the roots are then
+// set by k_do_seed.
 __global__ void __launch_bounds__(BLOCK) k_do_reset(dobfs_t d, int root) {
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -110,7 +112,7 @@ __global__ void __launch_bounds__(BLOCK) k_do_reset(dobfs_t d, int root) {
   if (blockIdx.x == 0) {
     for (int i = threadIdx.x; i < DO_TOTAL; i += blockDim.x) d.ctrl[i] = 0;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && root >= 0) {
       const long long deg = d.odeg[root];
       d.ctrl[DO_NF] = 1;
       d.ctrl[DO_MF] = deg;
@@ -123,6 +125,40 @@ __global__ void __launch_bounds__(BLOCK) k_do_reset(dobfs_t d, int root) {
         d.ctrl[DO_HE] = deg;
       }
     }
+  }
+}
+
+// Several roots (distinct ids, after k_do_reset with root < 0): level 0,
+// their bits in the visited and level-0 frontier words, and the level's
+// counters. Roots of more than heavy_deg edges are not listed: the heavy
+// list is marked overflowed, so each is walked by its own wavefront.
+__global__ void __launch_bounds__(BLOCK)
+    k_do_seed(dobfs_t d, const int32_t *__restrict__ roots, int64_t k) {
+  __shared__ long long sh[BLOCK / WAVE];
+  long long nf = 0, mf = 0;
+  bool heavy = false;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int r = roots[i];
+    d.lv[r] = 0;
+    const unsigned long long b = 1ull << (r & 63);
+    atomicOr((unsigned long long *)&d.vis[r >> 6], b);
+    atomicOr((unsigned long long *)&d.F[r >> 6], b);
+    const int deg = d.odeg[r];
+    ++nf;
+    mf += deg;
+    heavy |= deg > d.heavy_deg;
+  }
+  const long long a = block_sum(nf, sh);
+  const long long c = block_sum(mf, sh);
+  const int hv = __syncthreads_or(heavy);
+  if (threadIdx.x == 0) {
+    if (a) atomicAdd((unsigned long long *)&d.ctrl[DO_NF], (unsigned long long)a);
+    if (c) {
+      atomicAdd((unsigned long long *)&d.ctrl[DO_MF], (unsigned long long)c);
+      atomicAdd((unsigned long long *)&d.ctrl[DO_VISE], (unsigned long long)c);
+    }
+    if (hv) atomicOr((unsigned long long *)&d.ctrl[DO_HOVF], 1ull);
   }
 }
 

@@ -251,8 +251,11 @@ gsw_status gsw_bfs_multi(gsw_graph *g, const int32_t *roots, int32_t nroots,
 /* Batched BFS + CC over a disjoint union of `count` instances (instance i =
  * vertices offsets[i] .. offsets[i+1]-1, offsets[0] = 0, offsets[count] = n,
  * as gsw_gen_er_batch lays them out): BFS levels from every instance's
- * first vertex through the captured fused level loop (opts->configs LB WM or
- * TWC, opts->capture, opts->grid, TWC thresholds), then connected
+ * first vertex, by the device-resident direction-optimising loop from all
+ * the roots at once (the direction left free; opts->capture levels per
+ * replay, opts->grid), or with a forced "Push-..." by the captured fused
+ * push loop (opts->configs LB WM or TWC, opts->capture, opts->grid, TWC
+ * thresholds); then connected
  * components by sampled hooking as one captured launch sequence, skipping
  * in each instance the edges inside the component of its first vertex
  * (edges that leave an instance are always linked). Each instance's levels

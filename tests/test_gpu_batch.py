@@ -69,11 +69,14 @@ def test_batched_cc(gsw, orc, batch):
 
 
 @pytest.mark.parametrize("configs,capture,grid", [
-    ("Push-x-WM-x", 1, 0), ("Push-x-WM-x", 8, 7), ("Push-x-TWC-x", 4, 0)])
+    ("Push-x-WM-x", 1, 0), ("Push-x-WM-x", 8, 7), ("Push-x-TWC-x", 4, 0),
+    (None, 8, 0), (None, 3, 5), ("x-x-WM-x", 1, 0)])
 def test_batch_bfs_cc(gsw, orc, batch, configs, capture, grid):
     """BFS from every instance's first vertex and CC in one call (captured
-    BFS loop, captured CC sequence): every instance equals its own oracle
-    runs, also on repeated calls (the captured CC is replayed)."""
+    push loop when the direction is forced, else the device-resident
+    direction-optimising loop from all the roots; captured CC sequence):
+    every instance equals its own oracle runs, also on repeated calls (the
+    captured sequences are replayed)."""
     h, off, g = batch
     for rep in range(2):
         lv, lab = g.batch_bfs_cc(off, configs=configs, capture=capture, grid=grid)
@@ -101,6 +104,8 @@ def test_batch_bfs_cc_instances_sharing_edges(gsw, orc):
         x = orc.bfs(h.row_ptr, h.col, int(r))
         exp = np.minimum(exp, np.where(x < 0, big, x))
     assert np.array_equal(lv, np.where(exp == big, -1, exp).astype(np.int32))
+    with pytest.raises(gsw.GswError):  # the direction-optimising loop pulls too
+        g.batch_bfs_cc(np.array([0, 700, h.n], np.int64), configs="Pull-x-x-x")
     with pytest.raises(gsw.GswError):
         g.batch_bfs_cc(np.array([0, 5], np.int64))  # does not end at n
     with pytest.raises(gsw.GswError):

@@ -158,8 +158,11 @@ def check_instances(orc, h, off, lv, lab):
     assert not bad, "instances differ from the oracle: %s" % bad[:10]
 
 
-def run_batch(g, off, tune=None):
+def run_batch(g, off, tune=None, direction_optimising=False):
     roots = off[:-1].astype(np.int32)
+    if direction_optimising:
+        lv, lab = g.batch_bfs_cc(off, capture=8)
+        return lv, lab, tune
     if tune is None:
         tune = g.autotune_bfs(roots, capture=8)
     lv, lab = g.batch_bfs_cc(off, configs="Push-x-%s-x" % tune["lb"], capture=8,
@@ -178,6 +181,9 @@ def test_c4b_er_batch_full_piece(gsw, orc):
     lv, lab, tune = run_batch(g, off)
     say("C4b piece 0: autotuned", tune, "- checking 1000 instances")
     check_instances(orc, h, off, lv, lab)
+    lv2, lab2, _ = run_batch(g, off, direction_optimising=True)
+    say("C4b piece 0: direction-optimising loop")
+    assert np.array_equal(lv2, lv) and np.array_equal(lab2, lab)
     g.close()
     del h, g
     # seeded samples of consecutive instances from the other 19 pieces
@@ -189,6 +195,8 @@ def test_c4b_er_batch_full_piece(gsw, orc):
         say("C4b sample: instances %d..%d" % (first, first + 15))
         lv, lab, _ = run_batch(gs, offs, tune)
         check_instances(orc, hs, offs, lv, lab)
+        lv2, lab2, _ = run_batch(gs, offs, direction_optimising=True)
+        assert np.array_equal(lv2, lv) and np.array_equal(lab2, lab)
         gs.close()
 
 
