@@ -200,6 +200,8 @@ struct gsw_graph {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<gsw_level_record> records;
   int32_t last_class = -1;  // per-run learned choice (PageRank, CC)
+  bool disj_known = false, disj = false;  // batch offsets with no edge between
+  uint64_t disj_key = 0;                  //   instances (key: their hash)
   double sssp_window_scale = 1.0;  // SSSP initial window x this
   // 64-source BFS state (allocated on first use)
   uint64_t *ms_buf = nullptr;  // visit | A | B, n each
@@ -1026,7 +1028,8 @@ void cc_sample_phase(const gsw_graph *G, int32_t *lab, hipStream_t st) {
   for (int r = 0; r < 2; ++r) {
     hipLaunchKernelGGL(k_cc_sample, dim3(grid), dim3(BLOCK), 0, st,
                        (const int64_t *)G->dg.dg_row_ptr,
-                       (const int32_t *)G->dg.dg_adj_list, n, r, lab);
+                       (const int32_t *)G->dg.dg_adj_list, n, r, lab,
+                       (const int *)nullptr);
     hipLaunchKernelGGL(k_cc_compress, dim3(grid), dim3(BLOCK), 0, st, lab, n);
   }
   H_ERR(hipGetLastError());
@@ -1041,7 +1044,7 @@ void cc_finish_phase(const gsw_graph *G, int32_t *lab, int skip_root,
   hipLaunchKernelGGL(k_cc_rest, dim3(grid_for((n + 63) / 64 * 64)), dim3(BLOCK), 0,
                      st, (const int64_t *)G->dg.dg_row_ptr,
                      (const int32_t *)G->dg.dg_adj_list, n, 2, skip_root, skip_dev,
-                     off, count, lab);
+                     off, count, lab, 0);
   hipLaunchKernelGGL(k_cc_compress, dim3(grid_for(n)), dim3(BLOCK), 0, st, lab, n);
   H_ERR(hipGetLastError());
 }
@@ -1072,6 +1075,55 @@ void cc_sampled_launch(const gsw_graph *G, int32_t *lab, int32_t *ids, int sampl
 void cc_sampled_run(gsw_graph *G) {
   cc_buffers(G);
   cc_sampled_launch(G, G->cc_lab, G->cc_ids, CC_SAMPLE, gsw::stream());
+}
+
+// Whether no edge of the graph leaves its instance (G->inst_off, count + 1
+// offsets): rows are sorted, so one look at each row's first and last
+// column decides. Cached per offsets.
+bool batch_disjoint(gsw_graph *G, const int64_t *offsets, int64_t count) {
+  uint64_t key = 1469598103934665603ull ^ (uint64_t)count;
+  for (int64_t i = 0; i <= count; ++i) key = (key ^ (uint64_t)offsets[i]) * 1099511628211ull;
+  if (G->disj_known && G->disj_key == key) return G->disj;
+  using namespace gsw;
+  hipStream_t st = stream();
+  set_one_async<int32_t>(G->d_flag, 0);
+  hipLaunchKernelGGL(k_cc_cross, dim3(grid_for((G->hg.n + 63) / 64 * 64)), dim3(BLOCK), 0,
+                     st, (const int64_t *)G->dg.dg_row_ptr,
+                     (const int32_t *)G->dg.dg_adj_list, G->hg.n,
+                     (const int64_t *)G->inst_off, count, G->d_flag);
+  H_ERR(hipGetLastError());
+  H_ERR(hipMemcpyAsync(G->h_flag, G->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  sync();
+  G->disj = G->h_flag[0] == 0;
+  G->disj_key = key;
+  G->disj_known = true;
+  return G->disj;
+}
+
+// Batched CC over instances no edge leaves, from the BFS levels of every
+// instance's first vertex (k_cc_seed_levels): the reached vertices are
+// labelled at once and skipped; the others are hooked (two sampled rounds,
+// then their remaining edges) and compressed.
+void cc_batched_from_levels(gsw_graph *G, int64_t count, const int32_t *lv) {
+  using namespace gsw;
+  cc_buffers(G);
+  hipStream_t st = stream();
+  const int64_t n = G->hg.n;
+  int32_t *lab = G->cc_lab;
+  hipLaunchKernelGGL(k_cc_seed_levels, dim3(grid_for((n + 63) / 64 * 64)), dim3(BLOCK), 0,
+                     st, lab, (const int *)lv, (const int64_t *)G->inst_off, count, n);
+  for (int r = 0; r < 2; ++r) {
+    hipLaunchKernelGGL(k_cc_sample, dim3(grid_for(n)), dim3(BLOCK), 0, st,
+                       (const int64_t *)G->dg.dg_row_ptr,
+                       (const int32_t *)G->dg.dg_adj_list, n, r, lab, (const int *)lv);
+    hipLaunchKernelGGL(k_cc_compress, dim3(grid_for(n)), dim3(BLOCK), 0, st, lab, n);
+  }
+  hipLaunchKernelGGL(k_cc_rest, dim3(grid_for((n + 63) / 64 * 64)), dim3(BLOCK), 0, st,
+                     (const int64_t *)G->dg.dg_row_ptr, (const int32_t *)G->dg.dg_adj_list,
+                     n, 2, -1, (const int *)nullptr, (const int64_t *)G->inst_off, count,
+                     lab, 1);
+  hipLaunchKernelGGL(k_cc_compress, dim3(grid_for(n)), dim3(BLOCK), 0, st, lab, n);
+  H_ERR(hipGetLastError());
 }
 
 // Batched CC over disjoint instances (offsets on the device): the whole
@@ -1513,8 +1565,12 @@ gsw_status gsw_batch_bfs_cc(gsw_graph *G, const int64_t *offsets, int32_t count,
       levels = bfs_device_run(G, -1, capture, dgrid, o.max_levels,
                               (int64_t)roots.size());
     }
-    // CC: one captured launch sequence over the same graph
-    cc_batched_captured(G, count);
+    // CC: from the BFS levels when no edge leaves an instance, otherwise one
+    // captured launch sequence over the same graph
+    if (batch_disjoint(G, offsets, count))
+      cc_batched_from_levels(G, count, G->bfs->f.data.dg_wa);
+    else
+      cc_batched_captured(G, count);
     H_ERR(hipEventRecord(G->ev1, gsw::stream()));
     const double ms = elapsed(G);
     G->records.clear();

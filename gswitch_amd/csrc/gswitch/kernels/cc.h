@@ -63,12 +63,14 @@ __global__ void __launch_bounds__(BLOCK) k_cc_init(int *p, int64_t n) {
     p[v] = (int)v;
 }
 
-// link every vertex with its r-th neighbour
+// link every vertex with its r-th neighbour (with `skip_lv`, only the
+// vertices whose skip_lv entry is negative)
 __global__ void __launch_bounds__(BLOCK)
     k_cc_sample(const int64_t *__restrict__ rows, const int32_t *__restrict__ cols,
-                int64_t n, int r, int *p) {
+                int64_t n, int r, int *p, const int *__restrict__ skip_lv) {
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += (int64_t)gridDim.x * blockDim.x) {
+    if (skip_lv && skip_lv[v] >= 0) continue;
     const int64_t s = rows[v];
     if (rows[v + 1] - s > r) cc_link((int)v, cols[s + r], p);
   }
@@ -170,17 +172,70 @@ __device__ __forceinline__ int64_t cc_instance(const int64_t *off, int64_t count
   return lo;
 }
 
+// Instance of vertex v of a wavefront whose lanes hold base .. base + 63:
+// one search for base (every lane the same addresses), then forward steps.
+__device__ __forceinline__ int64_t cc_instance_wave(const int64_t *off, int64_t count,
+                                                    int64_t base, int64_t v) {
+  int64_t i = cc_instance(off, count, base);
+  while (i + 1 < count && off[i + 1] <= v) ++i;
+  return i;
+}
+
+// For a batch of instances whose edges stay inside their instance (checked
+// by k_cc_cross) and BFS levels from every instance's first vertex: the
+// vertices the BFS reached form the component of that first vertex, the
+// smallest id of the instance, so that id is their label; every other
+// vertex starts as its own root.
+__global__ void __launch_bounds__(BLOCK)
+    k_cc_seed_levels(int *p, const int *__restrict__ lv, const int64_t *__restrict__ off,
+                     int64_t count, int64_t n) {
+  const int lane = lane_id();
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> LOG_WAVE;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> LOG_WAVE;
+  for (int64_t base = wave << 6; base < n; base += nwaves << 6) {
+    const int64_t v = base + lane;
+    if (v < n)
+      p[v] = lv[v] >= 0 ? (int)off[cc_instance_wave(off, count, base, v)] : (int)v;
+  }
+}
+
+// *flag = 1 if an edge leaves its source's instance.
+__global__ void __launch_bounds__(BLOCK)
+    k_cc_cross(const int64_t *__restrict__ rows, const int32_t *__restrict__ cols,
+               int64_t n, const int64_t *__restrict__ off, int64_t count, int *flag) {
+  const int lane = lane_id();
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> LOG_WAVE;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> LOG_WAVE;
+  bool out = false;
+  for (int64_t base = wave << 6; base < n; base += nwaves << 6) {
+    const int64_t v = base + lane;
+    int64_t s = 0, e = 0, lo = 0, hi = 0;
+    if (v < n) {
+      s = rows[v];
+      e = rows[v + 1];
+      const int64_t i = cc_instance_wave(off, count, base, v);
+      lo = off[i];
+      hi = off[i + 1];
+    }
+    // columns of a row are sorted: its first and last entries decide
+    if (e > s && (cols[s] < lo || cols[e - 1] >= hi)) out = true;
+  }
+  if (ballot(out) && lane == 0) atomicOr(flag, 1);
+}
+
 // Phase 3. skip_root >= 0: S = {v : p[v] == skip_root} (one graph);
 // skip_root == -1: S = {v : p[v] == first vertex of v's instance} with
 // `off` the count + 1 instance offsets; skip_root == -2: no skipping. A
 // non-null skip_dev replaces skip_root by *skip_dev (written by k_cc_mode).
 // Edges [from, deg) of each row are visited (the sampled ones are done).
 // One wavefront per 64 vertices; rows longer than CC_WIDE are walked by
-// the whole wavefront, one at a time.
+// the whole wavefront, one at a time. `disjoint`: no edge leaves an
+// instance (skip_root == -1), so a vertex of S links nothing.
 __global__ void __launch_bounds__(BLOCK)
     k_cc_rest(const int64_t *__restrict__ rows, const int32_t *__restrict__ cols,
               int64_t n, int from, int skip_arg, const int *skip_dev,
-              const int64_t *__restrict__ off, int64_t count, int *p) {
+              const int64_t *__restrict__ off, int64_t count, int *p,
+              int disjoint) {
   const int skip_root = skip_dev ? *skip_dev : skip_arg;
   const int lane = lane_id();
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> LOG_WAVE;
@@ -203,8 +258,9 @@ __global__ void __launch_bounds__(BLOCK)
         hi = off[i + 1];
         in_s = root == (int)lo;
       }
-      // one graph: a vertex of S has nothing to link
-      if (in_s && skip_root >= 0) e = s;
+      // one graph, or instances with no edge between them: a vertex of S
+      // has nothing to link
+      if (in_s && (skip_root >= 0 || disjoint)) e = s;
     }
     const bool wide = (e - s) > CC_WIDE;
     if (!wide)

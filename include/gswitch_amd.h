@@ -255,8 +255,11 @@ gsw_status gsw_bfs_multi(gsw_graph *g, const int32_t *roots, int32_t nroots,
  * the roots at once (the direction left free; opts->capture levels per
  * replay, opts->grid), or with a forced "Push-..." by the captured fused
  * push loop (opts->configs LB WM or TWC, opts->capture, opts->grid, TWC
- * thresholds); then connected
- * components by sampled hooking as one captured launch sequence, skipping
+ * thresholds); then connected components. When no edge leaves an
+ * instance (checked once per offsets: rows are sorted, so each row's first
+ * and last column decide), the vertices the BFS reached take their
+ * instance's first vertex as label and only the others are hooked;
+ * otherwise sampled hooking runs as one captured launch sequence, skipping
  * in each instance the edges inside the component of its first vertex
  * (edges that leave an instance are always linked). Each instance's levels
  * equal its own run_bfs from its first vertex (application/bfs.cu:78-102)
