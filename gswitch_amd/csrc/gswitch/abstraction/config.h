@@ -87,6 +87,12 @@ struct config_t {
   // never deactivates one, e.g. PageRank with eps = 0): the inspector runs
   // the filter pass but reads no counters back, so levels need no host sync
   bool counts_known = false;
+  // with counts_known, a propagation-blocked expand applies the functor's
+  // filter to every target after combining its messages (what the next
+  // level's inspection would do first; such a filter reads no level);
+  // filter_done tells that inspection
+  bool pb_fuse_filter = false;
+  bool filter_done = false;
   ins_t force;
   // one line per expand on stdout (the reference's --verbose)
   bool verbose = false;

@@ -394,6 +394,9 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf,
         launch(push_body<G, F, false>{as, g, f, conf});
     } else {
       using Bd = pull_body<G, F>;
+      // (one group only: with several, a later group's phase 1 would read
+      // ranks an earlier group's filter had already updated)
+      conf.pb_fuse_filter = use_pb && conf.counts_known && g.pb->ng == 1;
       Bd body{as, g, f, conf};
       switch (lb) {
         case TM:
@@ -449,6 +452,15 @@ void executor_t::dispatch(active_set_t &as, G &g, F &f, config_t &conf,
               if (n2 > 0)
                 hipLaunchKernelGGL((k_pb_gather<Bd, MSG>), dim3(n2), dim3(PB_BLOCK), 0,
                                    st, body, q);
+            }
+            if (conf.pb_fuse_filter) {
+              // targets of bins split over workgroups: filtered once all
+              // parts are added
+              if (pb.nsplit > 0)
+                hipLaunchKernelGGL((k_pb_filter_split<Bd>),
+                                   dim3(grid_for(pb.nsplit)), dim3(BLOCK), 0, st, body,
+                                   (const int32_t *)pb.split, pb.nsplit);
+              conf.filter_done = true;
             }
             break;
           }

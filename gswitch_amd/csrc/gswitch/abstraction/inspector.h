@@ -78,11 +78,15 @@ struct inspector_t {
       need = false;
     }
     if (need && conf.counts_known) {
-      hipLaunchKernelGGL((gsw::k_inspect_vc<device_graph_t<CSR, E>, F>),
-                         dim3(gsw::grid_for((g.nvertexs + 63) / 64 * 64)),
-                         dim3(gsw::BLOCK), 0, gsw::stream(), as, g, f,
-                         stat.slots());
-      H_ERR(hipGetLastError());
+      // (the filter pass is skipped when the last expand applied it)
+      if (!conf.filter_done) {
+        hipLaunchKernelGGL((gsw::k_inspect_vc<device_graph_t<CSR, E>, F>),
+                           dim3(gsw::grid_for((g.nvertexs + 63) / 64 * 64)),
+                           dim3(gsw::BLOCK), 0, gsw::stream(), as, g, f,
+                           stat.slots());
+        H_ERR(hipGetLastError());
+      }
+      conf.filter_done = false;
       set_first_fets(as, fets, conf, (double)g.nvertexs, (double)g.nedges);
       need = false;
     }

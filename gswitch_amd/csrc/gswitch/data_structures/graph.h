@@ -128,6 +128,8 @@ struct pb_t {
   int64_t *p2 = nullptr;         // [n2][4]: bin, first slot, end slot, whole bin
   void *vals = nullptr;          // [m] messages (msg type of the functor)
   size_t vals_bytes = 0;
+  int32_t *split = nullptr;      // [nsplit] targets of bins split over pieces
+  int64_t nsplit = 0;
   int64_t ng = 1;                // groups of consecutive bins (host arrays:
   int64_t *g1h = nullptr;        //   [ng + 1] first phase-1 piece of each,
   int64_t *g2h = nullptr;        //   [ng + 1] first phase-2 piece of each)
@@ -143,6 +145,7 @@ struct pb_t {
     hipFree(p1);
     hipFree(p2);
     hipFree(vals);
+    hipFree(split);
     *this = pb_t();
   }
 };

@@ -392,6 +392,7 @@ __global__ void __launch_bounds__(PB_BLOCK) k_pb_gather(Bd body, pb_t pb) {
     for (int i = threadIdx.x; i < w; i += PB_BLOCK) {
       const int v = lo + i;
       if (body.as.fetch(v, want) >= 0) *body.f.wa_of(v) = s_acc[i];
+      if (body.conf.pb_fuse_filter) (void)body.f.filter(v, body.g);
     }
   } else {
     // one target: reduce the part in registers, then across the workgroup
@@ -429,6 +430,14 @@ __global__ void __launch_bounds__(PB_BLOCK) k_pb_gather(Bd body, pb_t pb) {
   }
   add_work<PB_BLOCK>(body.as.dg_work, threadIdx.x == 0 ? (unsigned long long)(e1 - e0) : 0ull,
                      0ull);
+}
+
+// The functor's filter on the targets of split bins (pb_fuse_filter).
+template <class Bd>
+__global__ void __launch_bounds__(BLOCK)
+    k_pb_filter_split(Bd body, const int32_t *__restrict__ split, int64_t k) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < k) (void)body.f.filter(split[i], body.g);
 }
 
 // Build the layout of `g` for chunks of S sources and bins of at most D
@@ -561,9 +570,11 @@ template <typename G> void pb_ensure(G &g, int S, int D) {
   }
   // phase-2 pieces: whole bins, or ecap-slot parts of a one-target bin
   std::vector<int64_t> p2;
+  std::vector<int32_t> split;
   for (int64_t b = 0; b < pb.B; ++b) {
     const int64_t e0 = off[b * pb.C], e1 = off[(b + 1) * pb.C];
     if (blo[b + 1] - blo[b] == 1 && e1 - e0 > ecap) {
+      split.push_back(blo[b]);
       for (int64_t k = e0; k < e1; k += ecap)
         p2.insert(p2.end(), {b, k, MIN(e1, k + ecap), (int64_t)0});
     } else {
@@ -610,6 +621,11 @@ template <typename G> void pb_ensure(G &g, int S, int D) {
   if (!p1.empty())
     H_ERR(hipMemcpy(pb.p1, p1.data(), sizeof(int32_t) * p1.size(), hipMemcpyHostToDevice));
   H_ERR(hipMemcpy(pb.p2, p2.data(), sizeof(int64_t) * p2.size(), hipMemcpyHostToDevice));
+  pb.nsplit = (int64_t)split.size();
+  H_ERR(hipMalloc((void **)&pb.split, sizeof(int32_t) * (split.size() + 1)));
+  if (!split.empty())
+    H_ERR(hipMemcpy(pb.split, split.data(), sizeof(int32_t) * split.size(),
+                    hipMemcpyHostToDevice));
 }
 
 // Message storage of `bytes` per edge (grown on demand).
