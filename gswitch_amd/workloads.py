@@ -6,13 +6,18 @@
     python -m gswitch_amd.workloads cc   --er-batch 1000
     python -m gswitch_amd.workloads bfs  --grid 4096
     python -m gswitch_amd.workloads copy --mib 1024
+    python -m gswitch_amd.workloads msbfs --scale 22 --seed 4
+    python -m gswitch_amd.workloads batch --er-batch 5000
 
 Graphs are the seeded R-MAT (a, b, c) = (0.57, 0.19, 0.19), edge factor 16;
 the root is the highest-degree vertex. `--er-batch K` replaces the R-MAT graph
 by instances 0..K-1 of the seed-5 Erdos-Renyi batch (n log-uniform in
 [1000, 200000], mean degree uniform in [2, 32]) as one union; `--grid S`
 by the S x S grid with 10 % of its edges deleted and 1 % long-range edges
-(seed 3; root vertex 0). `copy` runs the plain device copy
+(seed 3; root vertex 0). `msbfs` is the 64-source bitmask BFS from the 64
+highest-degree vertices; `batch` the batched BFS from every instance's
+first vertex + CC of an ER batch union (gsw_batch_bfs_cc, the
+direction-optimising device loop). `copy` runs the plain device copy
 kernel used to calibrate hardware byte counters. Each run prints one JSON
 line with the device time and the per-expand records.
 """
@@ -26,7 +31,7 @@ import gswitch_amd as gsw
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("app", choices=["bfs", "sssp", "pr", "cc", "copy"])
+    ap.add_argument("app", choices=["bfs", "sssp", "pr", "cc", "copy", "msbfs", "batch"])
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--iters", type=int, default=50)
@@ -48,9 +53,10 @@ def main(argv=None):
         print(json.dumps({"app": "copy", "bytes": a.mib << 20, "GBps": g}))
         return
     t = time.perf_counter()
+    off = None
     if a.er_batch > 0:
-        h, _ = gsw.erdos_renyi_batch(a.er_batch, 1000, 200000, 2.0, 32.0, seed=5,
-                                     weighted=(a.app == "sssp"))
+        h, off = gsw.erdos_renyi_batch(a.er_batch, 1000, 200000, 2.0, 32.0, seed=5,
+                                       weighted=(a.app == "sssp"))
     elif a.grid > 0:
         h = gsw.grid2d(a.grid, a.grid, 0.1, 0.01, 3, weighted=(a.app == "sssp"))
     else:
@@ -69,8 +75,18 @@ def main(argv=None):
         kw["capture"] = a.capture
     print("graph built in %.1fs: n=%d m=%d" % (time.perf_counter() - t, h.n, h.m),
           file=sys.stderr, flush=True)
+    if a.app == "batch" and off is None:
+        raise SystemExit("batch needs --er-batch K")
+    src = None
+    if a.app == "msbfs":
+        import numpy as np
+        src = np.argsort(-h.degrees, kind="stable")[:64].astype(np.int32)
     for _ in range(a.repeat):
-        if a.app == "bfs":
+        if a.app == "msbfs":
+            g.msbfs64(src, out=False)
+        elif a.app == "batch":
+            g.batch_bfs_cc(off, out=False, profile=not a.no_profile, capture=8)
+        elif a.app == "bfs":
             g.bfs(root, **kw)
         elif a.app == "sssp":
             g.sssp(root, **kw)
