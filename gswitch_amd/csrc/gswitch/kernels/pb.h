@@ -44,6 +44,7 @@ constexpr int PB_BLOCK = 1024;
 constexpr int PB_LDS_BYTES = 64 << 10;  // per phase: messages / accumulators
 constexpr int PB_QUADS = 2;             // aligned 4-slot quads a lane keeps in flight (phase 1)
 constexpr int PB_GQUADS = 4;            // the same in phase 2
+constexpr int PB_XCDS = 8;              // MI355X: workgroups dealt round-robin over 8 XCDs
 
 // four consecutive slots, loaded and stored as one access
 template <typename T> struct alignas(4 * sizeof(T)) quad_t {
@@ -241,6 +242,7 @@ __global__ void __launch_bounds__(PB_BLOCK) k_pb_scatter(Bd body, pb_t pb) {
   __shared__ MSG s_msg[CAP];
   const int32_t *pc = pb.p1 + 4 * (int64_t)blockIdx.x;
   const int c = pc[0], b0 = pc[1], b1 = pc[2];
+  if (b0 >= b1) return;  // padding of the XCD-aware piece order
   const int64_t lo = (int64_t)c * pb.S;
   const int w = (int)MIN((int64_t)pb.S, body.g.nvertexs - lo);
   MSG *vals = (MSG *)pb.vals;
@@ -562,9 +564,28 @@ template <typename G> void pb_ensure(G &g, int S, int D) {
     if (acc > 0)
       p1g[grp[b0]].insert(p1g[grp[b0]].end(), {(int32_t)c, (int32_t)b0, (int32_t)pb.B, 0});
   }
+  // XCD-aware order: workgroup i of a launch runs on XCD i % PB_XCDS, so a
+  // group's pieces are dealt out such that every XCD takes one contiguous
+  // run of the chunk-ordered list. The pieces of one chunk then run side by
+  // side on one XCD and restage the chunk's messages from its L2. Slots
+  // past the end are empty pieces (b0 == b1), which return at once.
+  // GSW_PB_XCD=0 keeps the chunk order.
+  bool deal = true;
+  if (const char *e = std::getenv("GSW_PB_XCD")) deal = std::atoi(e) != 0;
   std::vector<int32_t> p1;
   std::vector<int64_t> g1(1, 0);
   for (int64_t k = 0; k < K; ++k) {
+    const int64_t np = (int64_t)p1g[k].size() / 4;
+    if (deal && np > PB_XCDS) {
+      const int64_t per = (np + PB_XCDS - 1) / PB_XCDS;
+      std::vector<int32_t> o((size_t)(4 * per * PB_XCDS), 0);
+      for (int64_t b = 0; b < per * PB_XCDS; ++b) {
+        const int64_t i = (b % PB_XCDS) * per + b / PB_XCDS;
+        if (i < np)
+          for (int r = 0; r < 4; ++r) o[4 * b + r] = p1g[k][4 * i + r];
+      }
+      p1g[k].swap(o);
+    }
     p1.insert(p1.end(), p1g[k].begin(), p1g[k].end());
     g1.push_back((int64_t)p1.size() / 4);
   }
