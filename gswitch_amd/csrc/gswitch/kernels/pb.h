@@ -541,8 +541,14 @@ template <typename G> void pb_ensure(G &g, int S, int D) {
       grp[b] = (int32_t)k;
     }
   }
-  // chunk-major tile table and phase-1 pieces (runs of bins of <= ecap
-  // edges within one group), listed group by group
+  // chunk-major tile table and phase-1 pieces (runs of bins of <= ecap1
+  // edges within one group; ecap1 = ecap unless GSW_PB_ECAP1 is set),
+  // listed group by group
+  int64_t ecap1 = ecap;
+  if (const char *e = std::getenv("GSW_PB_ECAP1")) {
+    const long long x = std::atoll(e);
+    if (x > 0) ecap1 = x;
+  }
   std::vector<int64_t> cs((size_t)T);
   std::vector<int32_t> cl((size_t)T);
   std::vector<std::vector<int32_t>> p1g((size_t)K);
@@ -553,7 +559,7 @@ template <typename G> void pb_ensure(G &g, int S, int D) {
       const int64_t len = off[t + 1] - off[t];
       cs[c * pb.B + b] = off[t];
       cl[c * pb.B + b] = (int32_t)len;
-      if (acc > 0 && (acc + len > ecap || grp[b] != grp[b0])) {
+      if (acc > 0 && (acc + len > ecap1 || grp[b] != grp[b0])) {
         p1g[grp[b0]].insert(p1g[grp[b0]].end(), {(int32_t)c, (int32_t)b0, (int32_t)b, 0});
         b0 = b;
         acc = 0;
