@@ -1967,6 +1967,12 @@ gsw_status gsw_pagerank(gsw_graph *G, float damping, int32_t iters, float eps,
     maybe_learned(c, o, G, true, true, GSW_APP_PR);
     // with eps = 0 the filter keeps every vertex active: no per-level reads
     c.en.conf.counts_known = eps == 0.f;
+    // PR's filter zeroes the accumulator it consumes (GSW_PB_WA_ZERO=0 reads
+    // it back anyway)
+    {
+      const char *e = std::getenv("GSW_PB_WA_ZERO");
+      c.en.conf.wa_zero = !(e && std::atoi(e) == 0);
+    }
 
     hipStream_t st = gsw::stream();
     H_ERR(hipEventRecord(G->ev0, st));

@@ -92,6 +92,10 @@ struct config_t {
   // level's inspection would do first; such a filter reads no level);
   // filter_done tells that inspection
   bool pb_fuse_filter = false;
+  // the functor's filter leaves every WA at zero, its combine's identity
+  // (PageRank): a gather that applies the filter starts its LDS
+  // accumulators of whole bins from zero instead of reading the WA back
+  bool wa_zero = false;
   bool filter_done = false;
   ins_t force;
   // one line per expand on stdout (the reference's --verbose)

@@ -15,8 +15,9 @@
 //   phase 2  per target bin: the bin's slots are one contiguous range
 //            (bin-major layout); messages and 16-bit target offsets are
 //            streamed and combined with compAtomic into LDS accumulators
-//            that start from the targets' WA, which are then written back
-//            (the workgroup owns its targets). A target with more in-edges
+//            that start from the targets' WA (from zero when the fused
+//            filter keeps WA at zero, conf.wa_zero), which are then written
+//            back (the workgroup owns its targets). A target with more in-edges
 //            than a piece holds is a bin of its own, split over several
 //            workgroups that each reduce their part in registers and add it
 //            with one compAtomic on the target.
@@ -358,7 +359,11 @@ __global__ void __launch_bounds__(PB_BLOCK) k_pb_gather(Bd body, pb_t pb) {
       }
     };
     if (nq > 0) load(0, mv, di);
-    for (int i = threadIdx.x; i < w; i += PB_BLOCK) s_acc[i] = *body.f.wa_of(lo + i);
+    if (body.conf.pb_fuse_filter && body.conf.wa_zero) {
+      for (int i = threadIdx.x; i < w; i += PB_BLOCK) s_acc[i] = WA{};
+    } else {
+      for (int i = threadIdx.x; i < w; i += PB_BLOCK) s_acc[i] = *body.f.wa_of(lo + i);
+    }
     __syncthreads();
     if (t < a0 - e0) body.f.compAtomic(&s_acc[pb.dst[e0 + t]], (WA)vals[e0 + t], body.g);
     if (t < e1 - a1) body.f.compAtomic(&s_acc[pb.dst[a1 + t]], (WA)vals[a1 + t], body.g);
