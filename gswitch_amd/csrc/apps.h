@@ -97,12 +97,18 @@ struct PR : Functor<VC, double, float, Empty> {
   float eps;
   __device__ Status filter(int v, GU g) {
     double *cache = wa_of(v);
+    const double acc = *cache;
+    *cache = 0;
+    return filter_acc(v, acc, g);
+  }
+  // the filter given the accumulated value, with WA already at zero: a
+  // propagation-blocked gather passes its LDS accumulator and leaves WA alone
+  __device__ Status filter_acc(int v, double acc, GU g) {
     float *pr = ra_of(v);
     const int od = g.get_out_degree(v);
-    double np = *cache * (double)damp + (1.0 - (double)damp);
+    double np = acc * (double)damp + (1.0 - (double)damp);
     if (od) np /= od;
     const float err = (float)((np - (double)*pr) / (double)*pr);
-    *cache = 0;
     *pr = (float)np;
     if (err > -eps && err < eps) return Inactive;
     return Active;

@@ -59,6 +59,12 @@ template <typename F> struct pb_msg<F, std::void_t<typename F::msg_t>> {
   using type = typename F::msg_t;
 };
 
+// a functor with filter_acc(v, acc, g): its filter given the accumulated
+// value, for a WA kept at zero (the fused gather of conf.wa_zero)
+template <typename F, typename = void> struct pb_has_filter_acc : std::false_type {};
+template <typename F>
+struct pb_has_filter_acc<F, std::void_t<decltype(&F::filter_acc)>> : std::true_type {};
+
 // bin of target v: the last b with bin_lo[b] <= v
 __device__ __forceinline__ int64_t pb_bin_of(const int32_t *bin_lo, int64_t B,
                                              int64_t v) {
@@ -396,11 +402,24 @@ __global__ void __launch_bounds__(PB_BLOCK) k_pb_gather(Bd body, pb_t pb) {
       }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < w; i += PB_BLOCK) {
-      const int v = lo + i;
-      if (body.as.fetch(v, want) >= 0) *body.f.wa_of(v) = s_acc[i];
-      if (body.conf.pb_fuse_filter) (void)body.f.filter(v, body.g);
+    bool applied = false;
+    if constexpr (pb_has_filter_acc<decltype(body.f)>::value) {
+      if (body.conf.pb_fuse_filter && body.conf.wa_zero) {
+        // WA stays at zero: the filter takes the accumulator from LDS
+        for (int i = threadIdx.x; i < w; i += PB_BLOCK) {
+          const int v = lo + i;
+          const WA a = body.as.fetch(v, want) >= 0 ? s_acc[i] : *body.f.wa_of(v);
+          (void)body.f.filter_acc(v, a, body.g);
+        }
+        applied = true;
+      }
     }
+    if (!applied)
+      for (int i = threadIdx.x; i < w; i += PB_BLOCK) {
+        const int v = lo + i;
+        if (body.as.fetch(v, want) >= 0) *body.f.wa_of(v) = s_acc[i];
+        if (body.conf.pb_fuse_filter) (void)body.f.filter(v, body.g);
+      }
   } else {
     // one target: reduce the part in registers, then across the workgroup
     WA acc{};
